@@ -1,0 +1,121 @@
+/*
+ * TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of the umapers 1.1.4 UMAP fit / transform path
+ * (rust_umap/src/lib.rs).  It is the correctness oracle for the HIP library
+ * and the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product never links it.
+ *
+ * Every function cites the reference file:line it follows.  Parity status of
+ * each part is listed in the header comment of umap_oracle.cpp.
+ */
+#ifndef UMAP_ORACLE_H
+#define UMAP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors UmapParams (rust_umap/src/lib.rs:128-195) for the dense path. */
+typedef struct orc_params {
+  int32_t n_neighbors;
+  int32_t n_components;
+  int32_t n_epochs;            /* <= 0: default (500 if n <= 10000 else 200) */
+  int32_t metric;              /* 0 euclidean, 1 cosine */
+  float learning_rate;
+  float min_dist;
+  float spread;
+  float local_connectivity;
+  float set_op_mix_ratio;
+  float repulsion_strength;
+  int32_t negative_sample_rate;
+  int32_t init;                /* 0 random, 1 spectral */
+  uint64_t random_seed;
+  int32_t densmap;
+  float dens_lambda;
+  float dens_frac;
+  float dens_var_shift;
+  int32_t output_dens;
+  int32_t n_threads;           /* threads for the stages rayon runs in parallel */
+} orc_params;
+
+/* Same keys as FitTimingBreakdown (rust_umap/src/lib.rs:51-68). */
+typedef struct orc_timings {
+  double validation_sec, knn_sec, curve_params_sec, knn_validate_trim_sec,
+      smooth_knn_sec, density_original_sec, membership_sec, symmetrize_sec,
+      target_intersection_sec, prune_sec, init_sec, optimize_sec,
+      density_embedding_sec, output_copy_sec, store_state_sec, total_sec;
+} orc_timings;
+
+void orc_find_ab_params(float spread, float min_dist, float* a, float* b);
+
+void orc_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
+             int32_t n_threads, int64_t* idx, float* dist);
+void orc_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,
+                          int64_t d, int32_t k, int32_t metric,
+                          int32_t n_threads, int64_t* idx, float* dist);
+/* Single-pair distances with the reference's arithmetic. */
+float orc_sq_euclidean(const float* x, const float* y, int64_t d);
+float orc_cosine_distance(const float* x, const float* y, int64_t d);
+
+int32_t orc_knn_rows_include_self(const int64_t* idx, const float* dist,
+                                  int64_t n, int32_t k);
+void orc_smooth_knn_dist(const float* dist, int64_t n, int32_t k, float k_value,
+                         float local_connectivity, float bandwidth,
+                         int32_t include_self, float* sigmas, float* rhos);
+/* Membership + fuzzy union (+ prune when n_epochs > 0) into head-sorted
+ * edges.  Returns the edge count, or -1 when cap is too small. */
+int64_t orc_fuzzy_graph(const int64_t* idx, const float* dist,
+                        const float* sigmas, const float* rhos, int64_t n,
+                        int32_t k, float set_op_mix_ratio, int32_t n_epochs,
+                        int64_t cap, int64_t* heads, int64_t* tails, float* w);
+int64_t orc_connected_components(int64_t n, const int64_t* heads,
+                                 const int64_t* tails, int64_t n_edges,
+                                 int64_t* labels);
+/* Returns 1 on spectral success, 0 when it fell back to random init. */
+int32_t orc_initialize_embedding(const float* x, int64_t n, int64_t d,
+                                 int32_t dim, const int64_t* heads,
+                                 const int64_t* tails, const float* w,
+                                 int64_t n_edges, int32_t init, uint64_t seed,
+                                 float* y);
+void orc_make_epochs_per_sample(const float* w, int64_t n_edges,
+                                int32_t n_epochs, float* eps);
+/* Exact per-edge schedule: positive and negative sample counts over all
+ * epochs, plus per-epoch totals of active edges and negatives. */
+void orc_schedule_counts(const float* w, int64_t n_edges, int32_t n_epochs,
+                         int32_t negative_sample_rate, int32_t* pos_counts,
+                         int32_t* neg_counts, int64_t* active_per_epoch,
+                         int64_t* neg_per_epoch);
+void orc_optimize_layout(float* y, int64_t n, int32_t dim,
+                         const int64_t* heads, const int64_t* tails,
+                         const float* w, int64_t n_edges, int32_t n_epochs,
+                         float a, float b, float learning_rate,
+                         int32_t negative_sample_rate, float repulsion_strength,
+                         uint64_t seed, const float* dens_radii,
+                         int32_t dens_start_epoch, float dens_lambda,
+                         float dens_var_shift);
+int32_t orc_density_start_epoch(int32_t n_epochs, float dens_frac);
+void orc_local_radii(const int64_t* idx, const float* dist, int64_t n, int32_t k,
+                     float var_shift, float* out);
+void orc_embedding_radii(const float* y, int64_t n, int32_t dim,
+                         const int64_t* idx, int32_t k, float var_shift,
+                         float* out);
+
+/* Whole dense fit (lib.rs:1367-1464 via profile_fit_transform_dense :796-962).
+ * knn_idx/knn_dist may be NULL (exact kNN is then computed); otherwise they
+ * hold [n x knn_cols] precomputed neighbours.  Returns 0 on success. */
+int32_t orc_fit(const orc_params* p, const float* x, int64_t n, int64_t d,
+                const int64_t* knn_idx, const float* knn_dist, int64_t knn_cols,
+                float* y_out, float* sigmas, float* rhos, float* rad_orig,
+                float* rad_emb, orc_timings* t);
+/* Plain transform (lib.rs:1604-1685, 1738-1780). */
+int32_t orc_transform(const orc_params* p, const float* x_train, int64_t n,
+                      int64_t d, const float* y_train, float a, float b,
+                      const float* q, int64_t m, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

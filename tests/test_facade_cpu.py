@@ -1,0 +1,113 @@
+"""Facade logic that needs no GPU: argument normalisation, errors, out=
+buffers and fitted attributes, with a fake core standing in for the GPU (the
+reference's binding tests use the same pattern, test_binding.py:214-248)."""
+import numpy as np
+import pytest
+
+import umapers_amd._api as api
+from umapers_amd import Umap
+
+
+class FakeCore:
+    """Routes the facade to the CPU oracle (test-only)."""
+
+    def __init__(self, params):
+        from oracle import oracle as O
+
+        self.O = O
+        self.p = params
+        self.n_components = params.n_components
+        self.emb = None
+        self.calls = []
+
+    def _op(self):
+        return self.O.OracleParams(n_neighbors=self.p.n_neighbors, n_components=self.p.n_components,
+                                   n_epochs=self.p.n_epochs or None, init="random")
+
+    def fit(self, arr):
+        self.calls.append("fit")
+        self.emb = self.O.fit(arr, self._op()).embedding
+
+    def fit_transform_into(self, arr, out):
+        self.calls.append("fit_transform")
+        out[...] = self.O.fit(arr, self._op()).embedding
+        self.emb = out.copy()
+
+    def fit_transform(self, arr):
+        out = np.empty((arr.shape[0], self.n_components), np.float32)
+        self.fit_transform_into(arr, out)
+        return out
+
+    @property
+    def embedding(self):
+        return self.emb
+
+    def radii(self):
+        return None
+
+
+@pytest.fixture
+def fake(monkeypatch):
+    monkeypatch.setattr(api, "_Core", FakeCore)
+
+
+def test_defaults_and_get_set_params(fake):
+    m = Umap()
+    p = m.get_params()
+    assert p["n_neighbors"] == 15 and p["n_components"] == 2 and p["init"] == "spectral"
+    assert p["ann_mode"] == "auto" and p["knn_precision"] == "fp32"
+    m.set_params(n_neighbors=7, ann_mode="exact")
+    assert m.n_neighbors == 7 and m.use_approximate_knn is False
+    with pytest.raises(ValueError, match="Invalid parameter 'nope'"):
+        m.set_params(nope=1)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    ({"metric": "manhattan"}, "not available on the GPU path"),
+    ({"metric": "bogus"}, "unsupported metric 'bogus'"),
+    ({"ann_mode": "fast"}, "unsupported ann_mode 'fast'"),
+    ({"dens_lambda": -1.0}, "dens_lambda must be finite and >= 0"),
+    ({"init": "pca"}, "unsupported init 'pca'"),
+    ({"knn_precision": "fp16"}, "unsupported knn_precision"),
+    ({"transform_mode": "magic"}, "unsupported transform_mode"),
+    ({"target_metric": "categorical"}, "not available on the GPU path"),
+])
+def test_constructor_errors(fake, kw, msg):
+    with pytest.raises(ValueError, match=msg):
+        Umap(**kw)
+
+
+def test_metric_aliases(fake):
+    assert Umap(metric="L2").metric == "euclidean"
+    assert Umap(metric="cosine").metric == "cosine"
+
+
+def test_input_normalisation_and_out_buffer(fake):
+    x = np.random.default_rng(0).normal(size=(60, 5))  # float64, converted
+    m = Umap(n_neighbors=5, n_epochs=20)
+    y = m.fit_transform(x.tolist())
+    assert y.shape == (60, 2) and y.dtype == np.float32
+    assert m.n_features_in_ == 5 and m.n_samples_fit_ == 60
+    with pytest.raises(ValueError, match="must be a 2D array"):
+        m.fit_transform(np.zeros(5))
+    with pytest.raises(TypeError, match="out dtype must be float32"):
+        m.fit_transform(x, out=np.empty((60, 2)))
+    with pytest.raises(ValueError, match="C-contiguous"):
+        m.fit_transform(x, out=np.empty((2, 60), np.float32).T)
+    out = np.empty((60, 2), np.float32)
+    assert m.fit_transform(x, out=out) is out
+
+
+def test_sparse_input_rejected(fake):
+    sp = pytest.importorskip("scipy.sparse")
+    with pytest.raises(ValueError, match="sparse input"):
+        Umap().fit(sp.csr_matrix(np.eye(20)))
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    from umapers_amd import _lib as L
+
+    monkeypatch.setattr(L, "_lib", None)
+    monkeypatch.setattr(L, "LIB_PATH", "/nonexistent/libumapx.so")
+    with pytest.raises(L.UmapxNotBuilt):
+        Umap()

@@ -1,0 +1,101 @@
+"""smooth_knn_dist, membership, fuzzy union, prune and radii on the GPU vs the
+oracle."""
+import numpy as np
+import pytest
+
+from umapers_amd import datasets
+
+from . import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def _knn(oracle, x, k=15):
+    return oracle.knn(x, k)
+
+
+@pytest.fixture(scope="module")
+def mixture():
+    return datasets.gaussian_mixture(4000, 64, 6, 3.0, 21, (0.5, 1.5))
+
+
+def test_include_self_predicate(gpu, oracle, mixture):
+    idx, dist = _knn(oracle, mixture)
+    assert capi.include_self(idx, dist) == oracle.rows_include_self(idx, dist) is True
+    idx2 = idx.copy()
+    idx2[5, 0], idx2[5, 1] = idx[5, 1], idx[5, 0]
+    assert capi.include_self(idx2, dist) is False
+
+
+def test_smooth_knn_f32_mode_matches_oracle(gpu, oracle, mixture):
+    idx, dist = _knn(oracle, mixture)
+    s_o, r_o = oracle.smooth_knn_dist(dist, 15.0, 1.0, include_self=True)
+    s_g, r_g = capi.smooth(dist, 15.0, 1.0, True, fp64=False)
+    assert np.array_equal(r_g, r_o)
+    # exp differs by at most an ulp between libm and the GPU
+    rel = np.abs(s_g - s_o) / s_o
+    assert np.mean(rel == 0) > 0.95 and rel.max() < 1e-5
+
+
+def test_smooth_knn_fp64_sums_within_tolerance(gpu, oracle, mixture):
+    idx, dist = _knn(oracle, mixture)
+    s_o, r_o = oracle.smooth_knn_dist(dist, 15.0, 1.0, include_self=True)
+    s_g, r_g = capi.smooth(dist, 15.0, 1.0, True, fp64=True)
+    assert np.array_equal(r_g, r_o)
+    # both satisfy |psum - log2 k| < 1e-5; sigmas agree to ~1e-5 relative
+    rel = np.abs(s_g - s_o) / s_o
+    assert np.median(rel) < 1e-6 and rel.max() < 2e-4
+    act = dist[:, 1:].astype(np.float64)
+    psum = np.where(act - r_g[:, None] > 0, np.exp(-(act - r_g[:, None]) / s_g[:, None]), 1.0).sum(1)
+    assert np.all(np.abs(psum - np.log2(15.0)) < 2e-5)
+
+
+def test_smooth_knn_local_connectivity_variants(gpu, oracle, rng):
+    d = np.sort(rng.uniform(0.0, 2.0, size=(300, 10)).astype(np.float32), axis=1)
+    d[::7, :4] = 0.0  # rows with several zero distances
+    for lc in (0.0, 0.5, 1.0, 1.5, 3.0):
+        s_o, r_o = oracle.smooth_knn_dist(d, 10.0, lc, include_self=False)
+        s_g, r_g = capi.smooth(d, 10.0, lc, False, fp64=False)
+        assert np.array_equal(r_g, r_o), lc
+        assert np.allclose(s_g, s_o, rtol=1e-5, atol=0), lc
+
+
+def test_fuzzy_graph_matches_oracle(gpu, oracle, mixture):
+    idx, dist = _knn(oracle, mixture)
+    sig, rho = oracle.smooth_knn_dist(dist, 15.0, 1.0, include_self=True)
+    for prune in (0, 200, 5):
+        h_o, t_o, w_o = oracle.fuzzy_graph(idx, dist, sig, rho, 1.0, prune)
+        h_g, t_g, w_g = capi.fuzzy_graph(idx, dist, sig, rho, 1.0, prune)
+        # identical edge list in identical (head, tail) order
+        assert np.array_equal(h_g, h_o) and np.array_equal(t_g, t_o)
+        assert np.all(np.abs(w_g - w_o) <= 1e-5 * w_o)
+        # symmetric weights, head-sorted
+        key = dict(zip(zip(h_g.tolist(), t_g.tolist()), w_g.tolist()))
+        assert all(key[(b, a)] == v for (a, b), v in key.items())
+
+
+@pytest.mark.parametrize("mix", [0.0, 0.3, 1.0])
+def test_fuzzy_union_mix_ratio(gpu, oracle, mixture, mix):
+    idx, dist = _knn(oracle, mixture[:1500])
+    sig, rho = oracle.smooth_knn_dist(dist, 15.0, 1.0, include_self=True)
+    h_o, t_o, w_o = oracle.fuzzy_graph(idx, dist, sig, rho, mix, 0)
+    h_g, t_g, w_g = capi.fuzzy_graph(idx, dist, sig, rho, mix, 0)
+    assert np.array_equal(h_g, h_o) and np.array_equal(t_g, t_o)
+    assert np.all(np.abs(w_g - w_o) <= 1e-5 * w_o + 1e-7)
+
+
+def test_radii_match_oracle(gpu, oracle, mixture):
+    idx, dist = _knn(oracle, mixture)
+    assert np.allclose(capi.local_radii(idx, dist), oracle.local_radii(idx, dist), rtol=1e-6, atol=1e-6)
+    y = np.random.default_rng(2).normal(size=(mixture.shape[0], 2)).astype(np.float32)
+    assert np.allclose(capi.embedding_radii(y, idx), oracle.embedding_radii(y, idx), rtol=1e-6,
+                       atol=1e-6)
+
+
+def test_components_match_oracle(gpu, oracle, mixture):
+    idx, dist = _knn(oracle, mixture)
+    sig, rho = oracle.smooth_knn_dist(dist, 15.0)
+    h, t, w = oracle.fuzzy_graph(idx, dist, sig, rho, 1.0, 200)
+    nc_o, lab_o = oracle.connected_components(mixture.shape[0], h, t)
+    nc_g, lab_g = capi.components(mixture.shape[0], h, t, w)
+    assert nc_g == nc_o and np.array_equal(lab_g, lab_o)

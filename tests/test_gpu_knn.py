@@ -1,0 +1,91 @@
+"""Exact kNN on the GPU against the oracle (bit-exact) and FP64 (tie-tolerant)."""
+import numpy as np
+import pytest
+
+from oracle.knn_fp64 import knn_fp64
+from umapers_amd import datasets
+from umapers_amd.diagnostics import recall_at_k
+
+from . import capi
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(300, 8, 5), (500, 32, 15), (777, 37, 15), (1000, 64, 15), (900, 100, 15),
+          (1200, 128, 15), (1500, 192, 15), (600, 256, 15), (800, 64, 30)]
+
+
+@pytest.mark.parametrize("n,d,k", SHAPES)
+@pytest.mark.parametrize("metric", [0, 1])
+def test_fp32_knn_bit_exact_vs_oracle(gpu, oracle, rng, n, d, k, metric):
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    idx, dist = capi.knn(x, k, metric, 0)
+    oi, od = oracle.knn(x, k, "cosine" if metric else "euclidean")
+    assert np.array_equal(idx, oi)
+    assert np.array_equal(dist.view(np.int32), od.view(np.int32))
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_bf16_rerank_knn_recall_and_exact_distances(gpu, oracle, metric):
+    x = datasets.gaussian_mixture(3000, 128, 8, 3.0, 5, (0.5, 1.5))
+    idx, dist = capi.knn(x, 15, metric, 1, 2)
+    oi, od = oracle.knn(x, 15, "cosine" if metric else "euclidean")
+    assert recall_at_k(idx, oi) >= 0.99
+    # every returned distance is the exact f32 distance of that pair
+    same = idx == oi
+    assert np.array_equal(dist[same].view(np.int32), od[same].view(np.int32))
+
+
+def test_fp32_knn_vs_fp64_ties_only(gpu):
+    x = datasets.gaussian_mixture(2000, 128, 4, 3.0, 9, 1.0)
+    idx, dist = capi.knn(x, 15)
+    i64, d64 = knn_fp64(x, None, 15)
+    bad = 0
+    x64 = x.astype(np.float64)
+    for r in np.where(np.any(idx != i64, axis=1))[0]:
+        for j in set(idx[r]) ^ set(i64[r]):
+            dj = np.sqrt(((x64[r] - x64[j]) ** 2).sum())
+            if abs(dj - d64[r, -1]) > 1e-6 * d64[r, -1]:
+                bad += 1
+    assert bad == 0
+    assert np.all(np.abs(dist - d64) <= 1e-5 * d64 + 1e-6)
+
+
+def test_knn_to_reference_matches_oracle(gpu, oracle, rng):
+    r = rng.normal(size=(1500, 96)).astype(np.float32)
+    q = rng.normal(size=(333, 96)).astype(np.float32)
+    for metric in (0, 1):
+        idx, dist = capi.knn_to_reference(q, r, 15, metric)
+        oi, od = oracle.knn_to_reference(q, r, 15, "cosine" if metric else "euclidean")
+        assert np.array_equal(idx, oi)
+        assert np.array_equal(dist, od)
+
+
+def test_duplicate_rows_and_ties(gpu, oracle):
+    base = np.random.default_rng(4).normal(size=(50, 16)).astype(np.float32)
+    x = np.concatenate([base, base, base[:10]], axis=0)  # exact duplicates
+    idx, dist = capi.knn(x, 15)
+    oi, od = oracle.knn(x, 15)
+    assert np.array_equal(idx, oi) and np.array_equal(dist, od)
+    # a duplicated row's lower-index twin comes first: include_self is False
+    assert not capi.include_self(idx, dist)
+    # tie-break KAT (sparse.rs:1070-1122)
+    ref = np.array([[1.0, 0.0], [-1.0, 0.0], [0.0, 1.0], [0.0, -1.0]], np.float32)
+    qi, qd = capi.knn_to_reference(np.zeros((1, 2), np.float32), ref, 3)
+    assert qi[0].tolist() == [0, 1, 2] and np.all(qd == 1.0)
+
+
+def test_smallest_sizes(gpu, oracle):
+    x = np.random.default_rng(5).normal(size=(16, 3)).astype(np.float32)
+    idx, dist = capi.knn(x, 15)
+    oi, od = oracle.knn(x, 15)
+    assert np.array_equal(idx, oi) and np.array_equal(dist, od)
+
+
+def test_cosine_zero_rows(gpu, oracle, rng):
+    x = rng.normal(size=(200, 40)).astype(np.float32)
+    x[[3, 17]] = 0.0
+    idx, dist = capi.knn(x, 10, 1)
+    oi, od = oracle.knn(x, 10, "cosine")
+    # zero rows: distance 1 to every non-zero row, 0 to each other
+    assert np.array_equal(dist, od)
+    assert recall_at_k(idx, oi) > 0.99

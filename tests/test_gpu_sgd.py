@@ -1,0 +1,77 @@
+"""SGD epochs: exact schedule counts, closed-form steps, statistical parity."""
+import numpy as np
+import pytest
+
+from umapers_amd import datasets
+from umapers_amd.diagnostics import knn_preservation, trustworthiness
+
+from . import capi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def graph(oracle):
+    x = datasets.gaussian_mixture(3000, 32, 5, 3.0, 8, 1.0)
+    idx, dist = oracle.knn(x, 15)
+    sig, rho = oracle.smooth_knn_dist(dist, 15.0)
+    h, t, w = oracle.fuzzy_graph(idx, dist, sig, rho, 1.0, 200)
+    return x, h, t, w
+
+
+def test_schedule_counts_exact(gpu, oracle, graph):
+    x, h, t, w = graph
+    y0 = np.random.default_rng(0).uniform(-10, 10, size=(x.shape[0], 2)).astype(np.float32)
+    for epochs, nsr in ((200, 5), (37, 3), (500, 5)):
+        _, pc, nc = capi.optimize(y0, h, t, w, epochs, 1.58, 0.9, nsr=nsr, counts=True)
+        po, no, _, _ = oracle.schedule_counts(w, epochs, nsr)
+        assert np.array_equal(pc, po) and np.array_equal(nc, no)
+
+
+def test_single_edge_closed_form_attraction(gpu):
+    # two points, one edge each way, no negatives (nsr = 0 -> eps_neg = inf):
+    # epoch 1 is the only firing epoch with T = 2 and both edges at max weight
+    y0 = np.array([[0.0, 0.0], [3.0, 4.0]], np.float32)
+    h = np.array([0], np.int64)
+    t = np.array([1], np.int64)
+    w = np.array([1.0], np.float32)
+    a, b = 1.577, 0.895
+    y = capi.optimize(y0, h, t, w, 2, a, b, lr=1.0, nsr=0)
+    d2 = 25.0
+    pb = d2 ** b
+    gc = -2 * a * b * (pb / d2) / (a * pb + 1)
+    alpha = 1.0 * (1 - 1 / 2)
+    g = np.clip(gc * (y0[0] - y0[1]), -4, 4) * alpha
+    assert np.allclose(y[0], y0[0] + g, rtol=1e-5, atol=1e-6)
+    assert np.allclose(y[1], y0[1] - g, rtol=1e-5, atol=1e-6)
+
+
+def test_sgd_statistical_parity(gpu, oracle, graph):
+    x, h, t, w = graph
+    a, b = oracle.find_ab_params(1.0, 0.1)
+    y0, _ = oracle.initialize_embedding(x, 2, h, t, w, "spectral", 42)
+    yo = oracle.optimize_layout(y0, h, t, w, 200, a, b)
+    yg = capi.optimize(y0, h, t, w, 200, a, b)
+    assert np.isfinite(yg).all()
+    to, tg = trustworthiness(x, yo), trustworthiness(x, yg)
+    ko, kg = knn_preservation(x, yo), knn_preservation(x, yg)
+    assert tg >= to - 0.01, (tg, to)
+    assert kg >= ko - 0.01, (kg, ko)
+
+
+def test_densmap_term_changes_local_scale(gpu, oracle, graph):
+    x, h, t, w = graph
+    a, b = oracle.find_ab_params(1.0, 0.1)
+    idx, dist = oracle.knn(x, 15)
+    rad = oracle.local_radii(idx, dist)
+    y0, _ = oracle.initialize_embedding(x, 2, h, t, w, "spectral", 42)
+    start = oracle.density_start_epoch(200, 0.3)
+    yo = oracle.optimize_layout(y0, h, t, w, 200, a, b, radii=rad, dens_start=start)
+    yg = capi.optimize(y0, h, t, w, 200, a, b, radii=rad, dens_start=start)
+    assert np.isfinite(yg).all()
+    # the density term ties embedding radii to the original ones: compare the
+    # correlation between original and embedding radii with the oracle's
+    ro = np.corrcoef(rad, oracle.embedding_radii(yo, idx))[0, 1]
+    rg = np.corrcoef(rad, oracle.embedding_radii(yg, idx))[0, 1]
+    assert rg >= ro - 0.05, (rg, ro)
+    assert trustworthiness(x, yg) >= trustworthiness(x, yo) - 0.01

@@ -1,0 +1,489 @@
+"""Python facade with the umapers API (umap_rs/python/umapers/_api.py).
+
+``Umap`` keeps the reference's constructor keywords, defaults and methods
+(``fit``, ``fit_transform``, ``fit_transform_with_knn``, ``transform``,
+``profile_fit_transform``, ``get_params`` / ``set_params``) and its fitted
+attributes.  Underneath, ``_Core`` plays the part of the PyO3 ``UmapCore``
+(umap_rs/src/lib.rs:540-1320): it owns a ``umapx_handle`` and calls the C ABI,
+which runs the fit on the GPU.  There is no CPU fallback: a missing library
+or GPU raises.
+
+GPU-path keywords that the reference does not have:
+  knn_precision   "fp32" (default) or "bf16_rerank" (BF16 MFMA candidates,
+                  exact f32 re-rank of rerank_factor * k candidates)
+  rerank_factor   candidates per row over k (default 2)
+  transform_mode  "plain" (kNN + SGD, the reference's transform) or "learned"
+                  (MLP encoder trained on the fitted graph)
+  smooth_fp64     FP64 sums in the sigma bisection (default True)
+  mlp_*           learned-transform training knobs
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Any
+
+import numpy as np
+
+from . import _lib as L
+
+_PARAM_NAMES = (
+    "n_neighbors", "n_components", "n_epochs", "metric", "metric_kwds", "learning_rate",
+    "min_dist", "spread", "local_connectivity", "set_op_mix_ratio", "repulsion_strength",
+    "negative_sample_rate", "random_seed", "init", "ann_mode", "use_approximate_knn",
+    "approx_knn_candidates", "approx_knn_iters", "approx_knn_threshold", "densmap",
+    "dens_lambda", "dens_frac", "dens_var_shift", "output_dens", "target_metric",
+    "target_weight", "target_n_neighbors",
+    # GPU path
+    "knn_precision", "rerank_factor", "transform_mode", "smooth_fp64", "mlp_hidden",
+    "mlp_steps", "mlp_batch", "mlp_negatives", "mlp_learning_rate", "mlp_embedding_weight",
+)
+
+_GPU_METRICS = {"euclidean": L.METRIC_EUCLIDEAN, "cosine": L.METRIC_COSINE}
+
+
+def _as_f32_matrix(x: Any, name: str) -> np.ndarray:
+    """_api.py:113-117: C-contiguous float32, 2-D."""
+    arr = np.asarray(x, dtype=np.float32, order="C")
+    if arr.ndim != 2:
+        raise ValueError(f"{name} must be a 2D array, got ndim={arr.ndim}")
+    return arr
+
+
+def _as_knn_indices(x: Any, name: str) -> np.ndarray:
+    """_api.py:120-124: C-contiguous int64, 2-D."""
+    arr = np.asarray(x, dtype=np.int64, order="C")
+    if arr.ndim != 2:
+        raise ValueError(f"{name} must be a 2D array, got ndim={arr.ndim}")
+    return arr
+
+
+def _is_sparse(x: Any) -> bool:
+    return getattr(x, "format", None) in {"csr", "csc", "coo"}
+
+
+def _as_out_buffer(out: Any, shape: tuple[int, int]) -> np.ndarray:
+    """_api.py:193-206."""
+    if not isinstance(out, np.ndarray):
+        raise TypeError("out must be a NumPy ndarray")
+    if out.dtype != np.float32:
+        raise TypeError(f"out dtype must be float32, got {out.dtype}")
+    if out.ndim != 2:
+        raise ValueError(f"out must be 2D, got ndim={out.ndim}")
+    if not out.flags.c_contiguous:
+        raise ValueError("out must be C-contiguous")
+    if not out.flags.writeable:
+        raise ValueError("out must be writeable")
+    if tuple(out.shape) != tuple(shape):
+        raise ValueError(f"output buffer shape mismatch: expected {shape}, got {tuple(out.shape)}")
+    return out
+
+
+def _normalize_ann_mode(ann_mode: Any, use_approximate_knn: bool,
+                        approx_knn_threshold: int) -> tuple[str, bool, int]:
+    """_api.py:209-224."""
+    mode = ann_mode if isinstance(ann_mode, str) else str(ann_mode)
+    mode = mode.lower()
+    if mode == "auto":
+        return mode, use_approximate_knn, approx_knn_threshold
+    if mode == "exact":
+        return mode, False, approx_knn_threshold
+    if mode == "approximate":
+        return mode, True, 0
+    raise ValueError(f"unsupported ann_mode '{ann_mode}', expected auto|exact|approximate")
+
+
+_METRIC_ALIASES = {
+    "euclidean": "euclidean", "l2": "euclidean", "manhattan": "manhattan", "l1": "manhattan",
+    "cosine": "cosine", "chebyshev": "chebyshev", "linfinity": "chebyshev",
+    "linf": "chebyshev", "minkowski": "minkowski", "correlation": "correlation",
+    "canberra": "canberra", "braycurtis": "braycurtis", "bray_curtis": "braycurtis",
+}
+
+
+def _normalize_metric_name(metric: Any) -> str:
+    """_api.py:242-263, then the GPU path's own restriction."""
+    name = str(metric).lower()
+    if name not in _METRIC_ALIASES:
+        expected = "|".join(sorted(set(_METRIC_ALIASES.values())))
+        raise ValueError(f"unsupported metric '{metric}', expected {expected}")
+    canonical = _METRIC_ALIASES[name]
+    if canonical not in _GPU_METRICS:
+        raise ValueError(
+            f"metric '{canonical}' is not available on the GPU path, expected euclidean|cosine")
+    return canonical
+
+
+def _validate_dens_params(dens_lambda: float, dens_frac: float,
+                          dens_var_shift: float) -> tuple[float, float, float]:
+    """_api.py:302-316."""
+    dens_lambda, dens_frac, dens_var_shift = float(dens_lambda), float(dens_frac), float(dens_var_shift)
+    if not np.isfinite(dens_lambda) or dens_lambda < 0.0:
+        raise ValueError("dens_lambda must be finite and >= 0")
+    if not np.isfinite(dens_frac) or dens_frac < 0.0 or dens_frac > 1.0:
+        raise ValueError("dens_frac must be finite and in [0, 1]")
+    if not np.isfinite(dens_var_shift) or dens_var_shift <= 0.0:
+        raise ValueError("dens_var_shift must be finite and > 0")
+    return dens_lambda, dens_frac, dens_var_shift
+
+
+class _Core:
+    """Owner of one umapx_handle; the UmapCore of this package."""
+
+    def __init__(self, params: L.Params):
+        self._lib = L.lib()
+        self._params = params
+        self._h = ctypes.c_void_p()
+        L.check(self._lib.umapx_create(ctypes.byref(params), ctypes.byref(self._h)))
+        self.n_components = params.n_components
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.umapx_destroy(h)
+            except Exception:  # pragma: no cover - interpreter shutdown
+                pass
+            self._h = None
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def fit(self, arr: np.ndarray) -> None:
+        L.check(self._lib.umapx_fit(self._h, L.ptr(arr), arr.shape[0], arr.shape[1]))
+
+    def fit_transform_into(self, arr: np.ndarray, out: np.ndarray) -> None:
+        L.check(self._lib.umapx_fit_transform(self._h, L.ptr(arr), arr.shape[0], arr.shape[1],
+                                              L.ptr(out)))
+
+    def fit_transform(self, arr: np.ndarray) -> np.ndarray:
+        out = np.empty((arr.shape[0], self.n_components), np.float32)
+        self.fit_transform_into(arr, out)
+        return out
+
+    def fit_transform_with_knn_into(self, arr, idx, dist, out, knn_metric, validate) -> None:
+        metric = _GPU_METRICS.get(_METRIC_ALIASES.get(str(knn_metric).lower(), ""), -1)
+        if metric < 0:
+            raise ValueError(f"unsupported knn_metric '{knn_metric}'")
+        L.check(self._lib.umapx_fit_transform_with_knn(
+            self._h, L.ptr(arr), arr.shape[0], arr.shape[1], L.ptr(idx, L._i64p), idx.shape[0],
+            idx.shape[1], L.ptr(dist), dist.shape[0], dist.shape[1], metric, int(bool(validate)),
+            L.ptr(out)))
+
+    def profile_fit_transform(self, arr: np.ndarray) -> dict[str, Any]:
+        out = np.empty((arr.shape[0], self.n_components), np.float32)
+        t = L.Timings()
+        L.check(self._lib.umapx_profile_fit_transform(self._h, L.ptr(arr), arr.shape[0],
+                                                      arr.shape[1], L.ptr(out), ctypes.byref(t)))
+        td = t.as_dict()
+        return {
+            "embedding": out,
+            "timings": {k: td[k] for k in L.TIMING_KEYS},
+            "used_approximate_knn": bool(td["used_approximate_knn"]),
+            "n_samples": int(arr.shape[0]),
+            "n_features": int(arr.shape[1]),
+            "n_epochs": int(td["n_epochs"]),
+            "n_edges": int(td["n_edges"]),
+            "gpu": {k: td[k] for k in ("knn_kernel_ms", "knn_flops", "sgd_kernel_ms",
+                                       "init_used_spectral", "n_graph_components")},
+        }
+
+    def timings(self) -> dict[str, Any]:
+        t = L.Timings()
+        L.check(self._lib.umapx_get_timings(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def transform_into(self, arr: np.ndarray, out: np.ndarray, mode: str | None = None) -> None:
+        fn = {None: self._lib.umapx_transform, "plain": self._lib.umapx_transform_plain,
+              "learned": self._lib.umapx_transform_learned}[mode]
+        L.check(fn(self._h, L.ptr(arr), arr.shape[0], arr.shape[1], L.ptr(out)))
+
+    def transform(self, arr: np.ndarray, mode: str | None = None) -> np.ndarray:
+        out = np.empty((arr.shape[0], self.n_components), np.float32)
+        self.transform_into(arr, out, mode)
+        return out
+
+    @property
+    def embedding(self) -> np.ndarray | None:
+        n = ctypes.c_int64()
+        if self._lib.umapx_n_samples(self._h, ctypes.byref(n)) != L.OK:
+            return None
+        out = np.empty((n.value, self.n_components), np.float32)
+        L.check(self._lib.umapx_get_embedding(self._h, L.ptr(out)))
+        return out
+
+    @property
+    def n_features(self) -> int | None:
+        d = ctypes.c_int64()
+        if self._lib.umapx_n_features(self._h, ctypes.byref(d)) != L.OK:
+            return None
+        return int(d.value)
+
+    def radii(self) -> tuple[np.ndarray, np.ndarray] | None:
+        n = ctypes.c_int64()
+        if self._lib.umapx_n_samples(self._h, ctypes.byref(n)) != L.OK:
+            return None
+        ro = np.empty(n.value, np.float32)
+        re = np.empty(n.value, np.float32)
+        if self._lib.umapx_get_radii(self._h, L.ptr(ro), L.ptr(re)) != L.OK:
+            return None
+        return ro, re
+
+    def ab_params(self) -> tuple[float, float] | None:
+        a = ctypes.c_float()
+        b = ctypes.c_float()
+        if self._lib.umapx_get_ab(self._h, ctypes.byref(a), ctypes.byref(b)) != L.OK:
+            return None
+        return a.value, b.value
+
+
+class Umap:
+    """UMAP on one MI355X GPU with the umapers ``Umap`` interface.
+
+    Example
+    -------
+    >>> import numpy as np
+    >>> from umapers_amd import Umap
+    >>> x = np.random.default_rng(42).normal(size=(100, 8)).astype(np.float32)
+    >>> emb = Umap(n_neighbors=15, n_components=2).fit_transform(x)
+    """
+
+    def __init__(
+        self,
+        *,
+        n_neighbors: int = 15,
+        n_components: int = 2,
+        n_epochs: int | None = None,
+        metric: str = "euclidean",
+        metric_kwds: dict[str, Any] | None = None,
+        learning_rate: float = 1.0,
+        min_dist: float = 0.1,
+        spread: float = 1.0,
+        local_connectivity: float = 1.0,
+        set_op_mix_ratio: float = 1.0,
+        repulsion_strength: float = 1.0,
+        negative_sample_rate: int = 5,
+        random_seed: int = 42,
+        init: str = "spectral",
+        ann_mode: str = "auto",
+        use_approximate_knn: bool = True,
+        approx_knn_candidates: int = 50,
+        approx_knn_iters: int = 14,
+        approx_knn_threshold: int = 4096,
+        densmap: bool = False,
+        dens_lambda: float = 2.0,
+        dens_frac: float = 0.3,
+        dens_var_shift: float = 0.1,
+        output_dens: bool = False,
+        target_metric: str | None = None,
+        target_weight: float = 0.5,
+        target_n_neighbors: int | None = None,
+        knn_precision: str = "fp32",
+        rerank_factor: int = 2,
+        transform_mode: str = "plain",
+        smooth_fp64: bool = True,
+        mlp_hidden: int = 256,
+        mlp_steps: int = 1500,
+        mlp_batch: int = 4096,
+        mlp_negatives: int = 5,
+        mlp_learning_rate: float = 1e-3,
+        mlp_embedding_weight: float = 1.0,
+    ) -> None:
+        for fitted_attr in ("embedding_", "n_features_in_", "n_samples_fit_", "radii_original_",
+                            "radii_embedding_", "rad_orig_", "rad_emb_"):
+            if hasattr(self, fitted_attr):
+                delattr(self, fitted_attr)
+        self.n_neighbors = int(n_neighbors)
+        self.n_components = int(n_components)
+        self.n_epochs = None if n_epochs is None else int(n_epochs)
+        self.metric = _normalize_metric_name(metric)
+        if metric_kwds:
+            bad = ", ".join(sorted(str(k) for k in metric_kwds))
+            raise ValueError(f"unsupported metric_kwds for {self.metric}: {bad}")
+        self.metric_kwds = metric_kwds
+        self.learning_rate = float(learning_rate)
+        self.min_dist = float(min_dist)
+        self.spread = float(spread)
+        self.local_connectivity = float(local_connectivity)
+        self.set_op_mix_ratio = float(set_op_mix_ratio)
+        self.repulsion_strength = float(repulsion_strength)
+        self.negative_sample_rate = int(negative_sample_rate)
+        self.random_seed = int(random_seed)
+        self.init = str(init)
+        if self.init.lower() not in ("random", "spectral"):
+            raise ValueError(f"unsupported init '{init}', expected random|spectral")
+        if target_metric not in (None, "none", ""):
+            raise ValueError("supervised (target_metric) fitting is not available on the GPU path")
+        self.target_metric = None
+        self.target_weight = float(target_weight)
+        self.target_n_neighbors = target_n_neighbors
+        self.ann_mode, use_approximate_knn, approx_knn_threshold = _normalize_ann_mode(
+            ann_mode, use_approximate_knn, approx_knn_threshold)
+        self.use_approximate_knn = bool(use_approximate_knn)
+        self.approx_knn_candidates = int(approx_knn_candidates)
+        self.approx_knn_iters = int(approx_knn_iters)
+        self.approx_knn_threshold = int(approx_knn_threshold)
+        self.densmap = bool(densmap)
+        self.dens_lambda, self.dens_frac, self.dens_var_shift = _validate_dens_params(
+            dens_lambda, dens_frac, dens_var_shift)
+        self.output_dens = bool(output_dens)
+        if knn_precision not in ("fp32", "bf16_rerank"):
+            raise ValueError(f"unsupported knn_precision '{knn_precision}', expected fp32|bf16_rerank")
+        self.knn_precision = knn_precision
+        self.rerank_factor = int(rerank_factor)
+        if transform_mode not in ("plain", "learned"):
+            raise ValueError(f"unsupported transform_mode '{transform_mode}', expected plain|learned")
+        self.transform_mode = transform_mode
+        self.smooth_fp64 = bool(smooth_fp64)
+        self.mlp_hidden = int(mlp_hidden)
+        self.mlp_steps = int(mlp_steps)
+        self.mlp_batch = int(mlp_batch)
+        self.mlp_negatives = int(mlp_negatives)
+        self.mlp_learning_rate = float(mlp_learning_rate)
+        self.mlp_embedding_weight = float(mlp_embedding_weight)
+        self._core = _Core(self._c_params())
+
+    def _c_params(self) -> L.Params:
+        p = L.default_params()
+        p.n_neighbors = self.n_neighbors
+        p.n_components = self.n_components
+        p.n_epochs = 0 if self.n_epochs is None else self.n_epochs
+        p.metric = _GPU_METRICS[self.metric]
+        p.learning_rate = self.learning_rate
+        p.min_dist = self.min_dist
+        p.spread = self.spread
+        p.local_connectivity = self.local_connectivity
+        p.set_op_mix_ratio = self.set_op_mix_ratio
+        p.repulsion_strength = self.repulsion_strength
+        p.negative_sample_rate = self.negative_sample_rate
+        p.init = L.INIT_SPECTRAL if self.init.lower() == "spectral" else L.INIT_RANDOM
+        p.random_seed = self.random_seed & 0xFFFFFFFFFFFFFFFF
+        p.use_approximate_knn = int(self.use_approximate_knn)
+        p.approx_knn_candidates = self.approx_knn_candidates
+        p.approx_knn_iters = self.approx_knn_iters
+        p.approx_knn_threshold = self.approx_knn_threshold
+        p.densmap = int(self.densmap)
+        p.dens_lambda = self.dens_lambda
+        p.dens_frac = self.dens_frac
+        p.dens_var_shift = self.dens_var_shift
+        p.output_dens = int(self.output_dens)
+        p.knn_precision = L.KNN_FP32 if self.knn_precision == "fp32" else L.KNN_BF16_RERANK
+        p.rerank_factor = self.rerank_factor
+        p.smooth_fp64 = int(self.smooth_fp64)
+        p.transform_mode = L.TRANSFORM_PLAIN if self.transform_mode == "plain" else L.TRANSFORM_LEARNED
+        p.mlp_hidden = self.mlp_hidden
+        p.mlp_steps = self.mlp_steps
+        p.mlp_batch = self.mlp_batch
+        p.mlp_negatives = self.mlp_negatives
+        p.mlp_learning_rate = self.mlp_learning_rate
+        p.mlp_embedding_weight = self.mlp_embedding_weight
+        return p
+
+    def get_params(self, deep: bool = True) -> dict[str, Any]:
+        return {name: getattr(self, name) for name in _PARAM_NAMES}
+
+    def set_params(self, **params: Any) -> "Umap":
+        if not params:
+            return self
+        valid = set(_PARAM_NAMES)
+        for key in params:
+            if key not in valid:
+                raise ValueError(f"Invalid parameter '{key}' for estimator Umap")
+        current = self.get_params(deep=False)
+        current.update(params)
+        self.__init__(**current)
+        return self
+
+    def _sync_fitted(self, embedding: np.ndarray, n_features: int, n_samples: int) -> None:
+        self.embedding_ = embedding
+        self.n_features_in_ = int(n_features)
+        self.n_samples_fit_ = int(n_samples)
+        radii = self._core.radii() if self.output_dens else None
+        if radii is None:
+            for attr in ("radii_original_", "radii_embedding_", "rad_orig_", "rad_emb_"):
+                if hasattr(self, attr):
+                    delattr(self, attr)
+            return
+        self.radii_original_, self.radii_embedding_ = radii
+        self.rad_orig_, self.rad_emb_ = radii
+
+    def _reject_sparse(self, data: Any) -> None:
+        if _is_sparse(data):
+            raise ValueError("sparse input is not available on the GPU path; pass a dense array")
+
+    def fit(self, data: Any, y: Any | None = None) -> "Umap":
+        self._reject_sparse(data)
+        arr = _as_f32_matrix(data, "data")
+        self._core.fit(arr)
+        self._sync_fitted(self._core.embedding, arr.shape[1], arr.shape[0])
+        return self
+
+    def fit_transform(self, data: Any, y: Any | None = None, *,
+                      out: np.ndarray | None = None):
+        self._reject_sparse(data)
+        arr = _as_f32_matrix(data, "data")
+        if out is not None and self.output_dens:
+            raise ValueError("out cannot be used when output_dens=True")
+        if out is None:
+            embedding = self._core.fit_transform(arr)
+            self._sync_fitted(embedding, arr.shape[1], arr.shape[0])
+            if self.output_dens:
+                return embedding, self.radii_original_, self.radii_embedding_
+            return embedding
+        out_buf = _as_out_buffer(out, (arr.shape[0], self.n_components))
+        self._core.fit_transform_into(arr, out_buf)
+        self._sync_fitted(out_buf, arr.shape[1], arr.shape[0])
+        return out_buf
+
+    def profile_fit_transform(self, data: Any) -> dict[str, Any]:
+        if _is_sparse(data):
+            raise ValueError("profile_fit_transform currently supports dense input only")
+        if self.densmap or self.output_dens:
+            raise ValueError("profile_fit_transform currently supports standard UMAP only")
+        arr = _as_f32_matrix(data, "data")
+        result = self._core.profile_fit_transform(arr)
+        self._sync_fitted(result["embedding"], arr.shape[1], arr.shape[0])
+        return result
+
+    def fit_transform_with_knn(self, data: Any, knn_indices: Any, knn_dists: Any, *,
+                               knn_metric: str = "euclidean", validate_precomputed: bool = True,
+                               out: np.ndarray | None = None) -> np.ndarray:
+        arr = _as_f32_matrix(data, "data")
+        idx = _as_knn_indices(knn_indices, "knn_indices")
+        dist = _as_f32_matrix(knn_dists, "knn_dists")
+        shape = (arr.shape[0], self.n_components)
+        out_buf = np.empty(shape, np.float32) if out is None else _as_out_buffer(out, shape)
+        self._core.fit_transform_with_knn_into(arr, idx, dist, out_buf, knn_metric,
+                                               validate_precomputed)
+        self._sync_fitted(out_buf if out is None else out_buf.copy(), arr.shape[1], arr.shape[0])
+        return out_buf
+
+    def transform(self, query: Any, *, out: np.ndarray | None = None) -> np.ndarray:
+        self._reject_sparse(query)
+        arr = _as_f32_matrix(query, "query")
+        shape = (arr.shape[0], self.n_components)
+        if out is None:
+            return self._core.transform(arr)
+        out_buf = _as_out_buffer(out, shape)
+        self._core.transform_into(arr, out_buf)
+        return out_buf
+
+    def transform_plain(self, query: Any) -> np.ndarray:
+        """The reference's kNN + SGD transform, whatever transform_mode says."""
+        return self._core.transform(_as_f32_matrix(query, "query"), "plain")
+
+    def transform_learned(self, query: Any) -> np.ndarray:
+        """One pass through the learned encoder (trained on first use)."""
+        return self._core.transform(_as_f32_matrix(query, "query"), "learned")
+
+    def inverse_transform(self, embedded_query: Any, *, out: np.ndarray | None = None):
+        raise NotImplementedError("inverse_transform is not available on the GPU path")
+
+    @property
+    def ab_params(self) -> tuple[float, float] | None:
+        return self._core.ab_params()
+
+
+def fit_transform(data: Any, y: Any | None = None, **kwargs: Any):
+    """_api.py:1054-1094: embed a dataset in one call."""
+    return Umap(**kwargs).fit_transform(data, y=y)

@@ -1,0 +1,34 @@
+// Exact kNN on the GPU (candidate MFMA tiles + exact f32 re-rank).
+#pragma once
+
+#include "common.hpp"
+
+namespace umapx {
+
+// Reusable device scratch for knn_device (grown on demand).
+struct KnnWorkspace {
+  DevBuf<double> colsum;
+  DevBuf<unsigned char> qprep, rprep;
+  DevBuf<float> qnorm2, rnorm2, qnorm, rnorm;
+  DevBuf<int32_t> cand;
+};
+
+// Events around the candidate (MFMA) kernel for live roofline figures.
+struct KnnStats {
+  hipEvent_t cand_start = nullptr, cand_stop = nullptr;
+  double flops = 0.0;  // 2 * m * n * d, all ordered pairs
+  bool recorded = false;
+};
+
+int knn_padded_dim(int64_t d);
+int knn_candidate_count(int k, int rerank_factor);
+
+// Top-k of every query row q[m x d] against r[n x d] (device pointers).  When
+// q == r and m == n this is the self-kNN of the fit (self included, as the
+// reference does).  metric: 0 euclidean, 1 cosine.  precision: 0 FP32
+// candidates, 1 BF16 candidates (both re-ranked exactly in f32).
+void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int k,
+                int metric, int precision, int rerank_factor, int32_t* out_idx,
+                float* out_dist, KnnWorkspace& ws, hipStream_t s, KnnStats* stats);
+
+}  // namespace umapx

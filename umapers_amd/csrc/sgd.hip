@@ -1,0 +1,342 @@
+// SGD layout epochs on gfx950.
+//
+// The reference runs one Gauss-Seidel pass over the head-sorted edges per
+// epoch on a single thread (optimize_layout_training_flat,
+// rust_umap/src/lib.rs:5137-5318; densMAP loop :5324-5441).  Here each epoch is
+// one edge-parallel launch (Hogwild): a thread per edge checks the edge's f32
+// schedule, applies the attraction to head and tail with float atomics, then
+// draws its negatives from a counter-based hash of (seed, epoch, edge, draw)
+// and accumulates the repulsion on a register copy of the head.  The per-edge
+// schedule (epoch_of_next_sample, epoch_of_next_negative_sample) uses the
+// reference's exact f32 operations, so every edge fires and draws negatives
+// the same number of times as in the reference.
+#include "sgd.hpp"
+
+#include <cmath>
+
+namespace umapx {
+namespace {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// Uniform index in [0, n) for draw s of stream key (Lemire's multiply-shift
+// on the top 32 bits; n < 2^31).
+__device__ __forceinline__ uint32_t draw_index(uint64_t key, uint32_t s, uint32_t n) {
+  const uint64_t h = mix64(key + (static_cast<uint64_t>(s) + 1ULL) * 0x9e3779b97f4a7c15ULL);
+  return static_cast<uint32_t>(((h >> 32) * static_cast<uint64_t>(n)) >> 32);
+}
+
+__device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.f), 4.f); }
+
+__device__ __forceinline__ float pow_pos(float x, float p) { return __expf(p * __logf(x)); }
+
+// Schedule step for one edge in one epoch.  Returns false when the edge does
+// not fire; otherwise reports the number of negatives and updates state.
+struct EdgeSchedule {
+  __device__ static inline bool fire(float* __restrict__ eons, const float* __restrict__ eps,
+                                     int64_t e, float ef) {
+    const float cur = eons[e];
+    if (cur > ef) return false;
+    eons[e] = fadd(cur, eps[e]);
+    return true;
+  }
+  __device__ static inline int negatives(float* __restrict__ eonns, const float* __restrict__ eps,
+                                         int64_t e, float ef, float nsr) {
+    const float en = fdiv(eps[e], nsr);
+    if (!isfinite(en) || en <= 0.f) return -1;
+    const float cur = eonns[e];
+    const float fl = floorf(fdiv(fsub(ef, cur), en));
+    const int nneg = fl > 0.f ? static_cast<int>(fl) : 0;
+    eonns[e] = fadd(cur, fmul(static_cast<float>(nneg), en));
+    return nneg;
+  }
+};
+
+template <int DIM>
+__global__ __launch_bounds__(256) void sgd_epoch_kernel(
+    float* __restrict__ y, const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
+    const float* __restrict__ eps, float* __restrict__ eons, float* __restrict__ eonns,
+    int64_t ne, uint32_t n, int dim_rt, float ef, float alpha, float a, float b, float rep,
+    float nsr, uint64_t key_epoch, const float* __restrict__ radii, int dens_on, float dens_lambda,
+    float var_shift, int32_t* __restrict__ pos_counts, int32_t* __restrict__ neg_counts) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  if (!EdgeSchedule::fire(eons, eps, e, ef)) return;
+  const int dim = DIM > 0 ? DIM : dim_rt;
+  constexpr int MAXD = DIM > 0 ? DIM : 8;
+  const int32_t h = head[e];
+  const int32_t t = tail[e];
+  float yh[MAXD], yt[MAXD];
+  for (int c = 0; c < dim; ++c) {
+    yh[c] = y[static_cast<int64_t>(h) * dim + c];
+    yt[c] = y[static_cast<int64_t>(t) * dim + c];
+  }
+  float dh[MAXD];
+  for (int c = 0; c < dim; ++c) dh[c] = 0.f;
+  if (h != t) {
+    float d2 = 0.f;
+    for (int c = 0; c < dim; ++c) {
+      const float df = yh[c] - yt[c];
+      d2 += df * df;
+    }
+    if (d2 > 0.f) {
+      const float pb = pow_pos(d2, b);
+      const float gc = (-2.f * a * b) * (pb / d2) / (a * pb + 1.f);
+      for (int c = 0; c < dim; ++c) {
+        const float g = clip4(gc * (yh[c] - yt[c])) * alpha;
+        yh[c] += g;
+        yt[c] -= g;
+        dh[c] += g;
+        atomicAdd(&y[static_cast<int64_t>(t) * dim + c], -g);
+      }
+    }
+    if (dens_on) {
+      // apply_density_gradient (lib.rs:5110-5134) on the updated pair
+      float s2 = 0.f;
+      for (int c = 0; c < dim; ++c) {
+        const float df = yh[c] - yt[c];
+        s2 += df * df;
+      }
+      const float cur = sqrtf(s2);
+      const float target = fmaxf(expf((radii[h] + radii[t]) * 0.5f), 1e-6f);
+      const float coeff = 0.1f * dens_lambda * (cur - target) / (cur + var_shift);
+      for (int c = 0; c < dim; ++c) {
+        const float dir = (yh[c] - yt[c]) / (cur + 1e-6f);
+        const float g = clip4(coeff * dir) * alpha;
+        yh[c] -= g;
+        dh[c] -= g;
+        atomicAdd(&y[static_cast<int64_t>(t) * dim + c], g);
+      }
+    }
+  }
+  if (pos_counts) pos_counts[e] += 1;
+  const int nneg = EdgeSchedule::negatives(eonns, eps, e, ef, nsr);
+  if (nneg > 0) {
+    if (neg_counts) neg_counts[e] += nneg;
+    const uint64_t key = mix64(key_epoch ^ static_cast<uint64_t>(e));
+    for (int s = 0; s < nneg; ++s) {
+      const uint32_t k = draw_index(key, static_cast<uint32_t>(s), n);
+      if (static_cast<int32_t>(k) == h) continue;
+      float d2 = 0.f;
+      float yk[MAXD];
+      for (int c = 0; c < dim; ++c) {
+        yk[c] = y[static_cast<int64_t>(k) * dim + c];
+        const float df = yh[c] - yk[c];
+        d2 += df * df;
+      }
+      if (d2 > 0.f) {
+        const float pb = pow_pos(d2, b);
+        const float gc = rep / ((0.001f + d2) * (a * pb + 1.f));
+        if (gc > 0.f)
+          for (int c = 0; c < dim; ++c) {
+            const float g = clip4(gc * (yh[c] - yk[c])) * alpha;
+            yh[c] += g;
+            dh[c] += g;
+          }
+      }
+    }
+  }
+  for (int c = 0; c < dim; ++c)
+    if (dh[c] != 0.f) atomicAdd(&y[static_cast<int64_t>(h) * dim + c], dh[c]);
+}
+
+// One thread per query head: the transform SGD moves only the query point
+// (lib.rs:5444-5544).  Edge schedule state lives in the workspace arrays.
+template <int DIM>
+__global__ __launch_bounds__(128) void transform_sgd_kernel(
+    float* __restrict__ yq, int64_t m, int dim_rt, const float* __restrict__ yb, uint32_t n_base,
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ tail,
+    const float* __restrict__ eps, float* __restrict__ eons, float* __restrict__ eonns,
+    int n_epochs, float alpha0, float a, float b, float rep, float nsr, uint64_t seed) {
+  const int64_t hq = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (hq >= m) return;
+  const int dim = DIM > 0 ? DIM : dim_rt;
+  constexpr int MAXD = DIM > 0 ? DIM : 8;
+  float cur[MAXD];
+  for (int c = 0; c < dim; ++c) cur[c] = yq[hq * dim + c];
+  const int32_t s0 = row_ptr[hq], s1 = row_ptr[hq + 1];
+  const uint64_t key_head = mix64(seed ^ (static_cast<uint64_t>(hq) * 0x9E3779B97F4A7C15ULL));
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    const float alpha = fmul(alpha0, fsub(1.f, fdiv(static_cast<float>(epoch), static_cast<float>(n_epochs))));
+    const float ef = static_cast<float>(epoch);
+    for (int32_t e = s0; e < s1; ++e) {
+      bool finite = true;
+      for (int c = 0; c < dim; ++c) finite &= isfinite(cur[c]);
+      if (eons[e] > ef || !finite) continue;
+      const float* other = yb + static_cast<int64_t>(tail[e]) * dim;
+      float d2 = 0.f;
+      for (int c = 0; c < dim; ++c) {
+        const float df = cur[c] - other[c];
+        d2 += df * df;
+      }
+      float gc = 0.f;
+      if (d2 > 0.f) {
+        const float pb = pow_pos(d2, b);
+        gc = (-2.f * a * b) * (pb / d2) / (a * pb + 1.f);
+      }
+      for (int c = 0; c < dim; ++c) cur[c] += clip4(gc * (cur[c] - other[c])) * alpha;
+      eons[e] = fadd(eons[e], eps[e]);
+      const int nneg = EdgeSchedule::negatives(eonns, eps, e, ef, nsr);
+      if (nneg <= 0) continue;
+      const uint64_t key = mix64(key_head + static_cast<uint64_t>(epoch) * 0xD1B54A32D192ED03ULL +
+                                 static_cast<uint64_t>(e - s0));
+      for (int s = 0; s < nneg; ++s) {
+        const uint32_t k = draw_index(key, static_cast<uint32_t>(s), n_base);
+        const float* o2 = yb + static_cast<int64_t>(k) * dim;
+        float dd = 0.f;
+        for (int c = 0; c < dim; ++c) {
+          const float df = cur[c] - o2[c];
+          dd += df * df;
+        }
+        float g2 = 0.f;
+        if (dd > 0.f) {
+          const float pb = pow_pos(dd, b);
+          g2 = rep / ((0.001f + dd) * (a * pb + 1.f));
+        }
+        if (g2 > 0.f)
+          for (int c = 0; c < dim; ++c) cur[c] += clip4(g2 * (cur[c] - o2[c])) * alpha;
+      }
+    }
+  }
+  for (int c = 0; c < dim; ++c) yq[hq * dim + c] = cur[c];
+}
+
+__global__ void copy_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void div_kernel(const float* __restrict__ eps, float* __restrict__ out, int64_t n,
+                           float nsr) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = fdiv(eps[i], nsr);
+}
+
+// init_graph_transform (lib.rs:4793-4836), one thread per query row.
+__global__ void init_transform_kernel(int64_t m, int dim, const int32_t* __restrict__ row_ptr,
+                                      const int32_t* __restrict__ tail,
+                                      const float* __restrict__ w, const float* __restrict__ yb,
+                                      float* __restrict__ yq) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int32_t s0 = row_ptr[i], s1 = row_ptr[i + 1];
+  float* o = yq + i * dim;
+  if (s0 == s1) {
+    for (int c = 0; c < dim; ++c) o[c] = NAN;
+    return;
+  }
+  for (int32_t e = s0; e < s1; ++e)
+    if (fabsf(fsub(w[e], 1.f)) < 1.1920929e-7f) {
+      for (int c = 0; c < dim; ++c) o[c] = yb[static_cast<int64_t>(tail[e]) * dim + c];
+      return;
+    }
+  float rs = 0.f;
+  for (int32_t e = s0; e < s1; ++e) rs = fadd(rs, w[e]);
+  if (rs <= 0.f) {
+    for (int c = 0; c < dim; ++c) o[c] = NAN;
+    return;
+  }
+  for (int c = 0; c < dim; ++c) o[c] = 0.f;
+  for (int32_t e = s0; e < s1; ++e) {
+    const float ww = fdiv(w[e], rs);
+    for (int c = 0; c < dim; ++c)
+      o[c] = fadd(o[c], fmul(ww, yb[static_cast<int64_t>(tail[e]) * dim + c]));
+  }
+}
+
+uint64_t host_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const SgdArgs& args,
+                            SgdWorkspace& ws, hipStream_t s, SgdStats* stats) {
+  maybe_inject_fault("optimize_layout");
+  const int64_t ne = g.n_edges;
+  if (ne == 0 || n == 0 || args.n_epochs == 0) return;
+  if (dim > 8) throw_invalid("n_components must be <= 8 on the GPU path");
+  ws.eps.ensure(ne);
+  ws.eons.ensure(ne);
+  ws.eonns.ensure(ne);
+  epochs_per_sample_device(g.weight.get(), ne, g.max_weight, args.n_epochs, ws.eps.get(), s);
+  const float nsr = static_cast<float>(args.negative_sample_rate);
+  hipLaunchKernelGGL(copy_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, s, ws.eps.get(),
+                     ws.eons.get(), ne);
+  hipLaunchKernelGGL(div_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, s, ws.eps.get(),
+                     ws.eonns.get(), ne, nsr);
+  KERNEL_CHECK();
+  const float rep = 2.f * args.repulsion_strength * args.b;
+  const unsigned blocks = blocks_for(ne, 256);
+  if (stats) HIP_CHECK(hipEventRecord(stats->start, s));
+  for (int epoch = 0; epoch < args.n_epochs; ++epoch) {
+    const float alpha = args.learning_rate *
+                        (1.f - static_cast<float>(epoch) / static_cast<float>(args.n_epochs));
+    const uint64_t key_epoch = host_mix64(args.seed + static_cast<uint64_t>(epoch) * 0xD1B54A32D192ED03ULL);
+    const int dens_on = (args.radii != nullptr && epoch >= args.dens_start_epoch) ? 1 : 0;
+    if (dim == 2)
+      hipLaunchKernelGGL(sgd_epoch_kernel<2>, dim3(blocks), dim3(256), 0, s, y, g.head.get(),
+                         g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(), ne,
+                         static_cast<uint32_t>(n), dim, static_cast<float>(epoch), alpha, args.a,
+                         args.b, rep, nsr, key_epoch, args.radii, dens_on, args.dens_lambda,
+                         args.dens_var_shift, args.pos_counts, args.neg_counts);
+    else
+      hipLaunchKernelGGL(sgd_epoch_kernel<0>, dim3(blocks), dim3(256), 0, s, y, g.head.get(),
+                         g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(), ne,
+                         static_cast<uint32_t>(n), dim, static_cast<float>(epoch), alpha, args.a,
+                         args.b, rep, nsr, key_epoch, args.radii, dens_on, args.dens_lambda,
+                         args.dens_var_shift, args.pos_counts, args.neg_counts);
+    KERNEL_CHECK();
+  }
+  if (stats) {
+    HIP_CHECK(hipEventRecord(stats->stop, s));
+    stats->recorded = true;
+  }
+}
+
+void optimize_transform_device(float* yq, int64_t m, int dim, const float* y_base, int64_t n_base,
+                               const Graph& g, const SgdArgs& args, SgdWorkspace& ws,
+                               hipStream_t s) {
+  maybe_inject_fault("optimize_layout");
+  const int64_t ne = g.n_edges;
+  if (ne == 0 || m == 0 || args.n_epochs == 0) return;
+  if (dim > 8) throw_invalid("n_components must be <= 8 on the GPU path");
+  ws.eps.ensure(ne);
+  ws.eons.ensure(ne);
+  ws.eonns.ensure(ne);
+  epochs_per_sample_device(g.weight.get(), ne, g.max_weight, args.n_epochs, ws.eps.get(), s);
+  const float nsr = static_cast<float>(args.negative_sample_rate);
+  hipLaunchKernelGGL(copy_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, s, ws.eps.get(),
+                     ws.eons.get(), ne);
+  hipLaunchKernelGGL(div_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, s, ws.eps.get(),
+                     ws.eonns.get(), ne, nsr);
+  KERNEL_CHECK();
+  const float rep = 2.f * args.repulsion_strength * args.b;
+  if (dim == 2)
+    hipLaunchKernelGGL(transform_sgd_kernel<2>, dim3(blocks_for(m, 128)), dim3(128), 0, s, yq, m,
+                       dim, y_base, static_cast<uint32_t>(n_base), g.row_ptr.get(), g.tail.get(),
+                       ws.eps.get(), ws.eons.get(), ws.eonns.get(), args.n_epochs,
+                       args.learning_rate, args.a, args.b, rep, nsr, args.seed);
+  else
+    hipLaunchKernelGGL(transform_sgd_kernel<0>, dim3(blocks_for(m, 128)), dim3(128), 0, s, yq, m,
+                       dim, y_base, static_cast<uint32_t>(n_base), g.row_ptr.get(), g.tail.get(),
+                       ws.eps.get(), ws.eons.get(), ws.eonns.get(), args.n_epochs,
+                       args.learning_rate, args.a, args.b, rep, nsr, args.seed);
+  KERNEL_CHECK();
+}
+
+void init_graph_transform_device(int64_t m, int dim, const Graph& g, const float* y_base,
+                                 float* yq, hipStream_t s) {
+  if (m == 0) return;
+  hipLaunchKernelGGL(init_transform_kernel, dim3(blocks_for(m, 128)), dim3(128), 0, s, m, dim,
+                     g.row_ptr.get(), g.tail.get(), g.weight.get(), y_base, yq);
+  KERNEL_CHECK();
+}
+
+}  // namespace umapx

@@ -1,0 +1,773 @@
+// Spectral initialisation on gfx950.
+//
+// The reference runs, per connected component, 256 sequential f64 block
+// iterations Y <- orth((I + D^-1/2 W D^-1/2) Y) with 18 columns and a
+// twice-iterated modified Gram-Schmidt (rust_umap/src/lib.rs:4161-4326), then a
+// Ritz eigen-solve.  Here all components of at least 128 rows iterate together
+// on the GPU:
+//   - the shifted normalised adjacency product is a CSR row gather;
+//   - orthonormalisation is CholeskyQR2 per component (Gram matrix by a
+//     segmented reduction over the component's rows, an 18x18 Cholesky, a
+//     triangular solve per row, done twice).  In exact arithmetic it returns
+//     the same Q as Gram-Schmidt (the QR factor with a positive diagonal is
+//     unique) and its first-pass pivots are the residual norms Gram-Schmidt
+//     checks against 1e-12;
+//   - the 18x18 Ritz problems are solved on the host.
+// Smaller components take the reference's dense Laplacian path on the host,
+// and tiny ones its random layout, exactly as lib.rs:4670-4764 decides.
+#include "spectral.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "hostmath.hpp"
+
+namespace umapx {
+namespace {
+
+constexpr int kSub = 18;            // (n_components + 1) * 6 for 2-D output
+constexpr int kIters = 256;         // SPECTRAL_ITERATIVE_MAX_ITERS
+constexpr int kItemRows = 512;      // rows per reduction work item
+constexpr double kOrthoEps = 1e-12; // SPECTRAL_ORTHO_EPS
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double uniform01(uint64_t key) {
+  return static_cast<double>(mix64(key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------- components ---
+__global__ void init_labels_kernel(int32_t* label, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) label[i] = static_cast<int32_t>(i);
+}
+
+__global__ void hook_kernel(const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
+                            int64_t ne, int32_t* __restrict__ label, int32_t* __restrict__ changed) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int32_t lu = label[head[e]];
+  const int32_t lv = label[tail[e]];
+  if (lu == lv) return;
+  const int32_t hi = lu > lv ? lu : lv;
+  const int32_t lo = lu > lv ? lv : lu;
+  atomicMin(&label[hi], lo);
+  *changed = 1;
+}
+
+__global__ void jump_kernel(int32_t* __restrict__ label, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t l = label[i];
+  int32_t ll = label[l];
+  while (l != ll) {
+    l = ll;
+    ll = label[l];
+  }
+  label[i] = l;
+}
+
+// --------------------------------------------------------------- degree ---
+__global__ void degree_kernel(const int32_t* __restrict__ row_ptr, const float* __restrict__ w,
+                              int64_t n, double* __restrict__ deg, double* __restrict__ inv) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) s += static_cast<double>(fmaxf(w[e], 0.f));
+  deg[i] = s;
+  inv[i] = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+}
+
+// Start block: column 0 = sqrt(degree), the rest U(-1, 1) (lib.rs:4252-4260;
+// a counter-based stream replaces SmallRng).
+__global__ void start_block_kernel(const int32_t* __restrict__ perm, int64_t rows,
+                                   const int32_t* __restrict__ slot_of_pos,
+                                   const int32_t* __restrict__ local_of_pos,
+                                   const uint64_t* __restrict__ seeds,
+                                   const double* __restrict__ deg, double* __restrict__ y) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const int32_t v = perm[p];
+  const uint64_t key = seeds[slot_of_pos[p]] ^ 0x7A5B3D91CE42A117ULL;
+  const uint64_t lr = static_cast<uint64_t>(local_of_pos[p]);
+  double* row = y + static_cast<int64_t>(v) * kSub;
+  row[0] = sqrt(deg[v]);
+  for (int c = 1; c < kSub; ++c)
+    row[c] = 2.0 * uniform01(key + lr * kSub + static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ULL) - 1.0;
+}
+
+// out = Y + D^-1/2 W D^-1/2 Y over the rows listed in perm.
+__global__ __launch_bounds__(256) void shifted_mul_kernel(
+    const int32_t* __restrict__ perm, int64_t rows, const int32_t* __restrict__ row_ptr,
+    const int32_t* __restrict__ tail, const float* __restrict__ w, const double* __restrict__ inv,
+    const double* __restrict__ y, double* __restrict__ out) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const int32_t v = perm[p];
+  double acc[kSub];
+  const double* yv = y + static_cast<int64_t>(v) * kSub;
+#pragma unroll
+  for (int c = 0; c < kSub; ++c) acc[c] = yv[c];
+  const double iv = inv[v];
+  for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e) {
+    const int32_t t = tail[e];
+    const double nw = static_cast<double>(fmaxf(w[e], 0.f)) * iv * inv[t];
+    if (nw == 0.0 || t == v) continue;
+    const double2* yt = reinterpret_cast<const double2*>(y + static_cast<int64_t>(t) * kSub);
+#pragma unroll
+    for (int c = 0; c < kSub / 2; ++c) {
+      const double2 q = yt[c];
+      acc[2 * c] += nw * q.x;
+      acc[2 * c + 1] += nw * q.y;
+    }
+  }
+  double* o = out + static_cast<int64_t>(v) * kSub;
+#pragma unroll
+  for (int c = 0; c < kSub; ++c) o[c] = acc[c];
+}
+
+// Per work item (slot, begin, end over perm positions): upper triangle of
+// A^T B accumulated into gram[slot].
+__global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ items,
+                                                   const int32_t* __restrict__ perm,
+                                                   const double* __restrict__ a,
+                                                   const double* __restrict__ b,
+                                                   double* __restrict__ gram) {
+  __shared__ double sa[32][kSub + 1];
+  __shared__ double sb[32][kSub + 1];
+  const int32_t slot = items[blockIdx.x * 3 + 0];
+  const int32_t p0 = items[blockIdx.x * 3 + 1];
+  const int32_t p1 = items[blockIdx.x * 3 + 2];
+  const int tid = threadIdx.x;
+  // pair (i, j) with j >= i for tid < 171
+  int pi = 0, pj = 0;
+  {
+    int t = tid, i = 0;
+    while (i < kSub && t >= kSub - i) {
+      t -= kSub - i;
+      ++i;
+    }
+    pi = i;
+    pj = i + t;
+  }
+  const bool active = tid < kSub * (kSub + 1) / 2;
+  double acc = 0.0;
+  for (int32_t base = p0; base < p1; base += 32) {
+    __syncthreads();
+    for (int q = tid; q < 32 * kSub; q += 256) {
+      const int r = q / kSub, c = q % kSub;
+      double va = 0.0, vb = 0.0;
+      if (base + r < p1) {
+        const int64_t v = perm[base + r];
+        va = a[v * kSub + c];
+        vb = b[v * kSub + c];
+      }
+      sa[r][c] = va;
+      sb[r][c] = vb;
+    }
+    __syncthreads();
+    if (active)
+#pragma unroll 8
+      for (int r = 0; r < 32; ++r) acc += sa[r][pi] * sb[r][pj];
+  }
+  if (active) atomicAdd(&gram[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
+}
+
+// Cholesky of each slot's Gram matrix (upper factor R, G = R^T R).  A pivot
+// <= 1e-12 (or non-finite) marks the slot failed, as the Gram-Schmidt norm
+// check of lib.rs:4186-4190 would.
+__global__ void cholesky_kernel(const double* __restrict__ gram, double* __restrict__ rmat,
+                                int n_slots, int32_t* __restrict__ fail) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n_slots) return;
+  const double* g = gram + static_cast<int64_t>(slot) * kSub * kSub;
+  double* r = rmat + static_cast<int64_t>(slot) * kSub * kSub;
+  for (int j = 0; j < kSub; ++j) {
+    double piv = g[j * kSub + j];
+    for (int k = 0; k < j; ++k) piv -= r[k * kSub + j] * r[k * kSub + j];
+    if (!(piv > kOrthoEps) || !isfinite(piv)) {
+      fail[slot] = 1;
+      piv = 1.0;
+    }
+    const double rjj = sqrt(piv);
+    r[j * kSub + j] = rjj;
+    for (int i = j + 1; i < kSub; ++i) {
+      double v = g[j * kSub + i];
+      for (int k = 0; k < j; ++k) v -= r[k * kSub + j] * r[k * kSub + i];
+      r[j * kSub + i] = v / rjj;
+    }
+    for (int i = 0; i < j; ++i) r[j * kSub + i] = 0.0;
+  }
+}
+
+// y <- y R^-1 row by row (forward substitution over columns).
+__global__ void apply_rinv_kernel(const int32_t* __restrict__ perm, int64_t rows,
+                                  const int32_t* __restrict__ slot_of_pos,
+                                  const double* __restrict__ rmat, double* __restrict__ y) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const int32_t v = perm[p];
+  const double* r = rmat + static_cast<int64_t>(slot_of_pos[p]) * kSub * kSub;
+  double* row = y + static_cast<int64_t>(v) * kSub;
+  double z[kSub];
+#pragma unroll
+  for (int j = 0; j < kSub; ++j) {
+    double s = row[j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= z[k] * r[k * kSub + j];
+    z[j] = s / r[j * kSub + j];
+  }
+#pragma unroll
+  for (int j = 0; j < kSub; ++j) row[j] = z[j];
+}
+
+// coords = Y V_slot (kSub x dim) in f64, stored f32; per-slot max |coord|.
+__global__ void project_kernel(const int32_t* __restrict__ perm, int64_t rows,
+                               const int32_t* __restrict__ slot_of_pos,
+                               const double* __restrict__ y, const double* __restrict__ vmat,
+                               int dim, float* __restrict__ out, float* __restrict__ maxabs) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const int32_t v = perm[p];
+  const int32_t slot = slot_of_pos[p];
+  const double* row = y + static_cast<int64_t>(v) * kSub;
+  const double* vm = vmat + static_cast<int64_t>(slot) * kSub * dim;
+  float mx = 0.f;
+  for (int c = 0; c < dim; ++c) {
+    double s = 0.0;
+    for (int b = 0; b < kSub; ++b) s += row[b] * vm[b * dim + c];
+    const float f = static_cast<float>(s);
+    out[static_cast<int64_t>(v) * dim + c] = f;
+    mx = fmaxf(mx, fabsf(f));
+  }
+  atomicMax(reinterpret_cast<unsigned int*>(&maxabs[slot]), __float_as_uint(mx));
+}
+
+// y = coords * scale[slot] + N(0, 1e-4) + anchor[slot]  (noisy_scale_coords /
+// add_noise + anchor, lib.rs:4745-4791; counter-based normals).
+__global__ void finish_kernel(const int32_t* __restrict__ perm, int64_t rows,
+                              const int32_t* __restrict__ slot_of_pos,
+                              const int32_t* __restrict__ local_of_pos,
+                              const uint64_t* __restrict__ noise_seeds,
+                              const float* __restrict__ scale, const float* __restrict__ anchor,
+                              int dim, const float* __restrict__ coords, float* __restrict__ y) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const int32_t v = perm[p];
+  const int32_t slot = slot_of_pos[p];
+  const uint64_t key = noise_seeds[slot];
+  for (int c = 0; c < dim; ++c) {
+    const uint64_t ctr = static_cast<uint64_t>(local_of_pos[p]) * dim + c;
+    double u1 = uniform01(key + 2 * ctr * 0x9E3779B97F4A7C15ULL);
+    const double u2 = uniform01(key + (2 * ctr + 1) * 0x9E3779B97F4A7C15ULL);
+    if (u1 <= 0.0) u1 = 1e-300;
+    const double nz = 1e-4 * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    const float val = coords[static_cast<int64_t>(v) * dim + c] * scale[slot] + static_cast<float>(nz);
+    y[static_cast<int64_t>(v) * dim + c] = val + anchor[slot * dim + c];
+  }
+}
+
+__global__ void centroid_kernel(const int32_t* __restrict__ items, const int32_t* __restrict__ perm,
+                                const float* __restrict__ x, int d, double* __restrict__ cent) {
+  const int32_t slot = items[blockIdx.x * 3 + 0];
+  const int32_t p0 = items[blockIdx.x * 3 + 1];
+  const int32_t p1 = items[blockIdx.x * 3 + 2];
+  for (int f = threadIdx.x; f < d; f += blockDim.x) {
+    double s = 0.0;
+    for (int32_t p = p0; p < p1; ++p) s += static_cast<double>(x[static_cast<int64_t>(perm[p]) * d + f]);
+    atomicAdd(&cent[static_cast<int64_t>(slot) * d + f], s);
+  }
+}
+
+__global__ void scatter_rows_kernel(const int32_t* __restrict__ rows_idx, int64_t count, int dim,
+                                    const float* __restrict__ vals, float* __restrict__ y) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count * dim) return;
+  const int64_t r = i / dim;
+  const int c = static_cast<int>(i % dim);
+  y[static_cast<int64_t>(rows_idx[r]) * dim + c] = vals[i];
+}
+
+// ------------------------------------------------------------ host side ---
+struct Comp {
+  std::vector<int32_t> members;  // ascending
+};
+
+// lib.rs:3966-4023 normalized_laplacian_from_edges + lib.rs:3885-3926 with
+// drop_first: dense host path for small components.
+bool dense_embedding_from_edges(int64_t n, int dim,
+                                const std::vector<std::tuple<int32_t, int32_t, float>>& edges,
+                                std::vector<float>& coords) {
+  if (n == 0 || edges.empty()) return false;
+  std::vector<double> lap(static_cast<size_t>(n) * n, 0.0);
+  for (int64_t i = 0; i < n; ++i) lap[i * n + i] = 1.0;
+  std::vector<double> deg(n, 0.0);
+  std::vector<std::pair<int64_t, int64_t>> pairs;
+  for (const auto& e : edges) {
+    const int64_t h = std::get<0>(e), t = std::get<1>(e);
+    if (h == t) continue;
+    const double w = static_cast<double>(std::max(std::get<2>(e), 0.0f));
+    if (w <= 0.0) continue;
+    const int64_t i = std::min(h, t), j = std::max(h, t);
+    const double prev = lap[i * n + j];
+    if (w > prev) {
+      if (prev == 0.0) pairs.push_back({i, j});
+      const double delta = w - prev;
+      lap[i * n + j] = w;
+      lap[j * n + i] = w;
+      deg[i] += delta;
+      deg[j] += delta;
+    }
+  }
+  bool any = false;
+  for (double g : deg) any |= g > 0.0;
+  if (!any) return false;
+  std::vector<double> inv(n);
+  for (int64_t i = 0; i < n; ++i) inv[i] = deg[i] > 0.0 ? 1.0 / std::sqrt(deg[i]) : 0.0;
+  for (const auto& pr : pairs) {
+    const double v = lap[pr.first * n + pr.second] * inv[pr.first] * inv[pr.second];
+    lap[pr.first * n + pr.second] = -v;
+    lap[pr.second * n + pr.first] = -v;
+  }
+  for (int64_t i = 0; i < n; ++i) lap[i * n + i] = 1.0;
+  std::vector<double> ev, vec;
+  symmetric_eigen(lap, static_cast<int>(n), ev, vec);
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+  if (static_cast<int64_t>(order.size()) <= dim + 1) return false;
+  coords.assign(static_cast<size_t>(n) * dim, 0.f);
+  for (int c = 0; c < dim; ++c)
+    for (int64_t r = 0; r < n; ++r) coords[r * dim + c] = static_cast<float>(vec[r * n + order[c + 1]]);
+  for (float v : coords)
+    if (!std::isfinite(v)) return false;
+  return true;
+}
+
+// lib.rs:3928-3964 + 4518-4550: meta layout of the component centroids.
+bool meta_layout(const std::vector<std::vector<double>>& cent, int dim, uint64_t seed,
+                 std::vector<float>& layout) {
+  const size_t nc = cent.size();
+  if (nc == 0) return false;
+  if (nc == 1) {
+    layout.assign(dim, 0.f);
+    return true;
+  }
+  if (nc <= static_cast<size_t>(2 * dim)) {
+    const size_t k = (nc + 1) / 2;
+    layout.assign(nc * dim, 0.f);
+    for (size_t i = 0; i < nc; ++i) layout[i * dim + (i % k)] = i < k ? 1.f : -1.f;
+    return true;
+  }
+  std::vector<double> aff(nc * nc, 0.0);
+  for (size_t i = 0; i < nc; ++i) {
+    aff[i * nc + i] = 1.0;
+    for (size_t j = i + 1; j < nc; ++j) {
+      double d2 = 0.0;
+      for (size_t f = 0; f < cent[i].size(); ++f) {
+        const double df = cent[i][f] - cent[j][f];
+        d2 += df * df;
+      }
+      aff[i * nc + j] = aff[j * nc + i] = std::exp(-d2);
+    }
+  }
+  std::vector<double> deg(nc, 0.0);
+  for (size_t i = 0; i < nc; ++i)
+    for (size_t j = 0; j < nc; ++j) deg[i] += aff[i * nc + j];
+  std::vector<double> inv(nc);
+  for (size_t i = 0; i < nc; ++i) inv[i] = deg[i] > 0.0 ? 1.0 / std::sqrt(deg[i]) : 0.0;
+  std::vector<double> lap(nc * nc, 0.0);
+  for (size_t i = 0; i < nc; ++i) lap[i * nc + i] = 1.0;
+  for (size_t i = 0; i < nc; ++i) {
+    if (deg[i] <= 0.0) continue;
+    for (size_t j = 0; j < nc; ++j) {
+      if (deg[j] <= 0.0) continue;
+      const double w = aff[i * nc + j];
+      if (w > 0.0) lap[i * nc + j] -= w * inv[i] * inv[j];
+    }
+  }
+  std::vector<double> ev, vec;
+  symmetric_eigen(lap, static_cast<int>(nc), ev, vec);
+  std::vector<int> order(nc);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+  if (order.size() <= static_cast<size_t>(dim) + 1) return false;
+  layout.assign(nc * dim, 0.f);
+  for (int c = 0; c < dim; ++c)
+    for (size_t r = 0; r < nc; ++r) layout[r * dim + c] = static_cast<float>(vec[r * nc + order[c + 1]]);
+  // noisy_scale_coords(seed ^ 0xC85E7D6ADA318F21, 10, 1e-4)
+  float mx = 0.f;
+  for (float v : layout) mx = std::max(mx, std::fabs(v));
+  const float expansion = mx > 0.f ? 10.0f / mx : 1.0f;
+  SmallRng rng(seed ^ 0xC85E7D6ADA318F21ULL);
+  for (auto& v : layout) v = v * expansion + static_cast<float>(rng.normal(1e-4));
+  for (float v : layout)
+    if (!std::isfinite(v)) return false;
+  return true;
+}
+
+}  // namespace
+
+int64_t connected_components_device(const Graph& g, std::vector<int32_t>& labels_host,
+                                    SpectralWorkspace& ws, hipStream_t s) {
+  const int64_t n = g.n;
+  ws.label.ensure(n);
+  ws.flag.ensure(1);
+  hipLaunchKernelGGL(init_labels_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, ws.label.get(), n);
+  KERNEL_CHECK();
+  for (int it = 0; it < 10000; ++it) {
+    HIP_CHECK(hipMemsetAsync(ws.flag.get(), 0, sizeof(int32_t), s));
+    if (g.n_edges > 0)
+      hipLaunchKernelGGL(hook_kernel, dim3(blocks_for(g.n_edges, 256)), dim3(256), 0, s,
+                         g.head.get(), g.tail.get(), g.n_edges, ws.label.get(), ws.flag.get());
+    hipLaunchKernelGGL(jump_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, ws.label.get(), n);
+    KERNEL_CHECK();
+    int32_t changed = 0;
+    HIP_CHECK(hipMemcpyAsync(&changed, ws.flag.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (!changed) break;
+  }
+  labels_host.resize(n);
+  HIP_CHECK(hipMemcpyAsync(labels_host.data(), ws.label.get(), sizeof(int32_t) * n,
+                           hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  int64_t nc = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (labels_host[i] == i) ++nc;
+  return nc;
+}
+
+bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const Graph& g,
+                          uint64_t seed, float* y, SpectralWorkspace& ws, hipStream_t s,
+                          SpectralReport* report) {
+  maybe_inject_fault("init");
+  SpectralReport rep;
+  if (n <= dim + 1 || g.n_edges == 0) return false;
+  if ((dim + 1) * 6 != kSub)
+    throw_invalid("spectral init on the GPU supports n_components = 2; use init='random'");
+  // components, ordered by smallest member, members ascending
+  std::vector<int32_t> labels;
+  connected_components_device(g, labels, ws, s);
+  std::vector<int64_t> comp_index(n, -1);
+  std::vector<Comp> comps;
+  for (int64_t v = 0; v < n; ++v) {
+    const int32_t root = labels[v];
+    if (comp_index[root] < 0) {
+      comp_index[root] = static_cast<int64_t>(comps.size());
+      comps.emplace_back();
+    }
+    comps[comp_index[root]].members.push_back(static_cast<int32_t>(v));
+  }
+  const int64_t nc = static_cast<int64_t>(comps.size());
+  rep.n_components = nc;
+  const bool multi = nc > 1;
+
+  // per-component plan: 0 random, 1 dense host, 2 iterative GPU
+  std::vector<int> plan(nc, 2);
+  std::vector<uint64_t> comp_seed(nc, seed);
+  std::vector<float> layout;  // anchors
+  std::vector<float> range(nc, 1.f);
+  if (multi) {
+    // centroids on the GPU (f64), meta layout on the host
+    std::vector<int32_t> perm_all;
+    std::vector<int32_t> items;
+    for (int64_t c = 0; c < nc; ++c) {
+      const int32_t start = static_cast<int32_t>(perm_all.size());
+      perm_all.insert(perm_all.end(), comps[c].members.begin(), comps[c].members.end());
+      const int32_t end = static_cast<int32_t>(perm_all.size());
+      for (int32_t p = start; p < end; p += kItemRows) {
+        items.push_back(static_cast<int32_t>(c));
+        items.push_back(p);
+        items.push_back(std::min(end, p + kItemRows));
+      }
+    }
+    ws.perm.ensure(perm_all.size());
+    ws.items.ensure(items.size());
+    ws.cent.ensure(static_cast<size_t>(nc) * d);
+    HIP_CHECK(hipMemcpyAsync(ws.perm.get(), perm_all.data(), perm_all.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(ws.items.get(), items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemsetAsync(ws.cent.get(), 0, sizeof(double) * nc * d, s));
+    hipLaunchKernelGGL(centroid_kernel, dim3(static_cast<unsigned>(items.size() / 3)), dim3(256), 0, s,
+                       ws.items.get(), ws.perm.get(), x, static_cast<int>(d), ws.cent.get());
+    KERNEL_CHECK();
+    std::vector<double> csum(static_cast<size_t>(nc) * d);
+    HIP_CHECK(hipMemcpyAsync(csum.data(), ws.cent.get(), sizeof(double) * nc * d, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<std::vector<double>> cent(nc, std::vector<double>(d));
+    for (int64_t c = 0; c < nc; ++c) {
+      const double sc = 1.0 / static_cast<double>(comps[c].members.size());
+      for (int64_t f = 0; f < d; ++f) cent[c][f] = csum[c * d + f] * sc;
+    }
+    if (!meta_layout(cent, dim, seed ^ 0xA13F52A92D4CB801ULL, layout)) return false;
+    for (int64_t c = 0; c < nc; ++c) {
+      float r = INFINITY;
+      for (int64_t o = 0; o < nc; ++o) {
+        if (o == c) continue;
+        float sq = 0.f;
+        for (int k = 0; k < dim; ++k) {
+          const float df = layout[c * dim + k] - layout[o * dim + k];
+          sq += df * df;
+        }
+        const float dd = std::sqrt(sq);
+        if (dd > 0.f) r = std::min(r, dd / 2.0f);
+      }
+      if (!std::isfinite(r) || r <= 0.f) r = 1.f;
+      range[c] = r;
+      const size_t cs = comps[c].members.size();
+      if (cs < static_cast<size_t>(2 * dim) || cs <= static_cast<size_t>(dim) + 1) {
+        plan[c] = 0;
+      } else {
+        comp_seed[c] = seed ^ ((static_cast<uint64_t>(c) + 1) * 0xBF58476D1CE4E5B9ULL);
+        plan[c] = cs >= 128 ? 2 : 1;
+      }
+    }
+  } else {
+    layout.assign(dim, 0.f);
+    const int64_t thr = d <= 8 ? 2049 : 0;
+    plan[0] = n >= thr ? 2 : 1;
+  }
+
+  std::vector<float> y_host_rows;     // rows produced on the host
+  std::vector<int32_t> y_host_index;  // their global row ids
+  std::vector<std::tuple<int32_t, int32_t, float>> all_edges;
+  bool edges_loaded = false;
+  auto load_edges = [&]() {
+    if (edges_loaded) return;
+    std::vector<int32_t> hh(g.n_edges), tt(g.n_edges);
+    std::vector<float> ww(g.n_edges);
+    HIP_CHECK(hipMemcpyAsync(hh.data(), g.head.get(), 4 * g.n_edges, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(tt.data(), g.tail.get(), 4 * g.n_edges, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(ww.data(), g.weight.get(), 4 * g.n_edges, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    all_edges.resize(g.n_edges);
+    for (int64_t e = 0; e < g.n_edges; ++e) all_edges[e] = {hh[e], tt[e], ww[e]};
+    edges_loaded = true;
+  };
+  auto dense_component = [&](int64_t c, std::vector<float>& coords) {
+    load_edges();
+    const auto& mem = comps[c].members;
+    std::vector<int32_t> local(n, -1);
+    for (size_t i = 0; i < mem.size(); ++i) local[mem[i]] = static_cast<int32_t>(i);
+    std::vector<std::tuple<int32_t, int32_t, float>> ce;
+    for (const auto& e : all_edges) {
+      const int32_t h = std::get<0>(e), t = std::get<1>(e);
+      if (local[h] >= 0 && local[t] >= 0) ce.emplace_back(local[h], local[t], std::get<2>(e));
+    }
+    return dense_embedding_from_edges(static_cast<int64_t>(mem.size()), dim, ce, coords);
+  };
+
+  // host-side components (random / dense)
+  for (int64_t c = 0; c < nc; ++c) {
+    if (plan[c] == 2) continue;
+    const auto& mem = comps[c].members;
+    std::vector<float> lc;
+    if (plan[c] == 0) {
+      lc = random_init(static_cast<int64_t>(mem.size()), dim,
+                       seed ^ ((static_cast<uint64_t>(c) + 1) * 0x9E3779B9ULL), -range[c], range[c]);
+      ++rep.n_random;
+    } else {
+      if (!dense_component(c, lc)) return false;
+      ++rep.n_dense;
+      float mx = 0.f;
+      for (float v : lc) mx = std::max(mx, std::fabs(v));
+      if (multi) {
+        const float expansion = mx > 0.f ? range[c] / mx : 1.f;
+        for (auto& v : lc) v *= expansion;
+        SmallRng rng(seed ^ ((static_cast<uint64_t>(c) + 1) * 0x94D049BBULL));
+        for (auto& v : lc) v += static_cast<float>(rng.normal(1e-4));
+      } else {
+        const float expansion = mx > 0.f ? 10.0f / mx : 1.f;
+        SmallRng rng(seed ^ 0x5DEECE66DULL);
+        for (auto& v : lc) v = v * expansion + static_cast<float>(rng.normal(1e-4));
+      }
+    }
+    for (size_t i = 0; i < mem.size(); ++i) {
+      y_host_index.push_back(mem[i]);
+      for (int k = 0; k < dim; ++k) y_host_rows.push_back(lc[i * dim + k] + layout[c * dim + k]);
+    }
+  }
+
+  // GPU iterative components
+  std::vector<int64_t> gpu_comps;
+  for (int64_t c = 0; c < nc; ++c)
+    if (plan[c] == 2) gpu_comps.push_back(c);
+  const int n_slots = static_cast<int>(gpu_comps.size());
+  if (n_slots > 0) {
+    std::vector<int32_t> perm, slot_of_pos, local_of_pos, items;
+    std::vector<uint64_t> seeds(n_slots), noise_seeds(n_slots);
+    for (int sl = 0; sl < n_slots; ++sl) {
+      const int64_t c = gpu_comps[sl];
+      const int32_t start = static_cast<int32_t>(perm.size());
+      const auto& mem = comps[c].members;
+      if (static_cast<int64_t>(mem.size()) <= kSub) return false;
+      for (size_t i = 0; i < mem.size(); ++i) {
+        perm.push_back(mem[i]);
+        slot_of_pos.push_back(sl);
+        local_of_pos.push_back(static_cast<int32_t>(i));
+      }
+      const int32_t end = static_cast<int32_t>(perm.size());
+      for (int32_t p = start; p < end; p += kItemRows) {
+        items.push_back(sl);
+        items.push_back(p);
+        items.push_back(std::min(end, p + kItemRows));
+      }
+      seeds[sl] = comp_seed[c];
+      noise_seeds[sl] = multi ? (seed ^ ((static_cast<uint64_t>(c) + 1) * 0x94D049BBULL))
+                              : (seed ^ 0x5DEECE66DULL);
+    }
+    const int64_t rows = static_cast<int64_t>(perm.size());
+    const int n_items = static_cast<int>(items.size() / 3);
+    ws.perm.ensure(rows);
+    ws.slot_of.ensure(2 * rows);
+    ws.items.ensure(items.size());
+    ws.seeds.ensure(2 * n_slots);
+    HIP_CHECK(hipMemcpyAsync(ws.perm.get(), perm.data(), 4 * rows, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(ws.slot_of.get(), slot_of_pos.data(), 4 * rows, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(ws.slot_of.get() + rows, local_of_pos.data(), 4 * rows, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(ws.items.get(), items.data(), 4 * items.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(ws.seeds.get(), seeds.data(), 8 * n_slots, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(ws.seeds.get() + n_slots, noise_seeds.data(), 8 * n_slots, hipMemcpyHostToDevice, s));
+    const int32_t* d_perm = ws.perm.get();
+    const int32_t* d_slot = ws.slot_of.get();
+    const int32_t* d_local = ws.slot_of.get() + rows;
+
+    ws.deg.ensure(n);
+    ws.inv.ensure(n);
+    ws.y.ensure(static_cast<size_t>(n) * kSub);
+    ws.z.ensure(static_cast<size_t>(n) * kSub);
+    ws.gram.ensure(static_cast<size_t>(n_slots) * kSub * kSub);
+    ws.rmat.ensure(static_cast<size_t>(n_slots) * kSub * kSub);
+    ws.fail.ensure(n_slots);
+    HIP_CHECK(hipMemsetAsync(ws.fail.get(), 0, 4 * n_slots, s));
+    HIP_CHECK(hipMemsetAsync(ws.y.get(), 0, sizeof(double) * n * kSub, s));
+    HIP_CHECK(hipMemsetAsync(ws.z.get(), 0, sizeof(double) * n * kSub, s));
+    hipLaunchKernelGGL(degree_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(),
+                       g.weight.get(), n, ws.deg.get(), ws.inv.get());
+    hipLaunchKernelGGL(start_block_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
+                       rows, d_slot, d_local, ws.seeds.get(), ws.deg.get(), ws.y.get());
+    KERNEL_CHECK();
+    auto orthonormalize = [&](double* blk) {
+      for (int pass = 0; pass < 2; ++pass) {
+        HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * n_slots * kSub * kSub, s));
+        hipLaunchKernelGGL(gram_kernel, dim3(n_items), dim3(256), 0, s, ws.items.get(), d_perm,
+                           blk, blk, ws.gram.get());
+        hipLaunchKernelGGL(cholesky_kernel, dim3(blocks_for(n_slots, 64)), dim3(64), 0, s,
+                           ws.gram.get(), ws.rmat.get(), n_slots, ws.fail.get());
+        hipLaunchKernelGGL(apply_rinv_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
+                           rows, d_slot, ws.rmat.get(), blk);
+        KERNEL_CHECK();
+      }
+    };
+    double* yb = ws.y.get();
+    double* zb = ws.z.get();
+    orthonormalize(yb);
+    for (int it = 0; it < kIters; ++it) {
+      hipLaunchKernelGGL(shifted_mul_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
+                         rows, g.row_ptr.get(), g.tail.get(), g.weight.get(), ws.inv.get(), yb, zb);
+      KERNEL_CHECK();
+      orthonormalize(zb);
+      std::swap(yb, zb);
+    }
+    hipLaunchKernelGGL(shifted_mul_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
+                       g.row_ptr.get(), g.tail.get(), g.weight.get(), ws.inv.get(), yb, zb);
+    HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * n_slots * kSub * kSub, s));
+    hipLaunchKernelGGL(gram_kernel, dim3(n_items), dim3(256), 0, s, ws.items.get(), d_perm, yb, zb,
+                       ws.gram.get());
+    KERNEL_CHECK();
+    std::vector<double> ritz(static_cast<size_t>(n_slots) * kSub * kSub);
+    std::vector<int32_t> fail(n_slots);
+    HIP_CHECK(hipMemcpyAsync(ritz.data(), ws.gram.get(), 8 * ritz.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(fail.data(), ws.fail.get(), 4 * n_slots, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<double> vmat(static_cast<size_t>(n_slots) * kSub * dim);
+    std::vector<float> scale(n_slots), anchor(static_cast<size_t>(n_slots) * dim);
+    for (int sl = 0; sl < n_slots; ++sl) {
+      const int64_t c = gpu_comps[sl];
+      if (fail[sl]) {
+        // lib.rs:4339-4348: a failed subspace iteration falls back to the
+        // dense Laplacian only up to 4096 rows
+        const auto& mem = comps[c].members;
+        if (mem.size() > 4096) return false;
+        std::vector<float> lc;
+        if (!dense_component(c, lc)) return false;
+        float mxa = 0.f;
+        for (float v : lc) mxa = std::max(mxa, std::fabs(v));
+        if (multi) {
+          const float expansion = mxa > 0.f ? range[c] / mxa : 1.f;
+          SmallRng rng(seed ^ ((static_cast<uint64_t>(c) + 1) * 0x94D049BBULL));
+          for (auto& v : lc) v = v * expansion + static_cast<float>(rng.normal(1e-4));
+        } else {
+          const float expansion = mxa > 0.f ? 10.0f / mxa : 1.f;
+          SmallRng rng(seed ^ 0x5DEECE66DULL);
+          for (auto& v : lc) v = v * expansion + static_cast<float>(rng.normal(1e-4));
+        }
+        for (size_t i = 0; i < mem.size(); ++i) {
+          y_host_index.push_back(mem[i]);
+          for (int k = 0; k < dim; ++k) y_host_rows.push_back(lc[i * dim + k] + layout[c * dim + k]);
+        }
+        ++rep.n_dense;
+        for (int b = 0; b < kSub; ++b)
+          for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * kSub + b) * dim + k] = 0.0;
+        continue;
+      }
+      std::vector<double> a(ritz.begin() + static_cast<size_t>(sl) * kSub * kSub,
+                            ritz.begin() + static_cast<size_t>(sl + 1) * kSub * kSub);
+      for (int i = 0; i < kSub; ++i)
+        for (int j = 0; j < i; ++j) a[i * kSub + j] = a[j * kSub + i];
+      std::vector<double> ev, vec;
+      symmetric_eigen(a, kSub, ev, vec);
+      std::vector<int> order(kSub);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return ev[q] < ev[p]; });
+      for (int b = 0; b < kSub; ++b)
+        for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * kSub + b) * dim + k] = vec[b * kSub + order[k + 1]];
+      for (int k = 0; k < dim; ++k) anchor[sl * dim + k] = layout[c * dim + k];
+      ++rep.n_iterative;
+    }
+    ws.vmat.ensure(vmat.size());
+    ws.maxabs.ensure(n_slots);
+    ws.scale.ensure(n_slots);
+    ws.anchor.ensure(anchor.size());
+    DevBuf<float> coords(static_cast<size_t>(n) * dim);
+    HIP_CHECK(hipMemcpyAsync(ws.vmat.get(), vmat.data(), 8 * vmat.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemsetAsync(ws.maxabs.get(), 0, 4 * n_slots, s));
+    hipLaunchKernelGGL(project_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
+                       d_slot, yb, ws.vmat.get(), dim, coords.get(), ws.maxabs.get());
+    KERNEL_CHECK();
+    std::vector<float> mx(n_slots);
+    HIP_CHECK(hipMemcpyAsync(mx.data(), ws.maxabs.get(), 4 * n_slots, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int sl = 0; sl < n_slots; ++sl) {
+      const float target = multi ? range[gpu_comps[sl]] : 10.0f;
+      scale[sl] = mx[sl] > 0.f ? target / mx[sl] : 1.f;
+    }
+    HIP_CHECK(hipMemcpyAsync(ws.scale.get(), scale.data(), 4 * n_slots, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(ws.anchor.get(), anchor.data(), 4 * anchor.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(finish_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows, d_slot,
+                       d_local, ws.seeds.get() + n_slots, ws.scale.get(), ws.anchor.get(), dim,
+                       coords.get(), y);
+    KERNEL_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  if (!y_host_index.empty()) {
+    DevBuf<int32_t> ridx(y_host_index.size());
+    DevBuf<float> rv(y_host_rows.size());
+    HIP_CHECK(hipMemcpyAsync(ridx.get(), y_host_index.data(), 4 * y_host_index.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(rv.get(), y_host_rows.data(), 4 * y_host_rows.size(), hipMemcpyHostToDevice, s));
+    const int64_t cnt = static_cast<int64_t>(y_host_index.size());
+    hipLaunchKernelGGL(scatter_rows_kernel, dim3(blocks_for(cnt * dim, 256)), dim3(256), 0, s, ridx.get(),
+                       cnt, dim, rv.get(), y);
+    KERNEL_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  rep.used_spectral = true;
+  if (report) *report = rep;
+  return true;
+}
+
+}  // namespace umapx
