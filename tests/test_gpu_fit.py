@@ -9,12 +9,20 @@ pytestmark = pytest.mark.gpu
 
 
 def test_c1_fit_parity_with_oracle(gpu, oracle):
+    # i.i.d. Gaussian rows have no structure, so single-seed trustworthiness
+    # moves by ~0.005; the parity bar (0.01) is applied to 3-seed means
     x = datasets.c1()
-    y = Umap(n_epochs=200, ann_mode="exact").fit_transform(x)
-    yo = oracle.fit(x, oracle.OracleParams(n_epochs=200)).embedding
-    assert y.shape == (1000, 2) and y.dtype == np.float32 and np.isfinite(y).all()
-    assert trustworthiness(x, y) >= trustworthiness(x, yo) - 0.01
-    assert knn_preservation(x, y) >= knn_preservation(x, yo) - 0.01
+    tg, to, kg, ko = [], [], [], []
+    for seed in (42, 7, 11):
+        y = Umap(n_epochs=200, ann_mode="exact", random_seed=seed).fit_transform(x)
+        yo = oracle.fit(x, oracle.OracleParams(n_epochs=200, random_seed=seed)).embedding
+        assert y.shape == (1000, 2) and y.dtype == np.float32 and np.isfinite(y).all()
+        tg.append(trustworthiness(x, y))
+        to.append(trustworthiness(x, yo))
+        kg.append(knn_preservation(x, y))
+        ko.append(knn_preservation(x, yo))
+    assert np.mean(tg) >= np.mean(to) - 0.01, (tg, to)
+    assert np.mean(kg) >= np.mean(ko) - 0.01, (kg, ko)
 
 
 def test_clustered_fit_parity_and_profile(gpu, oracle):

@@ -68,10 +68,15 @@ def test_densmap_term_changes_local_scale(gpu, oracle, graph):
     start = oracle.density_start_epoch(200, 0.3)
     yo = oracle.optimize_layout(y0, h, t, w, 200, a, b, radii=rad, dens_start=start)
     yg = capi.optimize(y0, h, t, w, 200, a, b, radii=rad, dens_start=start)
+    yg0 = capi.optimize(y0, h, t, w, 200, a, b)
     assert np.isfinite(yg).all()
-    # the density term ties embedding radii to the original ones: compare the
-    # correlation between original and embedding radii with the oracle's
+    # the density term ties embedding radii to the original ones: the GPU
+    # run must show the effect (vs its own run without the term) at a size
+    # comparable to the oracle's; this correlation moves by ~0.01 between
+    # seeds on the CPU and more between Hogwild runs
     ro = np.corrcoef(rad, oracle.embedding_radii(yo, idx))[0, 1]
     rg = np.corrcoef(rad, oracle.embedding_radii(yg, idx))[0, 1]
-    assert rg >= ro - 0.05, (rg, ro)
+    rg0 = np.corrcoef(rad, oracle.embedding_radii(yg0, idx))[0, 1]
+    assert rg >= rg0 + 0.15, (rg, rg0)
+    assert rg >= ro - 0.1, (rg, ro)
     assert trustworthiness(x, yg) >= trustworthiness(x, yo) - 0.01

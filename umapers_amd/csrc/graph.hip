@@ -229,33 +229,31 @@ __global__ void scatter_incoming_kernel(const int32_t* __restrict__ idx,
   in_w[p] = w;
 }
 
-// One thread per row: sort the outgoing and incoming lists by neighbour index
-// and merge them (lib.rs:3611-3646 arithmetic).
-__global__ void union_rows_kernel(const int32_t* __restrict__ idx, const float* __restrict__ wdir,
-                                  int64_t n, int k, const int32_t* __restrict__ in_off,
-                                  int32_t* __restrict__ in_src, float* __restrict__ in_w,
-                                  float mix, int32_t* __restrict__ tmp_col,
-                                  float* __restrict__ tmp_w, int32_t* __restrict__ ucount,
-                                  uint32_t* __restrict__ wmax_bits) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int32_t oc[64];
-  float ow[64];
+// Serial merge of one row (used for rows whose outgoing + incoming lists do
+// not fit in one wavefront): sort both lists by neighbour index in place and
+// merge them with the arithmetic of lib.rs:3611-3646.
+__device__ void union_row_serial(const int32_t* __restrict__ idx, const float* __restrict__ wdir,
+                                 int64_t i, int k, int32_t b0, int32_t b1,
+                                 int32_t* __restrict__ in_src, float* __restrict__ in_w,
+                                 float mix, int32_t* __restrict__ tmp_col,
+                                 float* __restrict__ tmp_w, int32_t* __restrict__ ucount,
+                                 uint32_t* __restrict__ wmax_bits, int32_t* __restrict__ out_c,
+                                 float* __restrict__ out_w) {
+  // outgoing entries go to the scratch area out_c/out_w (k slots per row)
   int no = 0;
   for (int c = 0; c < k; ++c) {
     const float w = wdir[i * k + c];
     const int32_t j = idx[i * k + c];
     if (!(w > 0.f) || j == i) continue;
     int p = no++;
-    while (p > 0 && oc[p - 1] > j) {
-      oc[p] = oc[p - 1];
-      ow[p] = ow[p - 1];
+    while (p > 0 && out_c[p - 1] > j) {
+      out_c[p] = out_c[p - 1];
+      out_w[p] = out_w[p - 1];
       --p;
     }
-    oc[p] = j;
-    ow[p] = w;
+    out_c[p] = j;
+    out_w[p] = w;
   }
-  const int32_t b0 = in_off[i], b1 = in_off[i + 1];
   for (int32_t a = b0 + 1; a < b1; ++a) {
     const int32_t sv = in_src[a];
     const float wv = in_w[a];
@@ -269,22 +267,21 @@ __global__ void union_rows_kernel(const int32_t* __restrict__ idx, const float* 
     in_w[p] = wv;
   }
   const int64_t base = i * k + b0;
-  int cnt = 0;
-  int po = 0;
+  int cnt = 0, po = 0;
   int32_t pi = b0;
   float local_max = 0.f;
   while (po < no || pi < b1) {
     int32_t j;
     float wij = 0.f, wji = 0.f;
-    if (pi >= b1 || (po < no && oc[po] < in_src[pi])) {
-      j = oc[po];
-      wij = ow[po++];
-    } else if (po >= no || in_src[pi] < oc[po]) {
+    if (pi >= b1 || (po < no && out_c[po] < in_src[pi])) {
+      j = out_c[po];
+      wij = out_w[po++];
+    } else if (po >= no || in_src[pi] < out_c[po]) {
       j = in_src[pi];
       wji = in_w[pi++];
     } else {
-      j = oc[po];
-      wij = ow[po++];
+      j = out_c[po];
+      wij = out_w[po++];
       wji = in_w[pi++];
     }
     const float prod = fmul(wij, wji);
@@ -298,6 +295,90 @@ __global__ void union_rows_kernel(const int32_t* __restrict__ idx, const float* 
   }
   ucount[i] = cnt;
   if (cnt > 0) atomicMax(wmax_bits, __float_as_uint(local_max));
+}
+
+// One wavefront per row: the outgoing (k) and incoming entries of the row are
+// loaded one per lane with key 2*col + (incoming), sorted by a register
+// bitonic network, and adjacent equal columns are combined with the
+// arithmetic of lib.rs:3611-3646; surviving entries are compacted in column
+// order.  Rows with more than 64 entries take union_row_serial.
+__global__ __launch_bounds__(256) void union_rows_kernel(
+    const int32_t* __restrict__ idx, const float* __restrict__ wdir, int64_t n, int k,
+    const int32_t* __restrict__ in_off, int32_t* __restrict__ in_src, float* __restrict__ in_w,
+    float mix, int32_t* __restrict__ tmp_col, float* __restrict__ tmp_w,
+    int32_t* __restrict__ ucount, uint32_t* __restrict__ wmax_bits,
+    int32_t* __restrict__ serial_c, float* __restrict__ serial_w) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int32_t b0 = in_off[i], b1 = in_off[i + 1];
+  const int m_in = b1 - b0;
+  if (k + m_in > 64) {
+    if (lane == 0)
+      union_row_serial(idx, wdir, i, k, b0, b1, in_src, in_w, mix, tmp_col, tmp_w, ucount,
+                       wmax_bits, serial_c + i * k, serial_w + i * k);
+    return;
+  }
+  int key = INT_MAX;
+  float w = 0.f;
+  if (lane < k) {
+    const float wv = wdir[i * k + lane];
+    const int32_t j = idx[i * k + lane];
+    if (wv > 0.f && j != i) {
+      key = 2 * j;
+      w = wv;
+    }
+  } else if (lane < k + m_in) {
+    key = 2 * in_src[b0 + lane - k] + 1;
+    w = in_w[b0 + lane - k];
+  }
+  // bitonic sort ascending by key (keys are unique except INT_MAX padding)
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int ok = __shfl_xor(key, stride);
+      const float ow = __shfl_xor(w, stride);
+      const bool up = (lane & size) == 0;
+      const bool lower = (lane & stride) == 0;
+      const bool take = (lower == up) ? (ok < key) : (ok > key);
+      if (take) {
+        key = ok;
+        w = ow;
+      }
+    }
+  }
+  const int col = key >> 1;
+  const int next_key = __shfl_down(key, 1);
+  const float next_w = __shfl_down(w, 1);
+  const int prev_key = __shfl_up(key, 1);
+  const bool valid = key != INT_MAX;
+  const bool first = valid && (lane == 0 || (prev_key >> 1) != col || prev_key == INT_MAX);
+  float wij = 0.f, wji = 0.f;
+  if ((key & 1) == 0) {
+    wij = w;
+    if (lane < 63 && next_key != INT_MAX && (next_key >> 1) == col) wji = next_w;
+  } else {
+    wji = w;
+  }
+  const float prod = fmul(wij, wji);
+  const float sym = fadd(fmul(mix, fsub(fadd(wij, wji), prod)), fmul(fsub(1.f, mix), prod));
+  const bool keep = first && sym > 0.f;
+  const unsigned long long bal = __ballot(keep);
+  const int pos = __popcll(bal & ((1ULL << lane) - 1ULL));
+  const int64_t base = i * k + b0;
+  if (keep) {
+    tmp_col[base + pos] = col;
+    tmp_w[base + pos] = sym;
+  }
+  float mx = keep ? sym : 0.f;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  if (lane == 0) {
+    const int cnt = __popcll(bal);
+    ucount[i] = cnt;
+    if (cnt > 0) atomicMax(wmax_bits, __float_as_uint(mx));
+  }
 }
 
 // Rows of a bipartite graph keep their kNN order (lib.rs:3562-3595).
@@ -550,9 +631,12 @@ void fuzzy_graph_device(const int32_t* idx, const float* dist, const float* sigm
   ws.umax.ensure(1);
   HIP_CHECK(hipMemsetAsync(ws.umax.get(), 0, sizeof(uint32_t), s));
   if (k > 64) throw_invalid("n_neighbors must be <= 64 on the GPU path");
-  hipLaunchKernelGGL(union_rows_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, s, idx,
+  ws.serial_c.ensure(nk);
+  ws.serial_w.ensure(nk);
+  hipLaunchKernelGGL(union_rows_kernel, dim3(blocks_for(n, 4)), dim3(256), 0, s, idx,
                      ws.wdir.get(), n, k, ws.in_off.get(), ws.in_src.get(), ws.in_w.get(), mix,
-                     ws.tmp_col.get() , ws.tmp_w.get(), ws.ucount.get(), ws.umax.get());
+                     ws.tmp_col.get(), ws.tmp_w.get(), ws.ucount.get(), ws.umax.get(),
+                     ws.serial_c.get(), ws.serial_w.get());
   KERNEL_CHECK();
   hipLaunchKernelGGL(tmp_offsets_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                      ws.in_off.get(), n, k, ws.tmp_off.get());

@@ -31,7 +31,8 @@ struct GraphWorkspace {
   DevBuf<float> wdir;       // n*k directed weights
   DevBuf<int32_t> indeg, in_off, fill, in_src;
   DevBuf<float> in_w;
-  DevBuf<int32_t> tmp_col, ucount, kcount, tmp_off;
+  DevBuf<int32_t> tmp_col, ucount, kcount, tmp_off, serial_c;
+  DevBuf<float> serial_w;
   DevBuf<float> tmp_w;
   DevBuf<int32_t> flag;
   DevBuf<double> dsum;

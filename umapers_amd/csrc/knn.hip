@@ -26,10 +26,8 @@ namespace {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kWaves = 4;        // waves per block
 constexpr int kRowsPerWave = 32; // query rows owned by a wave
-constexpr int kRefTile = 64;     // reference rows staged per iteration
-constexpr int kBuf = 32;         // passes buffered per row between merges
+constexpr int kWaves = 4;        // waves per block
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
   uint32_t u = __float_as_uint(f);
@@ -122,85 +120,213 @@ __global__ void prep_rows_kernel(const float* __restrict__ x, int64_t n, int d, 
 }
 
 // ---------------------------------------------------------- candidates ----
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// One 16-byte fragment vector per lane feeds STEPS MFMA steps.  Fragments
+// are passed by value (never through pointers) so they stay in VGPRs.
 template <typename T>
 struct MfmaOp;
 
 template <>
 struct MfmaOp<float> {
-  static constexpr int kElems = 1;   // elements per lane per MFMA step
-  __device__ static inline floatx16 run(const float* a, const float* b, floatx16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x2f32(*a, *b, c, 0, 0, 0);
+  static constexpr int kSteps = 4;  // 4 x (32x32x2) per 16-byte vector
+  template <int ST>
+  __device__ static inline floatx16 run(u32x4 a, u32x4 b, floatx16 c) {
+    const floatx4 av = __builtin_bit_cast(floatx4, a);
+    const floatx4 bv = __builtin_bit_cast(floatx4, b);
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(av[ST], bv[ST], c, 0, 0, 0);
   }
 };
 
 template <>
 struct MfmaOp<uint16_t> {
-  static constexpr int kElems = 8;
-  __device__ static inline floatx16 run(const uint16_t* a, const uint16_t* b, floatx16 c) {
-    const bf16x8 av = *reinterpret_cast<const bf16x8*>(a);
-    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(b);
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
+  static constexpr int kSteps = 1;  // 1 x (32x32x16) per 16-byte vector
+  template <int ST>
+  __device__ static inline floatx16 run(u32x4 a, u32x4 b, floatx16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
   }
 };
 
-struct __align__(16) Vec16 {
-  uint32_t w[4];
+template <typename T, int ST>
+struct MfmaSteps {
+  __device__ static inline floatx16 run(u32x4 a, u32x4 b, floatx16 c) {
+    c = MfmaSteps<T, ST - 1>::run(a, b, c);
+    return MfmaOp<T>::template run<ST - 1>(a, b, c);
+  }
+};
+template <typename T>
+struct MfmaSteps<T, 0> {
+  __device__ static inline floatx16 run(u32x4, u32x4, floatx16 c) { return c; }
 };
 
-// LDS layout sizes (bytes) for one block.
-template <typename T, int DPAD, int KC>
+// LDS layout (bytes) of one block: two reference tiles (double buffer),
+// their norms, and per query row the sorted top-KC list, a pass buffer and
+// counters.
+template <typename T, int DPAD, int KC, int TILE, int BUF>
 struct CandLayout {
   static constexpr int kRowBytes = DPAD * static_cast<int>(sizeof(T)) + 16;
-  static constexpr int kTileBytes = kRefTile * kRowBytes;
-  static constexpr int kNormBytes = kRefTile * 4;
-  static constexpr int kListBytes = kWaves * kRowsPerWave * KC * 8;
-  static constexpr int kBufBytes = kWaves * kRowsPerWave * kBuf * 8;
-  static constexpr int kMetaBytes = kWaves * kRowsPerWave * 8 + kWaves * 4;
-  static constexpr int kTotal = kTileBytes + kNormBytes + kListBytes + kBufBytes + kMetaBytes;
+  static constexpr int kTileBytes = TILE * kRowBytes;
+  static constexpr int kNormBytes = TILE * 4;
+  static constexpr int kRows = kWaves * kRowsPerWave;
+  static constexpr int kListBytes = kRows * KC * 8;
+  static constexpr int kBufBytes = kRows * BUF * 8;
+  static constexpr int kMetaBytes = kRows * 8;
+  static constexpr int kTiles = 2 * (kTileBytes + kNormBytes);
+  static constexpr int kTotal = kTiles + kListBytes + kBufBytes + kMetaBytes;
 };
+constexpr int kLdsBudget = 160 * 1024;
+
+// Merge the cnt buffered candidates of one row into its sorted top-KC list
+// (wave-wide; the wave owns the row).  Few entries: one insertion each, a
+// ballot finds the slot and the tail shifts up one lane.  Otherwise the list
+// and the buffer are sorted together by a bitonic network over (score,
+// index), two entries per lane.  Returns the row's new threshold.
+template <int KC>
+__device__ __noinline__ float merge_row(float* __restrict__ ld_row, int32_t* __restrict__ li_row,
+                                           const float* __restrict__ bd_row,
+                                           const int32_t* __restrict__ bi_row, int cnt, int lane) {
+  if (cnt <= 4) {
+    float ld = lane < KC ? ld_row[lane] : INFINITY;
+    int32_t li = lane < KC ? li_row[lane] : INT_MAX;
+    for (int e = 0; e < cnt; ++e) {
+      const float cd = bd_row[e];
+      const int32_t ci = bi_row[e];
+      const bool less = lane < KC && (ld < cd || (ld == cd && li < ci));
+      const int pos = __popcll(__ballot(less));
+      if (pos < KC) {
+        const float ud = __shfl_up(ld, 1);
+        const int32_t ui = __shfl_up(li, 1);
+        if (lane > pos) {
+          ld = ud;
+          li = ui;
+        }
+        if (lane == pos) {
+          ld = cd;
+          li = ci;
+        }
+      }
+    }
+    if (lane < KC) {
+      ld_row[lane] = ld;
+      li_row[lane] = li;
+    }
+    return __shfl(ld, KC - 1);
+  }
+  // 128 slots: element j = 2*lane + q.  List entries occupy j < KC, buffered
+  // ones j in [KC, KC + cnt).
+  float d[2];
+  int32_t id[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int j = 2 * lane + q;
+    d[q] = INFINITY;
+    id[q] = INT_MAX;
+    if (j < KC) {
+      d[q] = ld_row[j];
+      id[q] = li_row[j];
+    } else if (j < KC + cnt) {
+      d[q] = bd_row[j - KC];
+      id[q] = bi_row[j - KC];
+    }
+  }
+  auto less = [](float ad, int32_t ai, float bd, int32_t bi) {
+    return ad < bd || (ad == bd && ai < bi);
+  };
+#pragma unroll
+  for (int size = 2; size <= 128; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride == 1) {
+        // partner is the other element of this lane
+        const int j = 2 * lane;
+        const bool up = (j & size) == 0;
+        const bool swap = up ? less(d[1], id[1], d[0], id[0]) : less(d[0], id[0], d[1], id[1]);
+        if (swap) {
+          const float td = d[0];
+          d[0] = d[1];
+          d[1] = td;
+          const int32_t ti = id[0];
+          id[0] = id[1];
+          id[1] = ti;
+        }
+      } else {
+        const int ls = stride >> 1;  // lane distance
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int j = 2 * lane + q;
+          const float od = __shfl_xor(d[q], ls);
+          const int32_t oi = __shfl_xor(id[q], ls);
+          const bool up = (j & size) == 0;
+          const bool lower = (j & stride) == 0;
+          const bool take = (lower == up) ? less(od, oi, d[q], id[q]) : less(d[q], id[q], od, oi);
+          if (take) {
+            d[q] = od;
+            id[q] = oi;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int j = 2 * lane + q;
+    if (j < KC) {
+      ld_row[j] = d[q];
+      li_row[j] = id[q];
+    }
+  }
+  // element KC - 1 sits in lane (KC - 1) / 2, slot (KC - 1) % 2
+  const float last = ((KC - 1) % 2) ? d[1] : d[0];
+  return __shfl(last, (KC - 1) / 2);
+}
 
 // Each block owns kWaves*kRowsPerWave query rows; each wave owns 32 of them
 // and keeps their A fragments in registers for the whole feature range.  The
 // feature axis is permuted so a lane's consecutive MFMA steps read
 // consecutive 16-byte chunks: at step s, lane half h supplies features
 // h*DPAD/2 + s*E .. +E (the dot product is invariant to this permutation).
-template <typename T, int DPAD, int KC>
-__global__ __launch_bounds__(256) void knn_candidates_kernel(
+// A 32x32 accumulator tile gives lane l the scores of rows
+// 8*(i/4) + 4*(l/32) + i%4 (i = 0..15) against column l%32, so for a fixed i
+// the 32 lanes of a half hold all 32 columns of one row: a ballot places that
+// row's passing candidates in its buffer without atomics.  The main loop is
+// software pipelined: the MFMAs of one 32-column subtile are issued before
+// the threshold filter of the previous subtile runs, so the matrix core works
+// while the VALU filters; reference tiles are double buffered in LDS.
+template <typename T, int DPAD, int KC, int TILE, int BUF>
+__global__ __launch_bounds__(kWaves * 64) void knn_candidates_kernel(
     const T* __restrict__ xq, int64_t m, const T* __restrict__ xr,
     const float* __restrict__ nr, int64_t n, int32_t* __restrict__ cand) {
-  using L = CandLayout<T, DPAD, KC>;
-  constexpr int E = MfmaOp<T>::kElems;
+  using L = CandLayout<T, DPAD, KC, TILE, BUF>;
+  constexpr int kRefTile = TILE;
+  constexpr int kBuf = BUF;
+  constexpr int kThreads = kWaves * 64;
   constexpr int VEC = 16 / static_cast<int>(sizeof(T));  // elements per 16 B
-  constexpr int STEPS_PER_VEC = VEC / E;
   constexpr int HALF = DPAD / 2;
   constexpr int NV = HALF / VEC;                        // 16-byte vectors per lane
   extern __shared__ __align__(16) unsigned char lds[];
-  unsigned char* s_tile = lds;
-  float* s_nr = reinterpret_cast<float*>(lds + L::kTileBytes);
-  float* s_ld = reinterpret_cast<float*>(lds + L::kTileBytes + L::kNormBytes);
-  int32_t* s_li = reinterpret_cast<int32_t*>(s_ld + kWaves * kRowsPerWave * KC);
-  float* s_bd = reinterpret_cast<float*>(lds + L::kTileBytes + L::kNormBytes + L::kListBytes);
-  int32_t* s_bi = reinterpret_cast<int32_t*>(s_bd + kWaves * kRowsPerWave * kBuf);
-  int32_t* s_cnt = reinterpret_cast<int32_t*>(lds + L::kTileBytes + L::kNormBytes +
-                                              L::kListBytes + L::kBufBytes);
-  float* s_thr = reinterpret_cast<float*>(s_cnt + kWaves * kRowsPerWave);
-  uint32_t* s_mask = reinterpret_cast<uint32_t*>(s_thr + kWaves * kRowsPerWave);
+  unsigned char* s_tiles = lds;  // [2][tile + norms]
+  float* s_ld = reinterpret_cast<float*>(lds + L::kTiles);
+  int32_t* s_li = reinterpret_cast<int32_t*>(s_ld + L::kRows * KC);
+  float* s_bd = reinterpret_cast<float*>(lds + L::kTiles + L::kListBytes);
+  int32_t* s_bi = reinterpret_cast<int32_t*>(s_bd + L::kRows * kBuf);
+  int32_t* s_cnt = reinterpret_cast<int32_t*>(lds + L::kTiles + L::kListBytes + L::kBufBytes);
+  float* s_thr = reinterpret_cast<float*>(s_cnt + L::kRows);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lrow = lane & 31;
   const int half = lane >> 5;
-  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * (kWaves * kRowsPerWave) + wave * kRowsPerWave;
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * L::kRows + wave * kRowsPerWave;
 
-  // wave-private LDS views
   float* w_ld = s_ld + wave * kRowsPerWave * KC;
   int32_t* w_li = s_li + wave * kRowsPerWave * KC;
   float* w_bd = s_bd + wave * kRowsPerWave * kBuf;
   int32_t* w_bi = s_bi + wave * kRowsPerWave * kBuf;
   int32_t* w_cnt = s_cnt + wave * kRowsPerWave;
   float* w_thr = s_thr + wave * kRowsPerWave;
-  uint32_t* w_mask = s_mask + wave;
 
   for (int i = lane; i < kRowsPerWave * KC; i += 64) {
     w_ld[i] = INFINITY;
@@ -208,120 +334,143 @@ __global__ __launch_bounds__(256) void knn_candidates_kernel(
   }
   if (lane < kRowsPerWave) {
     w_cnt[lane] = 0;
-    // rows past m never accept a candidate
-    w_thr[lane] = (q0 + lane < m) ? INFINITY : -INFINITY;
+    w_thr[lane] = (q0 + lane < m) ? INFINITY : -INFINITY;  // rows past m take nothing
   }
-  if (lane == 0) *w_mask = 0u;
 
-  // A fragments for this lane's query row (clamped for rows past m).
-  Vec16 a[NV];
+  u32x4 a[NV];
   {
     const int64_t qr = (q0 + lrow < m) ? (q0 + lrow) : (m - 1);
-    const Vec16* src = reinterpret_cast<const Vec16*>(xq + qr * DPAD + half * HALF);
+    const u32x4* src = reinterpret_cast<const u32x4*>(xq + qr * DPAD + half * HALF);
 #pragma unroll
     for (int v = 0; v < NV; ++v) a[v] = src[v];
   }
+
+  constexpr int CHUNKS = DPAD * static_cast<int>(sizeof(T)) / 16;
+  constexpr int PER_THREAD = (kRefTile * CHUNKS + kThreads - 1) / kThreads;
+  const int n_tiles = static_cast<int>((n + kRefTile - 1) / kRefTile);
+  u32x4 pre[PER_THREAD];
+  float pre_nr = INFINITY;
+  auto fetch = [&](int tile) {
+    const int64_t base = static_cast<int64_t>(tile) * kRefTile;
+#pragma unroll
+    for (int i = 0; i < PER_THREAD; ++i) {
+      const int c = tid + i * kThreads;
+      u32x4 v = {0u, 0u, 0u, 0u};
+      if (tile < n_tiles && c < kRefTile * CHUNKS && base + c / CHUNKS < n)
+        v = reinterpret_cast<const u32x4*>(xr + (base + c / CHUNKS) * DPAD)[c % CHUNKS];
+      pre[i] = v;
+    }
+    if (tid < kRefTile) pre_nr = (tile < n_tiles && base + tid < n) ? nr[base + tid] : INFINITY;
+  };
+  auto stage = [&](int buf) {
+    unsigned char* t = s_tiles + buf * (L::kTileBytes + L::kNormBytes);
+#pragma unroll
+    for (int i = 0; i < PER_THREAD; ++i) {
+      const int c = tid + i * kThreads;
+      if (c < kRefTile * CHUNKS)
+        *reinterpret_cast<u32x4*>(t + (c / CHUNKS) * L::kRowBytes + (c % CHUNKS) * 16) = pre[i];
+    }
+    if (tid < kRefTile) reinterpret_cast<float*>(t + L::kTileBytes)[tid] = pre_nr;
+  };
+  fetch(0);
+  stage(0);
+  fetch(1);
   __syncthreads();
 
-  // Rows held by accumulator register i of this lane: 8*(i/4) + 4*half + i%4.
   float thr[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) thr[i] = w_thr[8 * (i >> 2) + 4 * half + (i & 3)];
 
-  const int n_tiles = static_cast<int>((n + kRefTile - 1) / kRefTile);
-  for (int tile = 0; tile < n_tiles; ++tile) {
-    const int64_t r0 = static_cast<int64_t>(tile) * kRefTile;
-    __syncthreads();
-    // stage kRefTile reference rows (zero rows past n) and their norms
-    {
-      constexpr int CHUNKS = DPAD * static_cast<int>(sizeof(T)) / 16;
-      for (int c = tid; c < kRefTile * CHUNKS; c += 256) {
-        const int rr = c / CHUNKS;
-        const int cc = c % CHUNKS;
-        Vec16 v = {{0u, 0u, 0u, 0u}};
-        if (r0 + rr < n) v = reinterpret_cast<const Vec16*>(xr + (r0 + rr) * DPAD)[cc];
-        *reinterpret_cast<Vec16*>(s_tile + rr * L::kRowBytes + cc * 16) = v;
+  auto mma = [&](int buf, int sub) {
+    const unsigned char* brow = s_tiles + buf * (L::kTileBytes + L::kNormBytes) +
+                                (sub * 32 + lrow) * L::kRowBytes + half * HALF * sizeof(T);
+    u32x4 bv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) bv[v] = *reinterpret_cast<const u32x4*>(brow + v * 16);
+    floatx16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc = MfmaSteps<T, MfmaOp<T>::kSteps>::run(a[v], bv[v], acc);
+    return acc;
+  };
+  // threshold filter of one subtile: passing scores go to their row's buffer
+  auto filter = [&](const floatx16& acc, float ncol, int32_t col) {
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float sc = ncol - 2.0f * acc[i];
+      const bool keep = sc < thr[i];
+      const unsigned long long kb = __ballot(keep);
+      if (kb == 0ULL) continue;
+      any = true;
+      const int row0 = 8 * (i >> 2) + (i & 3), row1 = row0 + 4;
+      const int cnt0 = w_cnt[row0], cnt1 = w_cnt[row1];
+      if (keep) {
+        const unsigned mine = half ? static_cast<unsigned>(kb >> 32) : static_cast<unsigned>(kb);
+        const int row = half ? row1 : row0;
+        const int slot = (half ? cnt1 : cnt0) + __popc(mine & ((1u << lrow) - 1u));
+        w_bd[row * kBuf + slot] = sc;
+        w_bi[row * kBuf + slot] = col;
       }
-      if (tid < kRefTile) s_nr[tid] = (r0 + tid < n) ? nr[r0 + tid] : INFINITY;
+      if (lane == 0) {
+        w_cnt[row0] = cnt0 + __popc(static_cast<unsigned>(kb));
+        w_cnt[row1] = cnt1 + __popc(static_cast<unsigned>(kb >> 32));
+      }
     }
-    __syncthreads();
+    return any;
+  };
+  // merge every row whose buffer could not take another full subtile
+  auto flush = [&](int limit) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int my_cnt = lane < kRowsPerWave ? w_cnt[lane] : 0;
+    unsigned long long pending = __ballot(my_cnt > limit);
+    if (pending == 0ULL) return;
+    while (pending) {
+      const int row = __builtin_ctzll(pending);
+      pending &= pending - 1;
+      const int cnt = __shfl(my_cnt, row);
+      const float nt = merge_row<KC>(w_ld + row * KC, w_li + row * KC, w_bd + row * kBuf,
+                                     w_bi + row * kBuf, cnt, lane);
+      if (lane == 0) {
+        w_cnt[row] = 0;
+        w_thr[row] = nt;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) thr[i] = w_thr[8 * (i >> 2) + 4 * half + (i & 3)];
+  };
 
+  // pipeline: the MFMAs of subtile g are issued before the filter of g - 1
+  floatx16 acc_p;  // scores of the previous subtile
+  float ncol_p = INFINITY;
+  int32_t col_p = 0;
+  bool have_p = false;
+  for (int tile = 0; tile < n_tiles; ++tile) {
+    const int buf = tile & 1;
+    const float* tnr = reinterpret_cast<const float*>(s_tiles + buf * (L::kTileBytes + L::kNormBytes) +
+                                                      L::kTileBytes);
 #pragma unroll
     for (int sub = 0; sub < kRefTile / 32; ++sub) {
-      floatx16 acc;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-      const unsigned char* brow = s_tile + (sub * 32 + lrow) * L::kRowBytes + half * HALF * sizeof(T);
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const Vec16 bv = *reinterpret_cast<const Vec16*>(brow + v * 16);
-        const T* ap = reinterpret_cast<const T*>(&a[v]);
-        const T* bp = reinterpret_cast<const T*>(&bv);
-#pragma unroll
-        for (int st = 0; st < STEPS_PER_VEC; ++st) acc = MfmaOp<T>::run(ap + st * E, bp + st * E, acc);
-      }
-      // register-level filter against the per-row thresholds
-      const float ncol = s_nr[sub * 32 + lrow];
-      const int32_t col = static_cast<int32_t>(r0 + sub * 32 + lrow);
-      bool any = false;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const float s = ncol - 2.0f * acc[i];
-        if (s < thr[i]) {
-          const int row = 8 * (i >> 2) + 4 * half + (i & 3);
-          const int slot = atomicAdd(&w_cnt[row], 1);
-          w_bd[row * kBuf + slot] = s;
-          w_bi[row * kBuf + slot] = col;
-          atomicOr(w_mask, 1u << row);
-          any = true;
-        }
-      }
-      if (__any(any)) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        uint32_t mask = *w_mask;
-        while (mask) {
-          const int row = __builtin_ctz(mask);
-          mask &= mask - 1;
-          float ld = lane < KC ? w_ld[row * KC + lane] : INFINITY;
-          int32_t li = lane < KC ? w_li[row * KC + lane] : INT_MAX;
-          const int cnt = w_cnt[row];
-          for (int e = 0; e < cnt; ++e) {
-            const float cd = w_bd[row * kBuf + e];
-            const int32_t ci = w_bi[row * kBuf + e];
-            const bool less = lane < KC && (ld < cd || (ld == cd && li < ci));
-            const int pos = __popcll(__ballot(less));
-            if (pos < KC) {
-              const float ud = __shfl_up(ld, 1);
-              const int32_t ui = __shfl_up(li, 1);
-              if (lane > pos) {
-                ld = ud;
-                li = ui;
-              }
-              if (lane == pos) {
-                ld = cd;
-                li = ci;
-              }
-            }
-          }
-          if (lane < KC) {
-            w_ld[row * KC + lane] = ld;
-            w_li[row * KC + lane] = li;
-          }
-          const float nt = __shfl(ld, KC - 1);
-          if (lane == 0) {
-            w_cnt[row] = 0;
-            w_thr[row] = nt;
-          }
-        }
-        if (lane == 0) *w_mask = 0u;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int i = 0; i < 16; ++i) thr[i] = w_thr[8 * (i >> 2) + 4 * half + (i & 3)];
-      }
+      const floatx16 acc_c = mma(buf, sub);
+      if (have_p && filter(acc_p, ncol_p, col_p)) flush(kBuf - 32);
+      acc_p = acc_c;
+      ncol_p = tnr[sub * 32 + lrow];
+      col_p = static_cast<int32_t>(static_cast<int64_t>(tile) * kRefTile + sub * 32 + lrow);
+      have_p = true;
     }
+    // stage the next tile into the other buffer (its last reads were the
+    // previous tile's MFMAs, before the barrier that ended that iteration)
+    __syncthreads();
+    stage(buf ^ 1);
+    fetch(tile + 2);
+    __syncthreads();
   }
+  if (have_p && filter(acc_p, ncol_p, col_p)) flush(kBuf - 32);
+  flush(0);
   // emit candidate indices
   for (int i = lane; i < kRowsPerWave * KC; i += 64) {
     const int row = i / KC;
@@ -455,20 +604,33 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   }
 }
 
+template <typename T, int DPAD, int KC, int TILE, int BUF>
+void launch_candidates_cfg(const T* xq, int64_t m, const T* xr, const float* nr, int64_t n,
+                           int32_t* cand, hipStream_t s) {
+  using L = CandLayout<T, DPAD, KC, TILE, BUF>;
+  auto kern = knn_candidates_kernel<T, DPAD, KC, TILE, BUF>;
+  HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, L::kTotal));
+  const unsigned blocks = blocks_for(static_cast<size_t>(m), L::kRows);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWaves * 64), L::kTotal, s, xq, m, xr, nr, n, cand);
+  KERNEL_CHECK();
+}
+
+// Largest configuration that fits in LDS: 64-row reference tiles with
+// 64-deep pass buffers, then 32-row tiles, then 48- and 32-deep buffers.
 template <typename T, int DPAD, int KC>
 void launch_candidates(const T* xq, int64_t m, const T* xr, const float* nr, int64_t n,
                        int32_t* cand, hipStream_t s) {
-  using L = CandLayout<T, DPAD, KC>;
-  if constexpr (L::kTotal > 160 * 1024) {
-    throw_invalid("FP32 kNN with more than 192 features supports at most 32 candidates per row");
-  } else {
-    auto kern = knn_candidates_kernel<T, DPAD, KC>;
-    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, L::kTotal));
-    const unsigned blocks = blocks_for(static_cast<size_t>(m), kWaves * kRowsPerWave);
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), L::kTotal, s, xq, m, xr, nr, n, cand);
-    KERNEL_CHECK();
-  }
+  if constexpr (CandLayout<T, DPAD, KC, 64, 64>::kTotal <= kLdsBudget)
+    launch_candidates_cfg<T, DPAD, KC, 64, 64>(xq, m, xr, nr, n, cand, s);
+  else if constexpr (CandLayout<T, DPAD, KC, 32, 64>::kTotal <= kLdsBudget)
+    launch_candidates_cfg<T, DPAD, KC, 32, 64>(xq, m, xr, nr, n, cand, s);
+  else if constexpr (CandLayout<T, DPAD, KC, 32, 48>::kTotal <= kLdsBudget)
+    launch_candidates_cfg<T, DPAD, KC, 32, 48>(xq, m, xr, nr, n, cand, s);
+  else if constexpr (CandLayout<T, DPAD, KC, 32, 32>::kTotal <= kLdsBudget)
+    launch_candidates_cfg<T, DPAD, KC, 32, 32>(xq, m, xr, nr, n, cand, s);
+  else
+    throw_invalid("this feature count / candidate count does not fit the kNN kernel's LDS budget");
 }
 
 template <typename T, int KC>

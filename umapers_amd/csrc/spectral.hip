@@ -29,7 +29,14 @@ namespace {
 
 constexpr int kSub = 18;            // (n_components + 1) * 6 for 2-D output
 constexpr int kIters = 256;         // SPECTRAL_ITERATIVE_MAX_ITERS
-constexpr int kItemRows = 512;      // rows per reduction work item
+// The block is re-orthonormalised after every kOrthEvery products instead of
+// after each one.  For an upper-triangular R with a positive diagonal,
+// orth(A R^-1) = orth(A), so in exact arithmetic the iterates equal the
+// reference's; with the operator's spectrum in [0, 2] four unnormalised
+// products raise the block's condition number by at most (2 / lambda_18)^4,
+// far inside what CholeskyQR2 handles in f64.
+constexpr int kOrthEvery = 4;
+constexpr int kItemRows = 128;      // rows per work item of the iteration kernels
 constexpr double kOrthoEps = 1e-12; // SPECTRAL_ORTHO_EPS
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -83,147 +90,277 @@ __global__ void degree_kernel(const int32_t* __restrict__ row_ptr, const float* 
   inv[i] = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
 }
 
+// ------------------------------------------------ permuted block layout ---
+// Rows of the components solved on the GPU are renumbered so that every
+// component occupies a contiguous range of positions (members ascending).
+// The block Y, the CSR graph and the normalised weights all live in this
+// order, so a component's neighbour gathers stay inside its own range and
+// hit L2.
+__global__ void pos_of_kernel(const int32_t* __restrict__ perm, int64_t rows,
+                              int32_t* __restrict__ pos_of) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p < rows) pos_of[perm[p]] = static_cast<int32_t>(p);
+}
+
+__global__ void prow_count_kernel(const int32_t* __restrict__ perm, int64_t rows,
+                                  const int32_t* __restrict__ row_ptr, int32_t* __restrict__ cnt) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p < rows) cnt[p] = row_ptr[perm[p] + 1] - row_ptr[perm[p]];
+}
+
+// Normalised weights w_ij / sqrt(d_i d_j) in f64 (the value
+// shifted_normalized_adjacency_mul forms, lib.rs:4211).
+__global__ void prow_fill_kernel(const int32_t* __restrict__ perm, int64_t rows,
+                                 const int32_t* __restrict__ row_ptr,
+                                 const int32_t* __restrict__ tail, const float* __restrict__ w,
+                                 const double* __restrict__ inv, const int32_t* __restrict__ pos_of,
+                                 const int32_t* __restrict__ prow_ptr, int32_t* __restrict__ pcol,
+                                 double* __restrict__ pnw) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const int32_t v = perm[p];
+  int32_t o = prow_ptr[p];
+  const double iv = inv[v];
+  for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e, ++o) {
+    const int32_t t = tail[e];
+    pcol[o] = pos_of[t];
+    pnw[o] = (t == v) ? 0.0 : static_cast<double>(fmaxf(w[e], 0.f)) * iv * inv[t];
+  }
+}
+
 // Start block: column 0 = sqrt(degree), the rest U(-1, 1) (lib.rs:4252-4260;
 // a counter-based stream replaces SmallRng).
 __global__ void start_block_kernel(const int32_t* __restrict__ perm, int64_t rows,
                                    const int32_t* __restrict__ slot_of_pos,
                                    const int32_t* __restrict__ local_of_pos,
                                    const uint64_t* __restrict__ seeds,
-                                   const double* __restrict__ deg, double* __restrict__ y) {
+                                   const double* __restrict__ deg, double* __restrict__ yp) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= rows) return;
   const int32_t v = perm[p];
   const uint64_t key = seeds[slot_of_pos[p]] ^ 0x7A5B3D91CE42A117ULL;
   const uint64_t lr = static_cast<uint64_t>(local_of_pos[p]);
-  double* row = y + static_cast<int64_t>(v) * kSub;
+  double* row = yp + p * kSub;
   row[0] = sqrt(deg[v]);
   for (int c = 1; c < kSub; ++c)
     row[c] = 2.0 * uniform01(key + lr * kSub + static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ULL) - 1.0;
 }
 
-// out = Y + D^-1/2 W D^-1/2 Y over the rows listed in perm.
-__global__ __launch_bounds__(256) void shifted_mul_kernel(
-    const int32_t* __restrict__ perm, int64_t rows, const int32_t* __restrict__ row_ptr,
-    const int32_t* __restrict__ tail, const float* __restrict__ w, const double* __restrict__ inv,
-    const double* __restrict__ y, double* __restrict__ out) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= rows) return;
-  const int32_t v = perm[p];
-  double acc[kSub];
-  const double* yv = y + static_cast<int64_t>(v) * kSub;
-#pragma unroll
-  for (int c = 0; c < kSub; ++c) acc[c] = yv[c];
-  const double iv = inv[v];
-  for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e) {
-    const int32_t t = tail[e];
-    const double nw = static_cast<double>(fmaxf(w[e], 0.f)) * iv * inv[t];
-    if (nw == 0.0 || t == v) continue;
-    const double2* yt = reinterpret_cast<const double2*>(y + static_cast<int64_t>(t) * kSub);
-#pragma unroll
-    for (int c = 0; c < kSub / 2; ++c) {
-      const double2 q = yt[c];
-      acc[2 * c] += nw * q.x;
-      acc[2 * c + 1] += nw * q.y;
-    }
-  }
-  double* o = out + static_cast<int64_t>(v) * kSub;
-#pragma unroll
-  for (int c = 0; c < kSub; ++c) o[c] = acc[c];
+// Work items are laid out component by component.  Hardware hands block b to
+// XCD b % 8, so remap blocks so that each XCD walks a contiguous run of items:
+// an XCD then gathers from only a few components, whose rows fit in its L2.
+__device__ __forceinline__ int xcd_item(int b, int nblocks) {
+  const int q = nblocks / 8, r = nblocks % 8;
+  const int x = b % 8;
+  return x * q + (x < r ? x : r) + b / 8;
 }
 
-// Per work item (slot, begin, end over perm positions): upper triangle of
-// A^T B accumulated into gram[slot].
-__global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ items,
-                                                   const int32_t* __restrict__ perm,
-                                                   const double* __restrict__ a,
-                                                   const double* __restrict__ b,
-                                                   double* __restrict__ gram) {
-  __shared__ double sa[32][kSub + 1];
-  __shared__ double sb[32][kSub + 1];
-  const int32_t slot = items[blockIdx.x * 3 + 0];
-  const int32_t p0 = items[blockIdx.x * 3 + 1];
-  const int32_t p1 = items[blockIdx.x * 3 + 2];
+__device__ __forceinline__ void pair_of(int tid, int& pi, int& pj) {
+  int t = tid, i = 0;
+  while (i < kSub && t >= kSub - i) {
+    t -= kSub - i;
+    ++i;
+  }
+  pi = i;
+  pj = i + t;
+}
+
+__device__ __forceinline__ void zero_range(double* buf, int len) {
+  if (buf && blockIdx.x == 0)
+    for (int i = threadIdx.x; i < len; i += blockDim.x) buf[i] = 0.0;
+}
+
+// z = y + D^-1/2 W D^-1/2 y over one work item's rows (or z = y when SPMM is
+// false), then the item's share of Z^T Z (or Y^T Z when CROSS) is added to
+// gram[slot].  A group of 9 lanes owns a row: lane j carries columns 2j, 2j+1.
+template <bool SPMM, bool CROSS>
+__global__ __launch_bounds__(256) void iter_mul_gram_kernel(
+    const int32_t* __restrict__ items, const int32_t* __restrict__ prow_ptr,
+    const int32_t* __restrict__ pcol, const double* __restrict__ pnw,
+    const double* __restrict__ yp, double* __restrict__ zp, double* __restrict__ gram,
+    double* __restrict__ zero_buf, int zero_len) {
+  __shared__ double sz[kItemRows][kSub + 1];
+  __shared__ double sy[CROSS ? kItemRows : 1][kSub + 1];
+  zero_range(zero_buf, zero_len);
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int32_t slot = items[item * 3 + 0];
+  const int32_t p0 = items[item * 3 + 1];
+  const int32_t cnt = items[item * 3 + 2] - p0;
   const int tid = threadIdx.x;
-  // pair (i, j) with j >= i for tid < 171
-  int pi = 0, pj = 0;
-  {
-    int t = tid, i = 0;
-    while (i < kSub && t >= kSub - i) {
-      t -= kSub - i;
-      ++i;
-    }
-    pi = i;
-    pj = i + t;
-  }
-  const bool active = tid < kSub * (kSub + 1) / 2;
-  double acc = 0.0;
-  for (int32_t base = p0; base < p1; base += 32) {
-    __syncthreads();
-    for (int q = tid; q < 32 * kSub; q += 256) {
-      const int r = q / kSub, c = q % kSub;
-      double va = 0.0, vb = 0.0;
-      if (base + r < p1) {
-        const int64_t v = perm[base + r];
-        va = a[v * kSub + c];
-        vb = b[v * kSub + c];
+  const int grp = tid / 9, lj = tid % 9;
+  constexpr int kGroups = 256 / 9;
+  const double2* y2 = reinterpret_cast<const double2*>(yp);
+  double2* z2 = reinterpret_cast<double2*>(zp);
+  if (grp < kGroups) {
+    for (int r = grp; r < cnt; r += kGroups) {
+      const int64_t p = p0 + r;
+      const double2 self = y2[p * (kSub / 2) + lj];
+      double2 acc = self;
+      if (SPMM) {
+        const int32_t e0 = prow_ptr[p], e1 = prow_ptr[p + 1];
+        int32_t e = e0;
+        for (; e + 4 <= e1; e += 4) {
+          const int32_t t0 = pcol[e], t1 = pcol[e + 1], t2 = pcol[e + 2], t3 = pcol[e + 3];
+          const double w0 = pnw[e], w1 = pnw[e + 1], w2 = pnw[e + 2], w3 = pnw[e + 3];
+          const double2 q0 = y2[static_cast<int64_t>(t0) * (kSub / 2) + lj];
+          const double2 q1 = y2[static_cast<int64_t>(t1) * (kSub / 2) + lj];
+          const double2 q2 = y2[static_cast<int64_t>(t2) * (kSub / 2) + lj];
+          const double2 q3 = y2[static_cast<int64_t>(t3) * (kSub / 2) + lj];
+          acc.x += w0 * q0.x;
+          acc.y += w0 * q0.y;
+          acc.x += w1 * q1.x;
+          acc.y += w1 * q1.y;
+          acc.x += w2 * q2.x;
+          acc.y += w2 * q2.y;
+          acc.x += w3 * q3.x;
+          acc.y += w3 * q3.y;
+        }
+        for (; e < e1; ++e) {
+          const double wv = pnw[e];
+          const double2 q = y2[static_cast<int64_t>(pcol[e]) * (kSub / 2) + lj];
+          acc.x += wv * q.x;
+          acc.y += wv * q.y;
+        }
       }
-      sa[r][c] = va;
-      sb[r][c] = vb;
+      z2[p * (kSub / 2) + lj] = acc;
+      sz[r][2 * lj] = acc.x;
+      sz[r][2 * lj + 1] = acc.y;
+      if (CROSS) {
+        sy[r][2 * lj] = self.x;
+        sy[r][2 * lj + 1] = self.y;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < kSub * (kSub + 1) / 2) {
+    int pi, pj;
+    pair_of(tid, pi, pj);
+    double acc = 0.0;
+    if (CROSS) {
+      for (int r = 0; r < cnt; ++r) acc += sy[r][pi] * sz[r][pj];
+    } else {
+      for (int r = 0; r < cnt; ++r) acc += sz[r][pi] * sz[r][pj];
+    }
+    atomicAdd(&gram[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
+  }
+}
+
+// z = y + D^-1/2 W D^-1/2 y without the Gram reduction (the products between
+// two orthonormalisations).  9 lanes per row, 8 neighbour gathers in flight
+// per lane; rows of a component are contiguous, so their gathers stay in L2.
+__global__ __launch_bounds__(256) void spmm_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
+                                                   const int32_t* __restrict__ pcol,
+                                                   const double* __restrict__ pnw,
+                                                   const double* __restrict__ yp,
+                                                   double* __restrict__ zp) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = gid / 9;
+  const int lj = static_cast<int>(gid % 9);
+  if (p >= rows) return;
+  const double2* y2 = reinterpret_cast<const double2*>(yp);
+  double2 acc = y2[p * (kSub / 2) + lj];
+  const int32_t e0 = prow_ptr[p], e1 = prow_ptr[p + 1];
+  int32_t e = e0;
+  for (; e + 8 <= e1; e += 8) {
+    int32_t t[8];
+    double w[8];
+    double2 q[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      t[j] = pcol[e + j];
+      w[j] = pnw[e + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) q[j] = y2[static_cast<int64_t>(t[j]) * (kSub / 2) + lj];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      acc.x += w[j] * q[j].x;
+      acc.y += w[j] * q[j].y;
+    }
+  }
+  for (; e < e1; ++e) {
+    const double wv = pnw[e];
+    const double2 q = y2[static_cast<int64_t>(pcol[e]) * (kSub / 2) + lj];
+    acc.x += wv * q.x;
+    acc.y += wv * q.y;
+  }
+  reinterpret_cast<double2*>(zp)[p * (kSub / 2) + lj] = acc;
+}
+
+// Cholesky of gram[slot] (upper R, G = R^T R) in LDS, then dst = src R^-1 for
+// the item's rows, and (GRAM) the item's share of dst^T dst into gram_out.
+// A first-pass pivot <= 1e-12 (or non-finite) flags the slot as failed: it
+// equals the residual norm Gram-Schmidt checks in lib.rs:4186-4190.
+template <bool GRAM>
+__global__ __launch_bounds__(256) void iter_orth_kernel(
+    const int32_t* __restrict__ items, const double* __restrict__ gram_in,
+    const double* __restrict__ src, double* __restrict__ dst, double* __restrict__ gram_out,
+    int32_t* __restrict__ fail, double* __restrict__ zero_buf, int zero_len) {
+  __shared__ double G[kSub][kSub];
+  __shared__ double R[kSub][kSub + 1];
+  __shared__ double sz[GRAM ? kItemRows : 1][kSub + 1];
+  __shared__ int bad;
+  zero_range(zero_buf, zero_len);
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int32_t slot = items[item * 3 + 0];
+  const int32_t p0 = items[item * 3 + 1];
+  const int32_t cnt = items[item * 3 + 2] - p0;
+  const int tid = threadIdx.x;
+  const double* g = gram_in + static_cast<int64_t>(slot) * kSub * kSub;
+  for (int q = tid; q < kSub * kSub; q += 256) {
+    const int i = q / kSub, j = q % kSub;
+    G[i][j] = i <= j ? g[i * kSub + j] : g[j * kSub + i];
+    R[i][j] = 0.0;
+  }
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int j = 0; j < kSub; ++j) {
+    if (tid == 0) {
+      double piv = G[j][j];
+      for (int k = 0; k < j; ++k) piv -= R[k][j] * R[k][j];
+      if (!(piv > kOrthoEps) || !isfinite(piv)) {
+        bad = 1;
+        piv = 1.0;
+      }
+      R[j][j] = sqrt(piv);
     }
     __syncthreads();
-    if (active)
-#pragma unroll 8
-      for (int r = 0; r < 32; ++r) acc += sa[r][pi] * sb[r][pj];
-  }
-  if (active) atomicAdd(&gram[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
-}
-
-// Cholesky of each slot's Gram matrix (upper factor R, G = R^T R).  A pivot
-// <= 1e-12 (or non-finite) marks the slot failed, as the Gram-Schmidt norm
-// check of lib.rs:4186-4190 would.
-__global__ void cholesky_kernel(const double* __restrict__ gram, double* __restrict__ rmat,
-                                int n_slots, int32_t* __restrict__ fail) {
-  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= n_slots) return;
-  const double* g = gram + static_cast<int64_t>(slot) * kSub * kSub;
-  double* r = rmat + static_cast<int64_t>(slot) * kSub * kSub;
-  for (int j = 0; j < kSub; ++j) {
-    double piv = g[j * kSub + j];
-    for (int k = 0; k < j; ++k) piv -= r[k * kSub + j] * r[k * kSub + j];
-    if (!(piv > kOrthoEps) || !isfinite(piv)) {
-      fail[slot] = 1;
-      piv = 1.0;
+    if (tid > j && tid < kSub) {
+      double v = G[j][tid];
+      for (int k = 0; k < j; ++k) v -= R[k][j] * R[k][tid];
+      R[j][tid] = v / R[j][j];
     }
-    const double rjj = sqrt(piv);
-    r[j * kSub + j] = rjj;
-    for (int i = j + 1; i < kSub; ++i) {
-      double v = g[j * kSub + i];
-      for (int k = 0; k < j; ++k) v -= r[k * kSub + j] * r[k * kSub + i];
-      r[j * kSub + i] = v / rjj;
+    __syncthreads();
+  }
+  if (tid == 0 && bad && fail) fail[slot] = 1;
+  if (tid < cnt) {
+    const int64_t p = p0 + tid;
+    const double* row = src + p * kSub;
+    double z[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      double s = row[j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= z[k] * R[k][j];
+      z[j] = s / R[j][j];
     }
-    for (int i = 0; i < j; ++i) r[j * kSub + i] = 0.0;
+    double* out = dst + p * kSub;
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      out[j] = z[j];
+      if (GRAM) sz[tid][j] = z[j];
+    }
   }
-}
-
-// y <- y R^-1 row by row (forward substitution over columns).
-__global__ void apply_rinv_kernel(const int32_t* __restrict__ perm, int64_t rows,
-                                  const int32_t* __restrict__ slot_of_pos,
-                                  const double* __restrict__ rmat, double* __restrict__ y) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= rows) return;
-  const int32_t v = perm[p];
-  const double* r = rmat + static_cast<int64_t>(slot_of_pos[p]) * kSub * kSub;
-  double* row = y + static_cast<int64_t>(v) * kSub;
-  double z[kSub];
-#pragma unroll
-  for (int j = 0; j < kSub; ++j) {
-    double s = row[j];
-#pragma unroll
-    for (int k = 0; k < j; ++k) s -= z[k] * r[k * kSub + j];
-    z[j] = s / r[j * kSub + j];
+  if (GRAM) {
+    __syncthreads();
+    if (tid < kSub * (kSub + 1) / 2) {
+      int pi, pj;
+      pair_of(tid, pi, pj);
+      double acc = 0.0;
+      for (int r = 0; r < cnt; ++r) acc += sz[r][pi] * sz[r][pj];
+      atomicAdd(&gram_out[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
+    }
   }
-#pragma unroll
-  for (int j = 0; j < kSub; ++j) row[j] = z[j];
 }
 
 // coords = Y V_slot (kSub x dim) in f64, stored f32; per-slot max |coord|.
@@ -235,7 +372,7 @@ __global__ void project_kernel(const int32_t* __restrict__ perm, int64_t rows,
   if (p >= rows) return;
   const int32_t v = perm[p];
   const int32_t slot = slot_of_pos[p];
-  const double* row = y + static_cast<int64_t>(v) * kSub;
+  const double* row = y + p * kSub;
   const double* vm = vmat + static_cast<int64_t>(slot) * kSub * dim;
   float mx = 0.f;
   for (int c = 0; c < dim; ++c) {
@@ -638,50 +775,75 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
 
     ws.deg.ensure(n);
     ws.inv.ensure(n);
-    ws.y.ensure(static_cast<size_t>(n) * kSub);
-    ws.z.ensure(static_cast<size_t>(n) * kSub);
-    ws.gram.ensure(static_cast<size_t>(n_slots) * kSub * kSub);
-    ws.rmat.ensure(static_cast<size_t>(n_slots) * kSub * kSub);
+    ws.y.ensure(static_cast<size_t>(rows) * kSub);
+    ws.z.ensure(static_cast<size_t>(rows) * kSub);
+    ws.gram.ensure(static_cast<size_t>(2) * n_slots * kSub * kSub);
     ws.fail.ensure(n_slots);
-    HIP_CHECK(hipMemsetAsync(ws.fail.get(), 0, 4 * n_slots, s));
-    HIP_CHECK(hipMemsetAsync(ws.y.get(), 0, sizeof(double) * n * kSub, s));
-    HIP_CHECK(hipMemsetAsync(ws.z.get(), 0, sizeof(double) * n * kSub, s));
+    ws.label.ensure(n);  // reused as pos_of
     hipLaunchKernelGGL(degree_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(),
                        g.weight.get(), n, ws.deg.get(), ws.inv.get());
+    hipLaunchKernelGGL(pos_of_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
+                       ws.label.get());
+    KERNEL_CHECK();
+    // permuted CSR with normalised f64 weights
+    DevBuf<int32_t> pcnt(rows), prow_ptr(rows + 1);
+    hipLaunchKernelGGL(prow_count_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
+                       g.row_ptr.get(), pcnt.get());
+    KERNEL_CHECK();
+    ScanWorkspace scan_ws;
+    const int64_t pe = exclusive_scan(pcnt.get(), prow_ptr.get(), rows, scan_ws, s);
+    const int32_t pe32 = static_cast<int32_t>(pe);
+    HIP_CHECK(hipMemcpyAsync(prow_ptr.get() + rows, &pe32, 4, hipMemcpyHostToDevice, s));
+    DevBuf<int32_t> pcol(std::max<int64_t>(pe, 1));
+    DevBuf<double> pnw(std::max<int64_t>(pe, 1));
+    hipLaunchKernelGGL(prow_fill_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
+                       g.row_ptr.get(), g.tail.get(), g.weight.get(), ws.inv.get(), ws.label.get(),
+                       prow_ptr.get(), pcol.get(), pnw.get());
     hipLaunchKernelGGL(start_block_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
                        rows, d_slot, d_local, ws.seeds.get(), ws.deg.get(), ws.y.get());
     KERNEL_CHECK();
-    auto orthonormalize = [&](double* blk) {
-      for (int pass = 0; pass < 2; ++pass) {
-        HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * n_slots * kSub * kSub, s));
-        hipLaunchKernelGGL(gram_kernel, dim3(n_items), dim3(256), 0, s, ws.items.get(), d_perm,
-                           blk, blk, ws.gram.get());
-        hipLaunchKernelGGL(cholesky_kernel, dim3(blocks_for(n_slots, 64)), dim3(64), 0, s,
-                           ws.gram.get(), ws.rmat.get(), n_slots, ws.fail.get());
-        hipLaunchKernelGGL(apply_rinv_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
-                           rows, d_slot, ws.rmat.get(), blk);
-        KERNEL_CHECK();
-      }
-    };
+    const int glen = n_slots * kSub * kSub;
+    double* g1 = ws.gram.get();
+    double* g2 = ws.gram.get() + glen;
+    HIP_CHECK(hipMemsetAsync(ws.fail.get(), 0, 4 * n_slots, s));
+    HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * 2 * glen, s));
     double* yb = ws.y.get();
     double* zb = ws.z.get();
-    orthonormalize(yb);
-    for (int it = 0; it < kIters; ++it) {
-      hipLaunchKernelGGL(shifted_mul_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
-                         rows, g.row_ptr.get(), g.tail.get(), g.weight.get(), ws.inv.get(), yb, zb);
+    const int32_t* it_items = ws.items.get();
+    // Orthonormalise Z into Y (CholeskyQR2): g1 holds Z^T Z on entry.
+    auto orth_into_y = [&]() {
+      hipLaunchKernelGGL(iter_orth_kernel<true>, dim3(n_items), dim3(256), 0, s, it_items, g1, zb, zb,
+                         g2, ws.fail.get(), static_cast<double*>(nullptr), 0);
+      hipLaunchKernelGGL(iter_orth_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, g2, zb, yb,
+                         static_cast<double*>(nullptr), static_cast<int32_t*>(nullptr), g1, glen);
       KERNEL_CHECK();
-      orthonormalize(zb);
-      std::swap(yb, zb);
+    };
+    // start block: Z = Y, g1 = Y^T Y
+    hipLaunchKernelGGL((iter_mul_gram_kernel<false, false>), dim3(n_items), dim3(256), 0, s, it_items,
+                       prow_ptr.get(), pcol.get(), pnw.get(), yb, zb, g1, g2, glen);
+    KERNEL_CHECK();
+    orth_into_y();
+    for (int it = 0; it < kIters; ++it) {
+      if ((it + 1) % kOrthEvery == 0 || it + 1 == kIters) {
+        hipLaunchKernelGGL((iter_mul_gram_kernel<true, false>), dim3(n_items), dim3(256), 0, s, it_items,
+                           prow_ptr.get(), pcol.get(), pnw.get(), yb, zb, g1, g2, glen);
+        KERNEL_CHECK();
+        orth_into_y();
+      } else {
+        hipLaunchKernelGGL(spmm_kernel, dim3(blocks_for(rows * 9, 256)), dim3(256), 0, s, rows,
+                           prow_ptr.get(), pcol.get(), pnw.get(), yb, zb);
+        KERNEL_CHECK();
+        std::swap(yb, zb);
+      }
     }
-    hipLaunchKernelGGL(shifted_mul_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
-                       g.row_ptr.get(), g.tail.get(), g.weight.get(), ws.inv.get(), yb, zb);
-    HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * n_slots * kSub * kSub, s));
-    hipLaunchKernelGGL(gram_kernel, dim3(n_items), dim3(256), 0, s, ws.items.get(), d_perm, yb, zb,
-                       ws.gram.get());
+    // Ritz matrix Y^T (I + N) Y, upper triangle
+    hipLaunchKernelGGL((iter_mul_gram_kernel<true, true>), dim3(n_items), dim3(256), 0, s, it_items,
+                       prow_ptr.get(), pcol.get(), pnw.get(), yb, zb, g1,
+                       static_cast<double*>(nullptr), 0);
     KERNEL_CHECK();
     std::vector<double> ritz(static_cast<size_t>(n_slots) * kSub * kSub);
     std::vector<int32_t> fail(n_slots);
-    HIP_CHECK(hipMemcpyAsync(ritz.data(), ws.gram.get(), 8 * ritz.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(ritz.data(), g1, 8 * ritz.size(), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(fail.data(), ws.fail.get(), 4 * n_slots, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     std::vector<double> vmat(static_cast<size_t>(n_slots) * kSub * dim);
