@@ -89,3 +89,15 @@ def test_cosine_zero_rows(gpu, oracle, rng):
     # zero rows: distance 1 to every non-zero row, 0 to each other
     assert np.array_equal(dist, od)
     assert recall_at_k(idx, oi) > 0.99
+
+
+def test_knn_to_reference_many_queries(gpu, oracle, rng):
+    # 65536 queries fill one block per CU without splitting the references;
+    # the small shapes above exercise the split-and-merge path
+    r = rng.normal(size=(2000, 32)).astype(np.float32)
+    q = rng.normal(size=(65536, 32)).astype(np.float32)
+    for metric in (0, 1):
+        idx, dist = capi.knn_to_reference(q, r, 15, metric)
+        oi, od = oracle.knn_to_reference(q, r, 15, "cosine" if metric else "euclidean")
+        assert np.array_equal(idx, oi)
+        assert np.array_equal(dist, od)

@@ -3,10 +3,11 @@
 // Replaces exact_nearest_neighbors(_euclidean/_cosine) and
 // exact_nearest_neighbors_to_reference (rust_umap/src/lib.rs:2863-2928,
 // 3283-3399).  Two passes:
-//   1. candidates: ||y||^2 - 2 x.y score tiles on MFMA (FP32 32x32x2, or BF16
-//      32x32x16 with FP32 accumulation) over mean-centred rows; every wave
-//      owns 32 query rows and keeps a sorted top-KC (score, index) list per
-//      row in LDS, fed by a register-level threshold filter;
+//   1. candidates: x.y - ||y||^2/2 score tiles on MFMA (FP32 32x32x2, or BF16
+//      32x32x16 with FP32 accumulation) over mean-centred rows; every lane
+//      owns one query and half of each reference subtile and keeps its best
+//      H (score, index) pairs sorted in registers, so each query gets 2*H
+//      candidates;
 //   2. re-rank: the KC candidates of each row get the reference's exact f32
 //      distance (direct difference, 8-lane partial sums for d >= 64, summed in
 //      lane order, then the tail; cosine per lib.rs:2740-2765), are sorted by
@@ -19,15 +20,13 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 namespace umapx {
 namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-constexpr int kRowsPerWave = 32; // query rows owned by a wave
-constexpr int kWaves = 4;        // waves per block
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
   uint32_t u = __float_as_uint(f);
@@ -82,12 +81,13 @@ __global__ void column_sums_normalized_kernel(const float* __restrict__ x, int64
 }
 
 // Writes the padded, centred (and, for cosine, normalised) compute copy of
-// each row plus its squared norm in the compute precision.  One wave per row.
+// each row and, when half_norm is given, -||row||^2 / 2 of the rounded row.
+// One wave per row.
 template <typename T>
 __global__ void prep_rows_kernel(const float* __restrict__ x, int64_t n, int d, int dpad,
                                  int normalize, const double* __restrict__ colsum,
                                  double inv_n, T* __restrict__ out,
-                                 float* __restrict__ sqnorm) {
+                                 float* __restrict__ half_norm) {
   const int lane = threadIdx.x & 63;
   const int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (r >= n) return;
@@ -116,7 +116,7 @@ __global__ void prep_rows_kernel(const float* __restrict__ x, int64_t n, int d, 
     }
   }
   for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o);
-  if (lane == 0) sqnorm[r] = nrm;
+  if (lane == 0 && half_norm) half_norm[r] = -0.5f * nrm;
 }
 
 // ---------------------------------------------------------- candidates ----
@@ -161,321 +161,317 @@ struct MfmaSteps<T, 0> {
   __device__ static inline floatx16 run(u32x4, u32x4, floatx16 c) { return c; }
 };
 
-// LDS layout (bytes) of one block: two reference tiles (double buffer),
-// their norms, and per query row the sorted top-KC list, a pass buffer and
-// counters.
-template <typename T, int DPAD, int KC, int TILE, int BUF>
-struct CandLayout {
-  static constexpr int kRowBytes = DPAD * static_cast<int>(sizeof(T)) + 16;
-  static constexpr int kTileBytes = TILE * kRowBytes;
-  static constexpr int kNormBytes = TILE * 4;
-  static constexpr int kRows = kWaves * kRowsPerWave;
-  static constexpr int kListBytes = kRows * KC * 8;
-  static constexpr int kBufBytes = kRows * BUF * 8;
-  static constexpr int kMetaBytes = kRows * 8;
-  static constexpr int kTiles = 2 * (kTileBytes + kNormBytes);
-  static constexpr int kTotal = kTiles + kListBytes + kBufBytes + kMetaBytes;
+// Candidate pass.  The MFMA takes the reference subtile as its first operand
+// and the query group as its second, so in the 32x32 accumulator lane l holds
+// query l%32 against the 16 reference rows 8*(i/4) + 4*(l/32) + i%4
+// (i = 0..15): every lane owns one (query, half of the references) pair and
+// filters its scores against its own threshold with no cross-lane traffic.
+// The accumulator starts at -||y||^2/2 and the MFMA adds x.y, giving
+// x.y - ||y||^2/2 = -(||x - y||^2 - ||x||^2)/2: larger is nearer.  Each lane
+// keeps its best H (value, index) pairs sorted in registers; values above the
+// lane's threshold are appended to a lane-private LDS buffer and inserted in
+// batches (all 64 lanes at once) when a buffer could not take another
+// subtile.  The two halves of a query together give 2*H candidates, a
+// superset of the query's top H >= k under the same scores.
+template <int H>
+struct LaneTop {
+  float v[H];      // descending
+  int32_t c[H];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      v[j] = -INFINITY;
+      c[j] = -1;
+    }
+  }
+  __device__ __forceinline__ void insert(float x, int32_t ci) {
+#pragma unroll
+    for (int j = H - 1; j > 0; --j) {
+      const bool up = x > v[j - 1];
+      const bool here = x > v[j];
+      c[j] = up ? c[j - 1] : (here ? ci : c[j]);
+      v[j] = up ? v[j - 1] : (here ? x : v[j]);
+    }
+    c[0] = x > v[0] ? ci : c[0];
+    v[0] = x > v[0] ? x : v[0];
+  }
 };
+
 constexpr int kLdsBudget = 160 * 1024;
+constexpr int kTile = 32;   // reference rows per staged tile (one MFMA subtile)
+constexpr int kSlots = 3;   // tile ring: compute t, t+1 visible, t+2 being staged
 
-// Merge the cnt buffered candidates of one row into its sorted top-KC list
-// (wave-wide; the wave owns the row).  Few entries: one insertion each, a
-// ballot finds the slot and the tail shifts up one lane.  Otherwise the list
-// and the buffer are sorted together by a bitonic network over (score,
-// index), two entries per lane.  Returns the row's new threshold.
-template <int KC>
-__device__ __noinline__ float merge_row(float* __restrict__ ld_row, int32_t* __restrict__ li_row,
-                                           const float* __restrict__ bd_row,
-                                           const int32_t* __restrict__ bi_row, int cnt, int lane) {
-  if (cnt <= 4) {
-    float ld = lane < KC ? ld_row[lane] : INFINITY;
-    int32_t li = lane < KC ? li_row[lane] : INT_MAX;
-    for (int e = 0; e < cnt; ++e) {
-      const float cd = bd_row[e];
-      const int32_t ci = bi_row[e];
-      const bool less = lane < KC && (ld < cd || (ld == cd && li < ci));
-      const int pos = __popcll(__ballot(less));
-      if (pos < KC) {
-        const float ud = __shfl_up(ld, 1);
-        const int32_t ui = __shfl_up(li, 1);
-        if (lane > pos) {
-          ld = ud;
-          li = ui;
-        }
-        if (lane == pos) {
-          ld = cd;
-          li = ci;
-        }
-      }
-    }
-    if (lane < KC) {
-      ld_row[lane] = ld;
-      li_row[lane] = li;
-    }
-    return __shfl(ld, KC - 1);
+// LDS of one block: a ring of three reference tiles with their half norms,
+// two generations of per-wave flush flags, and the waves' pass buffers
+// ([slot][lane] pairs of 8 bytes).  Eight waves (two per SIMD, at most 256
+// registers each) when the buffers can stay at least 20 deep and the
+// resident query fragments plus the top list leave room, else four.  The
+// next tile's fragments are read into registers ahead of use when they are
+// at most eight 16-byte vectors per lane.
+template <typename T, int DPAD, int H>
+struct CandShape {
+  static constexpr int kNV = DPAD / 2 / (16 / static_cast<int>(sizeof(T)));
+  static constexpr bool kPrefetch = kNV <= 8;
+  static constexpr int kRowBytes = DPAD * static_cast<int>(sizeof(T)) + 16;
+  static constexpr int kStageBytes = kTile * kRowBytes + kTile * 4;
+  static constexpr int kTiles = kSlots * kStageBytes;
+  static constexpr int kFlagBytes = 16;
+  static constexpr int bytes(int waves, int buf) { return kTiles + kFlagBytes + waves * 64 * buf * 8; }
+  static constexpr int pick_buf(int waves) {
+    for (int b : {32, 28, 24, 20})
+      if (bytes(waves, b) <= kLdsBudget) return b;
+    return 0;
   }
-  // 128 slots: element j = 2*lane + q.  List entries occupy j < KC, buffered
-  // ones j in [KC, KC + cnt).
-  float d[2];
-  int32_t id[2];
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int j = 2 * lane + q;
-    d[q] = INFINITY;
-    id[q] = INT_MAX;
-    if (j < KC) {
-      d[q] = ld_row[j];
-      id[q] = li_row[j];
-    } else if (j < KC + cnt) {
-      d[q] = bd_row[j - KC];
-      id[q] = bi_row[j - KC];
-    }
-  }
-  auto less = [](float ad, int32_t ai, float bd, int32_t bi) {
-    return ad < bd || (ad == bd && ai < bi);
-  };
-#pragma unroll
-  for (int size = 2; size <= 128; size <<= 1) {
-#pragma unroll
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      if (stride == 1) {
-        // partner is the other element of this lane
-        const int j = 2 * lane;
-        const bool up = (j & size) == 0;
-        const bool swap = up ? less(d[1], id[1], d[0], id[0]) : less(d[0], id[0], d[1], id[1]);
-        if (swap) {
-          const float td = d[0];
-          d[0] = d[1];
-          d[1] = td;
-          const int32_t ti = id[0];
-          id[0] = id[1];
-          id[1] = ti;
-        }
-      } else {
-        const int ls = stride >> 1;  // lane distance
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int j = 2 * lane + q;
-          const float od = __shfl_xor(d[q], ls);
-          const int32_t oi = __shfl_xor(id[q], ls);
-          const bool up = (j & size) == 0;
-          const bool lower = (j & stride) == 0;
-          const bool take = (lower == up) ? less(od, oi, d[q], id[q]) : less(d[q], id[q], od, oi);
-          if (take) {
-            d[q] = od;
-            id[q] = oi;
-          }
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int j = 2 * lane + q;
-    if (j < KC) {
-      ld_row[j] = d[q];
-      li_row[j] = id[q];
-    }
-  }
-  // element KC - 1 sits in lane (KC - 1) / 2, slot (KC - 1) % 2
-  const float last = ((KC - 1) % 2) ? d[1] : d[0];
-  return __shfl(last, (KC - 1) / 2);
-}
+  static constexpr bool kRegsFit8 = H <= 16 && (kPrefetch || 4 * kNV + 2 * H <= 120);
+  static constexpr int kWaves = (pick_buf(8) && kRegsFit8) ? 8 : 4;
+  static constexpr int kBuf = pick_buf(kWaves);
+  static_assert(kBuf >= 20, "kNN candidate tiles do not fit in LDS");
+  static constexpr int kBytes = bytes(kWaves, kBuf);
+  static constexpr int kQueries = kWaves * 32;  // queries per block
+};
 
-// Each block owns kWaves*kRowsPerWave query rows; each wave owns 32 of them
-// and keeps their A fragments in registers for the whole feature range.  The
-// feature axis is permuted so a lane's consecutive MFMA steps read
-// consecutive 16-byte chunks: at step s, lane half h supplies features
-// h*DPAD/2 + s*E .. +E (the dot product is invariant to this permutation).
-// A 32x32 accumulator tile gives lane l the scores of rows
-// 8*(i/4) + 4*(l/32) + i%4 (i = 0..15) against column l%32, so for a fixed i
-// the 32 lanes of a half hold all 32 columns of one row: a ballot places that
-// row's passing candidates in its buffer without atomics.  The main loop is
-// software pipelined: the MFMAs of one 32-column subtile are issued before
-// the threshold filter of the previous subtile runs, so the matrix core works
-// while the VALU filters; reference tiles are double buffered in LDS.
-template <typename T, int DPAD, int KC, int TILE, int BUF>
-__global__ __launch_bounds__(kWaves * 64) void knn_candidates_kernel(
+// Blocks below full_groups each take one query group against all reference
+// rows.  The remaining groups (the last, partial round of blocks) are split:
+// block full_groups + t takes group full_groups + t / tail_split against rows
+// [s * tail_len, (s + 1) * tail_len), s = t % tail_split, and leaves
+// (score, index) lists for knn_merge_splits_kernel.  The loop is a
+// pipeline over 32-row tiles with one barrier per tile: tile t + 2 is staged
+// while tile t is multiplied, and the fragments of tile t + 1 are read from
+// LDS into the registers tile t's MFMAs have just consumed (when they fit)
+// before tile t's scores are filtered.  A wave whose buffers could not take
+// another tile raises a flag; after the barrier every wave of the block
+// drains its buffers, so no wave idles at the barrier while another drains.
+template <typename T, int DPAD, int H>
+__global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_candidates_kernel(
     const T* __restrict__ xq, int64_t m, const T* __restrict__ xr,
-    const float* __restrict__ nr, int64_t n, int32_t* __restrict__ cand) {
-  using L = CandLayout<T, DPAD, KC, TILE, BUF>;
-  constexpr int kRefTile = TILE;
-  constexpr int kBuf = BUF;
-  constexpr int kThreads = kWaves * 64;
+    const float* __restrict__ nr_half, int64_t n, int64_t full_groups, int tail_split,
+    int64_t tail_len, int32_t* __restrict__ cand_idx, uint2* __restrict__ cand_pairs) {
+  using C = CandShape<T, DPAD, H>;
+  constexpr int WAVES = C::kWaves;
+  constexpr int BUF = C::kBuf;
+  constexpr int kThreads = WAVES * 64;
   constexpr int VEC = 16 / static_cast<int>(sizeof(T));  // elements per 16 B
   constexpr int HALF = DPAD / 2;
   constexpr int NV = HALF / VEC;                        // 16-byte vectors per lane
+  constexpr bool kPrefetch = C::kPrefetch;
   extern __shared__ __align__(16) unsigned char lds[];
-  unsigned char* s_tiles = lds;  // [2][tile + norms]
-  float* s_ld = reinterpret_cast<float*>(lds + L::kTiles);
-  int32_t* s_li = reinterpret_cast<int32_t*>(s_ld + L::kRows * KC);
-  float* s_bd = reinterpret_cast<float*>(lds + L::kTiles + L::kListBytes);
-  int32_t* s_bi = reinterpret_cast<int32_t*>(s_bd + L::kRows * kBuf);
-  int32_t* s_cnt = reinterpret_cast<int32_t*>(lds + L::kTiles + L::kListBytes + L::kBufBytes);
-  float* s_thr = reinterpret_cast<float*>(s_cnt + L::kRows);
+  unsigned char* s_tiles = lds;
+  uint8_t* s_flag = lds + C::kTiles;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int lrow = lane & 31;
+  const int l32 = lane & 31;
   const int half = lane >> 5;
-  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * L::kRows + wave * kRowsPerWave;
+  const bool split_block = static_cast<int64_t>(blockIdx.x) >= full_groups;
+  const int64_t tail_block = static_cast<int64_t>(blockIdx.x) - full_groups;
+  const int split = split_block ? static_cast<int>(tail_block % tail_split) : 0;
+  const int64_t group = split_block ? full_groups + tail_block / tail_split : blockIdx.x;
+  const int64_t q0 = group * C::kQueries + wave * 32;
+  const bool live = q0 + l32 < m;
+  const int64_t r0 = split_block ? static_cast<int64_t>(split) * tail_len : 0;
+  const int64_t r1 = split_block ? (r0 + tail_len < n ? r0 + tail_len : n) : n;
+  const int n_tiles = static_cast<int>((r1 - r0 + kTile - 1) / kTile);
+  uint2* wbuf = reinterpret_cast<uint2*>(lds + C::kTiles + C::kFlagBytes) + wave * 64 * BUF + lane;
 
-  float* w_ld = s_ld + wave * kRowsPerWave * KC;
-  int32_t* w_li = s_li + wave * kRowsPerWave * KC;
-  float* w_bd = s_bd + wave * kRowsPerWave * kBuf;
-  int32_t* w_bi = s_bi + wave * kRowsPerWave * kBuf;
-  int32_t* w_cnt = s_cnt + wave * kRowsPerWave;
-  float* w_thr = s_thr + wave * kRowsPerWave;
-
-  for (int i = lane; i < kRowsPerWave * KC; i += 64) {
-    w_ld[i] = INFINITY;
-    w_li[i] = INT_MAX;
-  }
-  if (lane < kRowsPerWave) {
-    w_cnt[lane] = 0;
-    w_thr[lane] = (q0 + lane < m) ? INFINITY : -INFINITY;  // rows past m take nothing
-  }
-
-  u32x4 a[NV];
+  u32x4 qf[NV];
   {
-    const int64_t qr = (q0 + lrow < m) ? (q0 + lrow) : (m - 1);
+    const int64_t qr = live ? (q0 + l32) : (m - 1);
     const u32x4* src = reinterpret_cast<const u32x4*>(xq + qr * DPAD + half * HALF);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) a[v] = src[v];
+    for (int v = 0; v < NV; ++v) qf[v] = src[v];
   }
 
   constexpr int CHUNKS = DPAD * static_cast<int>(sizeof(T)) / 16;
-  constexpr int PER_THREAD = (kRefTile * CHUNKS + kThreads - 1) / kThreads;
-  const int n_tiles = static_cast<int>((n + kRefTile - 1) / kRefTile);
-  u32x4 pre[PER_THREAD];
-  float pre_nr = INFINITY;
-  auto fetch = [&](int tile) {
-    const int64_t base = static_cast<int64_t>(tile) * kRefTile;
+  constexpr int PER_THREAD = (kTile * CHUNKS + kThreads - 1) / kThreads;
+  // a thread's share of one tile in flight from global memory to LDS
+  struct Pre {
+    u32x4 v[PER_THREAD];
+    float nr;
+  };
+  auto fetch = [&](int tile, Pre& p) {
+    const int64_t base = r0 + static_cast<int64_t>(tile) * kTile;
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i) {
       const int c = tid + i * kThreads;
       u32x4 v = {0u, 0u, 0u, 0u};
-      if (tile < n_tiles && c < kRefTile * CHUNKS && base + c / CHUNKS < n)
+      if (tile < n_tiles && c < kTile * CHUNKS && base + c / CHUNKS < r1)
         v = reinterpret_cast<const u32x4*>(xr + (base + c / CHUNKS) * DPAD)[c % CHUNKS];
-      pre[i] = v;
+      p.v[i] = v;
     }
-    if (tid < kRefTile) pre_nr = (tile < n_tiles && base + tid < n) ? nr[base + tid] : INFINITY;
+    p.nr = -INFINITY;
+    if (tid < kTile && tile < n_tiles && base + tid < r1) p.nr = nr_half[base + tid];
   };
-  auto stage = [&](int buf) {
-    unsigned char* t = s_tiles + buf * (L::kTileBytes + L::kNormBytes);
+  auto stage = [&](int tile, const Pre& p) {
+    unsigned char* t = s_tiles + (tile % kSlots) * C::kStageBytes;
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i) {
       const int c = tid + i * kThreads;
-      if (c < kRefTile * CHUNKS)
-        *reinterpret_cast<u32x4*>(t + (c / CHUNKS) * L::kRowBytes + (c % CHUNKS) * 16) = pre[i];
+      if (c < kTile * CHUNKS)
+        *reinterpret_cast<u32x4*>(t + (c / CHUNKS) * C::kRowBytes + (c % CHUNKS) * 16) = p.v[i];
     }
-    if (tid < kRefTile) reinterpret_cast<float*>(t + L::kTileBytes)[tid] = pre_nr;
+    if (tid < kTile) reinterpret_cast<float*>(t + kTile * C::kRowBytes)[tid] = p.nr;
   };
-  fetch(0);
-  stage(0);
-  fetch(1);
-  __syncthreads();
-
-  float thr[16];
+  // fragments (first MFMA operand) and accumulator seeds of one tile
+  struct Frags {
+    u32x4 rf[kPrefetch ? NV : 1];
+    floatx16 seed;
+  };
+  auto load = [&](int tile, Frags& f) {
+    const unsigned char* st = s_tiles + (tile % kSlots) * C::kStageBytes;
+    if constexpr (kPrefetch) {
+      const unsigned char* rrow = st + l32 * C::kRowBytes + half * HALF * sizeof(T);
 #pragma unroll
-  for (int i = 0; i < 16; ++i) thr[i] = w_thr[8 * (i >> 2) + 4 * half + (i & 3)];
-
-  auto mma = [&](int buf, int sub) {
-    const unsigned char* brow = s_tiles + buf * (L::kTileBytes + L::kNormBytes) +
-                                (sub * 32 + lrow) * L::kRowBytes + half * HALF * sizeof(T);
-    u32x4 bv[NV];
+      for (int v = 0; v < NV; ++v) f.rf[v] = *reinterpret_cast<const u32x4*>(rrow + v * 16);
+    }
+    const float* tn = reinterpret_cast<const float*>(st + kTile * C::kRowBytes) + 4 * half;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) bv[v] = *reinterpret_cast<const u32x4*>(brow + v * 16);
-    floatx16 acc;
+    for (int g = 0; g < 4; ++g) {
+      const floatx4 nv = *reinterpret_cast<const floatx4*>(tn + 8 * g);
+      f.seed[4 * g + 0] = nv[0];
+      f.seed[4 * g + 1] = nv[1];
+      f.seed[4 * g + 2] = nv[2];
+      f.seed[4 * g + 3] = nv[3];
+    }
+  };
+  auto mma = [&](int tile, const Frags& f) {
+    floatx16 acc = f.seed;
+    if constexpr (kPrefetch) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+      for (int v = 0; v < NV; ++v) acc = MfmaSteps<T, MfmaOp<T>::kSteps>::run(f.rf[v], qf[v], acc);
+    } else {
+      const unsigned char* rrow = s_tiles + (tile % kSlots) * C::kStageBytes + l32 * C::kRowBytes +
+                                  half * HALF * sizeof(T);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc = MfmaSteps<T, MfmaOp<T>::kSteps>::run(a[v], bv[v], acc);
+      for (int v = 0; v < NV; ++v)
+        acc = MfmaSteps<T, MfmaOp<T>::kSteps>::run(*reinterpret_cast<const u32x4*>(rrow + v * 16),
+                                                   qf[v], acc);
+    }
     return acc;
   };
-  // threshold filter of one subtile: passing scores go to their row's buffer
-  auto filter = [&](const floatx16& acc, float ncol, int32_t col) {
-    bool any = false;
+
+  LaneTop<H> top;
+  top.init();
+  float thr = live ? -INFINITY : INFINITY;  // lanes past m take nothing
+  int cnt = 0;
+  // appends this lane's passing scores to its buffer
+  auto filter = [&](const floatx16& acc, int32_t colbase) {
+    float mx = acc[0];
+#pragma unroll
+    for (int i = 1; i < 16; ++i) mx = fmaxf(mx, acc[i]);
+    if (__ballot(mx > thr) == 0ULL) return;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-      const float sc = ncol - 2.0f * acc[i];
-      const bool keep = sc < thr[i];
-      const unsigned long long kb = __ballot(keep);
-      if (kb == 0ULL) continue;
-      any = true;
-      const int row0 = 8 * (i >> 2) + (i & 3), row1 = row0 + 4;
-      const int cnt0 = w_cnt[row0], cnt1 = w_cnt[row1];
-      if (keep) {
-        const unsigned mine = half ? static_cast<unsigned>(kb >> 32) : static_cast<unsigned>(kb);
-        const int row = half ? row1 : row0;
-        const int slot = (half ? cnt1 : cnt0) + __popc(mine & ((1u << lrow) - 1u));
-        w_bd[row * kBuf + slot] = sc;
-        w_bi[row * kBuf + slot] = col;
-      }
-      if (lane == 0) {
-        w_cnt[row0] = cnt0 + __popc(static_cast<unsigned>(kb));
-        w_cnt[row1] = cnt1 + __popc(static_cast<unsigned>(kb >> 32));
+      if (acc[i] > thr) {
+        wbuf[cnt * 64] = make_uint2(__float_as_uint(acc[i]),
+                                    static_cast<uint32_t>(colbase + 8 * (i >> 2) + (i & 3)));
+        ++cnt;
       }
     }
-    return any;
   };
-  // merge every row whose buffer could not take another full subtile
-  auto flush = [&](int limit) {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const int my_cnt = lane < kRowsPerWave ? w_cnt[lane] : 0;
-    unsigned long long pending = __ballot(my_cnt > limit);
-    if (pending == 0ULL) return;
-    while (pending) {
-      const int row = __builtin_ctzll(pending);
-      pending &= pending - 1;
-      const int cnt = __shfl(my_cnt, row);
-      const float nt = merge_row<KC>(w_ld + row * KC, w_li + row * KC, w_bd + row * kBuf,
-                                     w_bi + row * kBuf, cnt, lane);
-      if (lane == 0) {
-        w_cnt[row] = 0;
-        w_thr[row] = nt;
+  auto flush = [&](auto with_union) {
+    for (int j = 0; __ballot(j < cnt) != 0ULL; ++j) {
+      if (j < cnt) {
+        const uint2 e = wbuf[j * 64];
+        const float x = __uint_as_float(e.x);
+        if (x > thr) {
+          top.insert(x, static_cast<int32_t>(e.y));
+          thr = fmaxf(thr, top.v[H - 1]);
+        }
       }
     }
+    cnt = 0;
+    if constexpr (!decltype(with_union)::value) return;
+    // The two halves of a query only need the H best of their union: a
+    // score below the H-th best of both lists can never make that set.  That
+    // H-th best is the max over i + j = H of min(i-th best here, j-th best
+    // there); every term is a lower bound of it, so the even and the two
+    // quarter splits, and both ends, give a valid (nearly tight) threshold.
+    // exchanged through this lane's first two buffer slots, empty now
+    wbuf[0] = make_uint2(__float_as_uint(top.v[H - 1]), __float_as_uint(top.v[H / 2 - 1]));
+    wbuf[64] = make_uint2(__float_as_uint(top.v[3 * H / 4 - 1]), __float_as_uint(top.v[H / 4 - 1]));
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int i = 0; i < 16; ++i) thr[i] = w_thr[8 * (i >> 2) + 4 * half + (i & 3)];
+    const uint2 o1 = wbuf[(lane ^ 32) - lane];
+    const uint2 o2 = wbuf[64 + (lane ^ 32) - lane];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const float b1 = __uint_as_float(o1.x), b2 = __uint_as_float(o1.y);
+    const float b3 = __uint_as_float(o2.x), b4 = __uint_as_float(o2.y);
+    const float kth = fmaxf(fmaxf(fmaxf(top.v[H - 1], b1), fminf(top.v[H / 2 - 1], b2)),
+                            fmaxf(fminf(top.v[H / 4 - 1], b3), fminf(top.v[3 * H / 4 - 1], b4)));
+    thr = fmaxf(thr, kth);
   };
 
-  // pipeline: the MFMAs of subtile g are issued before the filter of g - 1
-  floatx16 acc_p;  // scores of the previous subtile
-  float ncol_p = INFINITY;
-  int32_t col_p = 0;
-  bool have_p = false;
+  Pre pre;
+  fetch(0, pre);
+  stage(0, pre);
+  fetch(1, pre);
+  stage(1, pre);
+  fetch(2, pre);
+  if (tid < C::kFlagBytes) s_flag[tid] = 0;
+  __syncthreads();
+
+  Frags cur;
+  load(0, cur);
+  bool drain = false;
   for (int tile = 0; tile < n_tiles; ++tile) {
-    const int buf = tile & 1;
-    const float* tnr = reinterpret_cast<const float*>(s_tiles + buf * (L::kTileBytes + L::kNormBytes) +
-                                                      L::kTileBytes);
+    if (drain) flush(std::true_type{});
+    const floatx16 acc = mma(tile, cur);
+    // the next tile's fragments reuse the registers the MFMAs just consumed
+    __builtin_amdgcn_sched_barrier(0);
+    if (tile + 1 < n_tiles) load(tile + 1, cur);
+    // slot (tile + 2) % 3 last held tile - 1, whose reads all ended before
+    // the barrier that closed the previous iteration
+    stage(tile + 2, pre);
+    fetch(tile + 3, pre);
+    filter(acc, static_cast<int32_t>(r0 + static_cast<int64_t>(tile) * kTile + 4 * half));
+    const bool need = __ballot(cnt > BUF - 16) != 0ULL;
+    if (lane == 0) s_flag[(tile & 1) * 8 + wave] = need ? 1 : 0;
+    __syncthreads();
+    // block-uniform; readfirstlane lets the compiler see it as uniform
+    drain = __builtin_amdgcn_readfirstlane(
+                *reinterpret_cast<const uint64_t*>(s_flag + (tile & 1) * 8) != 0ULL ? 1 : 0) != 0;
+  }
+  flush(std::false_type{});
+  if (!live) return;
+  const int64_t q = q0 + l32;
+  if (!split_block) {
+    int32_t* dst = cand_idx + q * (2 * H) + half * H;
 #pragma unroll
-    for (int sub = 0; sub < kRefTile / 32; ++sub) {
-      const floatx16 acc_c = mma(buf, sub);
-      if (have_p && filter(acc_p, ncol_p, col_p)) flush(kBuf - 32);
-      acc_p = acc_c;
-      ncol_p = tnr[sub * 32 + lrow];
-      col_p = static_cast<int32_t>(static_cast<int64_t>(tile) * kRefTile + sub * 32 + lrow);
-      have_p = true;
+    for (int j = 0; j < H; ++j) dst[j] = top.c[j];
+  } else {
+    const int64_t qt = q - full_groups * C::kQueries;
+    uint2* dst = cand_pairs + ((qt * tail_split + split) * 2 + half) * H;
+#pragma unroll
+    for (int j = 0; j < H; ++j) dst[j] = make_uint2(__float_as_uint(top.v[j]), static_cast<uint32_t>(top.c[j]));
+  }
+}
+
+// Best H of the n_split lists of each (query, half) of the split query rows
+// [q_begin, m): the per-split top-H lists together hold the overall top H.
+// One thread per (query, half).
+template <int H>
+__global__ void knn_merge_splits_kernel(const uint2* __restrict__ pairs, int64_t q_begin,
+                                        int64_t m, int n_split, int32_t* __restrict__ cand_idx) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= 2 * (m - q_begin)) return;
+  const int64_t qt = t >> 1;
+  const int64_t q = q_begin + qt;
+  const int half = static_cast<int>(t & 1);
+  LaneTop<H> top;
+  top.init();
+  for (int s = 0; s < n_split; ++s) {
+    const uint2* src = pairs + ((qt * n_split + s) * 2 + half) * H;
+    for (int j = 0; j < H; ++j) {
+      const uint2 e = src[j];
+      const float x = __uint_as_float(e.x);
+      if (static_cast<int32_t>(e.y) >= 0 && x > top.v[H - 1]) top.insert(x, static_cast<int32_t>(e.y));
     }
-    // stage the next tile into the other buffer (its last reads were the
-    // previous tile's MFMAs, before the barrier that ended that iteration)
-    __syncthreads();
-    stage(buf ^ 1);
-    fetch(tile + 2);
-    __syncthreads();
   }
-  if (have_p && filter(acc_p, ncol_p, col_p)) flush(kBuf - 32);
-  flush(0);
-  // emit candidate indices
-  for (int i = lane; i < kRowsPerWave * KC; i += 64) {
-    const int row = i / KC;
-    if (q0 + row < m) cand[(q0 + row) * KC + (i % KC)] = w_li[i] == INT_MAX ? -1 : w_li[i];
-  }
+  int32_t* dst = cand_idx + q * (2 * H) + half * H;
+#pragma unroll
+  for (int j = 0; j < H; ++j) dst[j] = top.c[j];
 }
 
 // ------------------------------------------------------------- re-rank ----
@@ -604,44 +600,76 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   }
 }
 
-template <typename T, int DPAD, int KC, int TILE, int BUF>
-void launch_candidates_cfg(const T* xq, int64_t m, const T* xr, const float* nr, int64_t n,
-                           int32_t* cand, hipStream_t s) {
-  using L = CandLayout<T, DPAD, KC, TILE, BUF>;
-  auto kern = knn_candidates_kernel<T, DPAD, KC, TILE, BUF>;
+// Whole rounds of query groups (one block per CU) scan all reference rows;
+// the groups of a last, partial round are split over the reference rows so
+// that round ends sooner.  Returns the split count for those groups: the
+// count minimising the round's length, with a small charge per split for the
+// extra threshold warm-up, keeping at least eight tiles per split.
+int choose_tail_split(int64_t tail_groups, int64_t n, int cus) {
+  if (tail_groups <= 0) return 1;
+  int best = 1;
+  double best_cost = 1e300;
+  for (int s = 1; s <= 32; ++s) {
+    if (s > 1 && n < static_cast<int64_t>(s) * 8 * kTile) break;
+    const int64_t rounds = (tail_groups * s + cus - 1) / cus;
+    const double cost = static_cast<double>(rounds) / s + 0.01 * s;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = s;
+    }
+  }
+  return best;
+}
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus > 0 ? cus : 1;
+}
+
+template <typename T, int DPAD, int H>
+void launch_candidates(const T* xq, int64_t m, const T* xr, const float* nr_half, int64_t n,
+                       int32_t* cand, DevBuf<uint2>& pairs, hipStream_t s) {
+  using C = CandShape<T, DPAD, H>;
+  auto kern = knn_candidates_kernel<T, DPAD, H>;
   HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, L::kTotal));
-  const unsigned blocks = blocks_for(static_cast<size_t>(m), L::kRows);
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWaves * 64), L::kTotal, s, xq, m, xr, nr, n, cand);
+                                hipFuncAttributeMaxDynamicSharedMemorySize, C::kBytes));
+  const int64_t groups = (m + C::kQueries - 1) / C::kQueries;
+  const int cus = device_cus();
+  int64_t full = groups / cus * cus;
+  int tail_split = choose_tail_split(groups - full, n, cus);
+  auto split_rows = [&](int sp) { return ((n + sp - 1) / sp + kTile - 1) / kTile * kTile; };
+  while (tail_split > 1 && (tail_split - 1) * split_rows(tail_split) >= n) --tail_split;  // no empty split
+  if (tail_split == 1) full = groups;
+  const int64_t tail_len = split_rows(tail_split);
+  const int64_t tail_rows = m - full * C::kQueries;
+  uint2* pp = nullptr;
+  if (tail_rows > 0) {
+    pairs.ensure(static_cast<size_t>(tail_rows) * tail_split * 2 * H);
+    pp = pairs.get();
+  }
+  const int64_t blocks = full + (groups - full) * tail_split;
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(blocks)), dim3(C::kWaves * 64), C::kBytes, s,
+                     xq, m, xr, nr_half, n, full, tail_split, tail_len, cand, pp);
   KERNEL_CHECK();
+  if (tail_rows > 0) {
+    hipLaunchKernelGGL(knn_merge_splits_kernel<H>,
+                       dim3(blocks_for(static_cast<size_t>(2 * tail_rows), 256)), dim3(256), 0, s,
+                       pp, full * C::kQueries, m, tail_split, cand);
+    KERNEL_CHECK();
+  }
 }
 
-// Largest configuration that fits in LDS: 64-row reference tiles with
-// 64-deep pass buffers, then 32-row tiles, then 48- and 32-deep buffers.
-template <typename T, int DPAD, int KC>
-void launch_candidates(const T* xq, int64_t m, const T* xr, const float* nr, int64_t n,
-                       int32_t* cand, hipStream_t s) {
-  if constexpr (CandLayout<T, DPAD, KC, 64, 64>::kTotal <= kLdsBudget)
-    launch_candidates_cfg<T, DPAD, KC, 64, 64>(xq, m, xr, nr, n, cand, s);
-  else if constexpr (CandLayout<T, DPAD, KC, 32, 64>::kTotal <= kLdsBudget)
-    launch_candidates_cfg<T, DPAD, KC, 32, 64>(xq, m, xr, nr, n, cand, s);
-  else if constexpr (CandLayout<T, DPAD, KC, 32, 48>::kTotal <= kLdsBudget)
-    launch_candidates_cfg<T, DPAD, KC, 32, 48>(xq, m, xr, nr, n, cand, s);
-  else if constexpr (CandLayout<T, DPAD, KC, 32, 32>::kTotal <= kLdsBudget)
-    launch_candidates_cfg<T, DPAD, KC, 32, 32>(xq, m, xr, nr, n, cand, s);
-  else
-    throw_invalid("this feature count / candidate count does not fit the kNN kernel's LDS budget");
-}
-
-template <typename T, int KC>
-void dispatch_dpad(int dpad, const T* xq, int64_t m, const T* xr, const float* nr, int64_t n,
-                   int32_t* cand, hipStream_t s) {
+template <typename T, int H>
+void dispatch_dpad(int dpad, const T* xq, int64_t m, const T* xr, const float* nr_half, int64_t n,
+                   int32_t* cand, DevBuf<uint2>& pairs, hipStream_t s) {
   switch (dpad) {
-    case 32: launch_candidates<T, 32, KC>(xq, m, xr, nr, n, cand, s); break;
-    case 64: launch_candidates<T, 64, KC>(xq, m, xr, nr, n, cand, s); break;
-    case 128: launch_candidates<T, 128, KC>(xq, m, xr, nr, n, cand, s); break;
-    case 192: launch_candidates<T, 192, KC>(xq, m, xr, nr, n, cand, s); break;
-    case 256: launch_candidates<T, 256, KC>(xq, m, xr, nr, n, cand, s); break;
+    case 32: launch_candidates<T, 32, H>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 64: launch_candidates<T, 64, H>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 128: launch_candidates<T, 128, H>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 192: launch_candidates<T, 192, H>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 256: launch_candidates<T, 256, H>(xq, m, xr, nr_half, n, cand, pairs, s); break;
     default: throw_invalid("unsupported padded feature count");
   }
 }
@@ -654,10 +682,12 @@ int knn_padded_dim(int64_t d) {
   return -1;
 }
 
+// Candidates per query: two halves of H each, H the smallest of {16, 32}
+// holding both k and half of k * rerank_factor.
 int knn_candidate_count(int k, int rerank_factor) {
-  const int want = std::max(k, k * std::max(rerank_factor, 1));
-  if (want <= 32) return 32;
-  if (want <= 64) return 64;
+  const int want = std::max(k, (k * std::max(rerank_factor, 1) + 1) / 2);
+  if (want <= 16) return 32;
+  if (want <= 32) return 64;
   return -1;
 }
 
@@ -669,7 +699,7 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
   const int dpad = knn_padded_dim(d);
   if (dpad < 0) throw_invalid("exact GPU kNN supports at most 256 features");
   const int kc = knn_candidate_count(k, rerank_factor);
-  if (kc < 0 || k > kc) throw_invalid("n_neighbors * rerank_factor must be <= 64");
+  if (kc < 0) throw_invalid("n_neighbors must be <= 32 and n_neighbors * rerank_factor <= 64");
   if (k > n) throw_invalid("n_neighbors must not exceed the number of reference rows");
   const bool self = (q == r) && (m == n);
   const bool bf16 = precision == 1;
@@ -691,10 +721,7 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
 
   ws.rprep.ensure(static_cast<size_t>(n) * dpad * esz);
   ws.rnorm2.ensure(static_cast<size_t>(n));
-  if (!self) {
-    ws.qprep.ensure(static_cast<size_t>(m) * dpad * esz);
-    ws.qnorm2.ensure(static_cast<size_t>(m));
-  }
+  if (!self) ws.qprep.ensure(static_cast<size_t>(m) * dpad * esz);
   auto prep = [&](const float* x, int64_t rows, void* out, float* nrm) {
     const unsigned blocks = blocks_for(static_cast<size_t>(rows), 4);
     if (bf16)
@@ -710,7 +737,7 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
   prep(r, n, ws.rprep.get(), ws.rnorm2.get());
   const void* qp = ws.rprep.get();
   if (!self) {
-    prep(q, m, ws.qprep.get(), ws.qnorm2.get());
+    prep(q, m, ws.qprep.get(), nullptr);
     qp = ws.qprep.get();
   }
 
@@ -718,22 +745,22 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
   if (stats) HIP_CHECK(hipEventRecord(stats->cand_start, s));
   if (bf16) {
     if (kc == 32)
+      dispatch_dpad<uint16_t, 16>(dpad, static_cast<const uint16_t*>(qp), m,
+                                  reinterpret_cast<const uint16_t*>(ws.rprep.get()),
+                                  ws.rnorm2.get(), n, ws.cand.get(), ws.pairs, s);
+    else
       dispatch_dpad<uint16_t, 32>(dpad, static_cast<const uint16_t*>(qp), m,
                                   reinterpret_cast<const uint16_t*>(ws.rprep.get()),
-                                  ws.rnorm2.get(), n, ws.cand.get(), s);
-    else
-      dispatch_dpad<uint16_t, 64>(dpad, static_cast<const uint16_t*>(qp), m,
-                                  reinterpret_cast<const uint16_t*>(ws.rprep.get()),
-                                  ws.rnorm2.get(), n, ws.cand.get(), s);
+                                  ws.rnorm2.get(), n, ws.cand.get(), ws.pairs, s);
   } else {
     if (kc == 32)
+      dispatch_dpad<float, 16>(dpad, static_cast<const float*>(qp), m,
+                               reinterpret_cast<const float*>(ws.rprep.get()), ws.rnorm2.get(),
+                               n, ws.cand.get(), ws.pairs, s);
+    else
       dispatch_dpad<float, 32>(dpad, static_cast<const float*>(qp), m,
                                reinterpret_cast<const float*>(ws.rprep.get()), ws.rnorm2.get(),
-                               n, ws.cand.get(), s);
-    else
-      dispatch_dpad<float, 64>(dpad, static_cast<const float*>(qp), m,
-                               reinterpret_cast<const float*>(ws.rprep.get()), ws.rnorm2.get(),
-                               n, ws.cand.get(), s);
+                               n, ws.cand.get(), ws.pairs, s);
   }
   if (stats) {
     HIP_CHECK(hipEventRecord(stats->cand_stop, s));

@@ -9,8 +9,9 @@ namespace umapx {
 struct KnnWorkspace {
   DevBuf<double> colsum;
   DevBuf<unsigned char> qprep, rprep;
-  DevBuf<float> qnorm2, rnorm2, qnorm, rnorm;
+  DevBuf<float> rnorm2, qnorm, rnorm;  // rnorm2: -||y||^2/2 of the compute rows
   DevBuf<int32_t> cand;
+  DevBuf<uint2> pairs;  // per-split candidate lists when the reference rows are split
 };
 
 // Events around the candidate (MFMA) kernel for live roofline figures.
