@@ -97,6 +97,10 @@ typedef struct umapx_timings {
   int32_t used_approximate_knn; /* always 0 */
   int32_t init_used_spectral;
   int32_t n_graph_components;
+  /* umapx_profile_fit_transform only (0 otherwise): positive updates and
+   * negative samples over all epochs, from the exact firing schedule */
+  int64_t sgd_positive_updates;
+  int64_t sgd_negative_samples;
 } umapx_timings;
 
 typedef struct umapx_handle umapx_handle;

@@ -96,6 +96,8 @@ class Timings(ctypes.Structure):
         ("used_approximate_knn", ctypes.c_int32),
         ("init_used_spectral", ctypes.c_int32),
         ("n_graph_components", ctypes.c_int32),
+        ("sgd_positive_updates", ctypes.c_int64),
+        ("sgd_negative_samples", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:

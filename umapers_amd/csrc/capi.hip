@@ -223,6 +223,7 @@ struct umapx_handle {
   Graph graph;
   MlpModel mlp;
   umapx_timings timings{};
+  bool count_schedule = false;  // set by the profile entry point
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
   void ensure_stream() {
@@ -332,6 +333,12 @@ struct umapx_handle {
       float ms = 0.f;
       HIP_CHECK(hipEventElapsedTime(&ms, sst.start, sst.stop));
       t.sgd_kernel_ms = ms;
+    }
+    if (count_schedule) {
+      // outside optimize_sec; reads only the schedule the epochs used
+      schedule_totals_device(sws, g.n_edges, n_epochs, p.negative_sample_rate, s,
+                             &t.sgd_positive_updates, &t.sgd_negative_samples);
+      clk.lap();
     }
     DevBuf<float> re;
     if (dens_out) {
@@ -604,6 +611,11 @@ int32_t umapx_profile_fit_transform(umapx_handle* h, const float* x, int64_t n, 
                                     float* y_out, umapx_timings* t) {
   return guarded([&] {
     require_handle(h);
+    struct Flag {
+      bool& f;
+      ~Flag() { f = false; }
+    } reset{h->count_schedule};
+    h->count_schedule = true;
     h->fit_host(x, n, d, nullptr, nullptr, y_out);
     if (t) *t = h->timings;
   });

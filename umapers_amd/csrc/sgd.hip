@@ -562,6 +562,38 @@ __global__ void div_kernel(const float* __restrict__ eps, float* __restrict__ ou
   if (i < n) out[i] = fdiv(eps[i], nsr);
 }
 
+// Replays every edge's firing schedule with the epoch kernels' own f32
+// arithmetic (EdgeSchedule on per-thread copies of eons / eonns, started as
+// optimize_layout_device starts them) without touching the embedding, and
+// sums the positive updates and the negative samples over all epochs.
+__global__ __launch_bounds__(256) void schedule_totals_kernel(const float* __restrict__ eps,
+                                                              int64_t ne, int n_epochs, float nsr,
+                                                              unsigned long long* __restrict__ totals) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long pos = 0, neg = 0;
+  if (e < ne) {
+    const float ep = eps[e];
+    float eons = ep;
+    float eonns = fdiv(ep, nsr);
+    for (int t = 0; t < n_epochs; ++t) {
+      const float ef = static_cast<float>(t);
+      if (!EdgeSchedule::fire(&eons, &ep, 0, ef)) continue;
+      ++pos;
+      const int k = EdgeSchedule::negatives(&eonns, &ep, 0, ef, nsr);
+      if (k > 0) neg += static_cast<unsigned long long>(k);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    pos += __shfl_xor(pos, o);
+    neg += __shfl_xor(neg, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&totals[0], pos);
+    atomicAdd(&totals[1], neg);
+  }
+}
+
 // init_graph_transform (lib.rs:4793-4836), one thread per query row.
 __global__ void init_transform_kernel(int64_t m, int dim, const int32_t* __restrict__ row_ptr,
                                       const int32_t* __restrict__ tail,
@@ -741,6 +773,24 @@ void init_graph_transform_device(int64_t m, int dim, const Graph& g, const float
   hipLaunchKernelGGL(init_transform_kernel, dim3(blocks_for(m, 128)), dim3(128), 0, s, m, dim,
                      g.row_ptr.get(), g.tail.get(), g.weight.get(), y_base, yq);
   KERNEL_CHECK();
+}
+
+void schedule_totals_device(const SgdWorkspace& ws, int64_t ne, int n_epochs,
+                            int negative_sample_rate, hipStream_t s, int64_t* positive_updates,
+                            int64_t* negative_samples) {
+  *positive_updates = 0;
+  *negative_samples = 0;
+  if (ne == 0 || n_epochs == 0) return;
+  DevBuf<unsigned long long> tot(2);
+  HIP_CHECK(hipMemsetAsync(tot.get(), 0, 2 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(schedule_totals_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, s,
+                     ws.eps.get(), ne, n_epochs, static_cast<float>(negative_sample_rate), tot.get());
+  KERNEL_CHECK();
+  unsigned long long h[2];
+  HIP_CHECK(hipMemcpyAsync(h, tot.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  *positive_updates = static_cast<int64_t>(h[0]);
+  *negative_samples = static_cast<int64_t>(h[1]);
 }
 
 }  // namespace umapx

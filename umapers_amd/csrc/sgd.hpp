@@ -48,6 +48,12 @@ struct SgdStats {
 void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const SgdArgs& args,
                             SgdWorkspace& ws, hipStream_t s, SgdStats* stats);
 
+// Positive updates and negative samples summed over all epochs of the last
+// optimize_layout_device on this workspace (the exact schedule, replayed).
+void schedule_totals_device(const SgdWorkspace& ws, int64_t ne, int n_epochs,
+                            int negative_sample_rate, hipStream_t s, int64_t* positive_updates,
+                            int64_t* negative_samples);
+
 // optimize_layout_transform (lib.rs:5444-5544): only query rows move.
 void optimize_transform_device(float* yq, int64_t m, int dim, const float* y_base, int64_t n_base,
                                const Graph& g, const SgdArgs& args, SgdWorkspace& ws,

@@ -46,6 +46,33 @@ def gaussian_mixture(n: int, d: int, n_clusters: int, center_std: float, seed: i
     return (x, labels) if return_labels else x
 
 
+def mixture_queries(m: int, d: int, n_clusters: int, center_std: float, seed: int, point_std,
+                    query_seed: int) -> np.ndarray:
+    """Fresh rows from the mixture that gaussian_mixture(.., seed, point_std)
+    draws: its centres and cluster scales, new labels and noise from
+    query_seed."""
+    rng = np.random.default_rng(seed)
+    centers = rng.normal(0.0, center_std, size=(n_clusters, d))
+    if isinstance(point_std, tuple):
+        stds = rng.uniform(point_std[0], point_std[1], size=n_clusters)
+    else:
+        stds = np.full(n_clusters, float(point_std))
+    rq = np.random.default_rng(query_seed)
+    labels = rq.integers(0, n_clusters, size=m)
+    x = np.empty((m, d), np.float32)
+    chunk = 65536
+    for s in range(0, m, chunk):
+        lab = labels[s:s + chunk]
+        z = rq.normal(0.0, 1.0, size=(lab.shape[0], d))
+        x[s:s + chunk] = (centers[lab] + stds[lab, None] * z).astype(np.float32)
+    return x
+
+
+def c3_queries(m: int = 20000, d: int = 128, query_seed: int = 111) -> np.ndarray:
+    """Transform queries for the c3 model: same centres, new draws."""
+    return mixture_queries(m, d, 32, 3.0, 11, (0.5, 1.5), query_seed)
+
+
 def c2(n: int = 20000, d: int = 192, return_labels: bool = False):
     return gaussian_mixture(n, d, 10, 4.0, 7, 1.0, return_labels)
 
