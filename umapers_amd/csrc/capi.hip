@@ -318,6 +318,7 @@ struct umapx_handle {
     sa.negative_sample_rate = p.negative_sample_rate;
     sa.repulsion_strength = p.repulsion_strength;
     sa.seed = p.random_seed ^ 0x9E3779B97F4A7C15ULL;
+    sa.symmetric = true;  // the fuzzy union: sorted rows, every edge has its reverse
     if (dens_active) {
       sa.radii = ro.get();
       sa.dens_start_epoch = density_start(n_epochs, p.dens_frac);

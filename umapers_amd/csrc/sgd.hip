@@ -4,9 +4,11 @@
 // epoch on a single thread (optimize_layout_training_flat,
 // rust_umap/src/lib.rs:5137-5318; densMAP loop :5324-5441).  Here each epoch is
 // one edge-parallel launch (Hogwild): a thread per edge checks the edge's f32
-// schedule, applies the attraction to head and tail with float atomics, then
-// draws its negatives from a counter-based hash of (seed, epoch, edge, draw)
-// and accumulates the repulsion on a register copy of the head.  The per-edge
+// schedule, applies the attraction to its head (summed per head across the
+// wave) and to its tail (float atomics, or for the fit's symmetric graph a
+// move deferred to the end of the epoch), then draws its negatives from a
+// counter-based hash of (seed, epoch, edge, draw) and accumulates the
+// repulsion on a register copy of the head.  The per-edge
 // schedule (epoch_of_next_sample, epoch_of_next_negative_sample) uses the
 // reference's exact f32 operations, so every edge fires and draws negatives
 // the same number of times as in the reference.
@@ -154,64 +156,96 @@ __global__ __launch_bounds__(256) void sgd_epoch_kernel(
 // that the head deltas of a wave can be summed per head (edges are sorted by
 // head, so equal heads are adjacent lanes) before a single atomic per head
 // segment; the negatives of an edge are gathered with independent loads.
-template <bool TAIL_ATOMIC>
+// Tail moves: kTailAtomic adds them with float atomics at once; kTailDeferred
+// (symmetric graphs) stores each edge's tail move in the slot of its reverse
+// edge rev[e], which lies in the tail's own row, and apply_deferred_kernel
+// adds every row's slots to its vertex at the end of the epoch.  Random
+// float atomics on the tails dominate the epoch otherwise; the deferred form
+// replaces them with one plain store per firing edge and one atomic per
+// (wave, row) segment.  A launch covers edges [e_begin, e_end).
+constexpr int kTailAtomic = 0;
+constexpr int kTailDeferred = 1;
+constexpr int64_t kDeferredMinEdges = int64_t{1} << 20;
+constexpr int kRangeLaunches = 8;
+
+template <int TAIL>
 __global__ __launch_bounds__(256) void sgd_epoch2_kernel(
     float2* __restrict__ y, const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
     const float* __restrict__ eps, float* __restrict__ eons, float* __restrict__ eonns, int64_t ne,
     uint32_t n, float ef, float alpha, float a, float b, float rep, float nsr, uint64_t key_epoch,
     const float* __restrict__ radii, int dens_on, float dens_lambda, float var_shift,
-    int32_t* __restrict__ pos_counts, int32_t* __restrict__ neg_counts) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int32_t* __restrict__ rev, float2* __restrict__ tdelta,
+    int32_t* __restrict__ pos_counts, int32_t* __restrict__ neg_counts, int64_t e_begin,
+    int64_t e_end) {
+  const int64_t e = e_begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   bool active = false;
   int32_t h = -1;
   float dhx = 0.f, dhy = 0.f;
-  if (e < ne) {
+  if (e < e_end) {
     h = head[e];
     active = EdgeSchedule::fire(eons, eps, e, ef);
   }
   if (active) {
     const int32_t t = tail[e];
+    const int32_t slot = TAIL == kTailDeferred ? rev[e] : -1;
     float2 yh = y[h];
     float2 yt = y[t];
-    if (h != t) {
-      const float dx = yh.x - yt.x, dy = yh.y - yt.y;
+    // attraction (lib.rs:5196-5212) and density term (lib.rs:5110-5134) of
+    // edge (ia -> ib) on register copies, accumulating both endpoint moves
+    auto pull = [&](float2& ya, float2& yb, int32_t ia, int32_t ib, float2& da, float2& db) {
+      const float dx = ya.x - yb.x, dy = ya.y - yb.y;
       const float d2 = dx * dx + dy * dy;
       if (d2 > 0.f) {
         const float pb = pow_pos(d2, b);
         const float gc = (-2.f * a * b) * pb * rcp(d2 * (a * pb + 1.f));
         const float gx = clip4(gc * dx) * alpha, gy = clip4(gc * dy) * alpha;
-        yh.x += gx;
-        yh.y += gy;
-        yt.x -= gx;
-        yt.y -= gy;
-        dhx += gx;
-        dhy += gy;
-        if (TAIL_ATOMIC) {
-          atomicAdd(&y[t].x, -gx);
-          atomicAdd(&y[t].y, -gy);
-        } else {
-          y[t] = make_float2(yt.x, yt.y);
-        }
+        ya.x += gx;
+        ya.y += gy;
+        yb.x -= gx;
+        yb.y -= gy;
+        da.x += gx;
+        da.y += gy;
+        db.x -= gx;
+        db.y -= gy;
       }
       if (dens_on) {
-        const float sx = yh.x - yt.x, sy = yh.y - yt.y;
+        const float sx = ya.x - yb.x, sy = ya.y - yb.y;
         const float cur = sqrtf(sx * sx + sy * sy);
-        const float target = fmaxf(expf((radii[h] + radii[t]) * 0.5f), 1e-6f);
+        const float target = fmaxf(expf((radii[ia] + radii[ib]) * 0.5f), 1e-6f);
         const float coeff = 0.1f * dens_lambda * (cur - target) / (cur + var_shift);
         const float inv = 1.f / (cur + 1e-6f);
         const float gx = clip4(coeff * sx * inv) * alpha, gy = clip4(coeff * sy * inv) * alpha;
-        yh.x -= gx;
-        yh.y -= gy;
-        dhx -= gx;
-        dhy -= gy;
-        if (TAIL_ATOMIC) {
-          atomicAdd(&y[t].x, gx);
-          atomicAdd(&y[t].y, gy);
-        } else {
-          y[t] = make_float2(yt.x + gx, yt.y + gy);
-        }
+        ya.x -= gx;
+        ya.y -= gy;
+        yb.x += gx;
+        yb.y += gy;
+        da.x -= gx;
+        da.y -= gy;
+        db.x += gx;
+        db.y += gy;
       }
+    };
+    if (h != t) {
+      float2 dh = make_float2(0.f, 0.f), dt = make_float2(0.f, 0.f);
+      if (slot < 0) {
+        pull(yh, yt, h, t, dh, dt);
+        if (dt.x != 0.f || dt.y != 0.f) {
+          atomicAdd(&y[t].x, dt.x);
+          atomicAdd(&y[t].y, dt.y);
+        }
+      } else if (h < t) {
+        // The reverse edge (t, h) fires in the same epochs (equal weight,
+        // equal schedule).  The reference moves the pair twice per firing,
+        // the second time from the positions the first move left (row h
+        // before row t); with the tail move deferred, that order is replayed
+        // here for both edges and the edge (t, h) itself skips attraction.
+        pull(yh, yt, h, t, dh, dt);
+        pull(yt, yh, t, h, dt, dh);
+        if (dt.x != 0.f || dt.y != 0.f) tdelta[slot] = dt;
+      }
+      dhx += dh.x;
+      dhy += dh.y;
     }
     if (pos_counts) pos_counts[e] += 1;
     const int nneg = EdgeSchedule::negatives(eonns, eps, e, ef, nsr);
@@ -263,230 +297,6 @@ __global__ __launch_bounds__(256) void sgd_epoch2_kernel(
   if (seg_start && h >= 0 && (dhx != 0.f || dhy != 0.f)) {
     atomicAdd(&y[h].x, dhx);
     atomicAdd(&y[h].y, dhy);
-  }
-}
-
-// Symmetric-graph 2-D epoch.  After the fuzzy union every edge (h, t) has a
-// reverse (t, h) of equal weight, so both fire in the same epochs; in the
-// reference the pair moves h by +g (as head) and again by +g (as the tail of
-// (t, h), whose coefficient is odd in y_t - y_h), and likewise for the
-// density term.  Here each edge therefore applies 2g to its head only, which
-// removes every tail atomic.  Head deltas are summed per head over the
-// wave's lanes; a head whose edges all sit inside this wave is written with a
-// plain store (no other wave writes it), one that straddles a wave boundary
-// with an atomic.
-__global__ __launch_bounds__(256) void sgd_epoch2_sym_kernel(
-    float2* __restrict__ y, const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
-    const float* __restrict__ eps, float* __restrict__ eons, float* __restrict__ eonns, int64_t ne,
-    uint32_t n, float ef, float alpha, float a, float b, float rep, float nsr, uint64_t key_epoch,
-    const float* __restrict__ radii, int dens_on, float dens_lambda, float var_shift) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  bool active = false;
-  int32_t h = -1;
-  float dhx = 0.f, dhy = 0.f;
-  float2 yh0 = make_float2(0.f, 0.f);
-  if (e < ne) {
-    h = head[e];
-    active = EdgeSchedule::fire(eons, eps, e, ef);
-  }
-  if (active) {
-    const int32_t t = tail[e];
-    float2 yh = y[h];
-    yh0 = yh;
-    const float2 yt = y[t];
-    if (h != t) {
-      const float dx = yh.x - yt.x, dy = yh.y - yt.y;
-      const float d2 = dx * dx + dy * dy;
-      if (d2 > 0.f) {
-        const float pb = pow_pos(d2, b);
-        const float gc = (-2.f * a * b) * pb * rcp(d2 * (a * pb + 1.f));
-        const float gx = 2.f * clip4(gc * dx) * alpha, gy = 2.f * clip4(gc * dy) * alpha;
-        yh.x += gx;
-        yh.y += gy;
-        dhx += gx;
-        dhy += gy;
-      }
-      if (dens_on) {
-        const float sx = yh.x - yt.x, sy = yh.y - yt.y;
-        const float cur = sqrtf(sx * sx + sy * sy);
-        const float target = fmaxf(expf((radii[h] + radii[t]) * 0.5f), 1e-6f);
-        const float coeff = 0.1f * dens_lambda * (cur - target) / (cur + var_shift);
-        const float inv = 1.f / (cur + 1e-6f);
-        const float gx = 2.f * clip4(coeff * sx * inv) * alpha;
-        const float gy = 2.f * clip4(coeff * sy * inv) * alpha;
-        yh.x -= gx;
-        yh.y -= gy;
-        dhx -= gx;
-        dhy -= gy;
-      }
-    }
-    const int nneg = EdgeSchedule::negatives(eonns, eps, e, ef, nsr);
-    if (nneg > 0) {
-      const uint64_t key = mix64(key_epoch ^ static_cast<uint64_t>(e));
-      constexpr int kChunk = 8;
-      for (int s0 = 0; s0 < nneg; s0 += kChunk) {
-        float2 yk[kChunk];
-        int32_t kk[kChunk];
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          kk[j] = (s0 + j < nneg) ? static_cast<int32_t>(draw_index(key, static_cast<uint32_t>(s0 + j), n)) : h;
-          yk[j] = y[kk[j]];
-        }
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          if (kk[j] == h) continue;
-          const float dx = yh.x - yk[j].x, dy = yh.y - yk[j].y;
-          const float d2 = dx * dx + dy * dy;
-          if (d2 > 0.f) {
-            const float pb = pow_pos(d2, b);
-            const float gc = rep * rcp((0.001f + d2) * (a * pb + 1.f));
-            if (gc > 0.f) {
-              const float gx = clip4(gc * dx) * alpha, gy = clip4(gc * dy) * alpha;
-              yh.x += gx;
-              yh.y += gy;
-              dhx += gx;
-              dhy += gy;
-            }
-          }
-        }
-      }
-    }
-  }
-  // any lane of a head segment that fired knows y_h at epoch start
-  int has = active ? 1 : 0;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const float ox = __shfl_down(dhx, off);
-    const float oy = __shfl_down(dhy, off);
-    const float px = __shfl_down(yh0.x, off);
-    const float py = __shfl_down(yh0.y, off);
-    const int oa = __shfl_down(has, off);
-    const int32_t oh = __shfl_down(h, off);
-    if (lane + off < 64 && oh == h) {
-      dhx += ox;
-      dhy += oy;
-      if (!has && oa) {
-        yh0.x = px;
-        yh0.y = py;
-        has = 1;
-      }
-    }
-  }
-  const int32_t prev = __shfl_up(h, 1);
-  const int32_t next = __shfl_down(h, 1);
-  const bool seg_start = (lane == 0) || (prev != h);
-  const bool seg_end = (lane == 63) || (next != h);
-  const unsigned long long ends = __ballot(seg_end);
-  if (seg_start && h >= 0 && has) {
-    // last lane of this segment, then whether the head's edges continue into
-    // the neighbouring waves
-    const int end_lane = __builtin_ctzll(ends >> lane) + lane;
-    const int64_t e_last = e + (end_lane - lane);
-    const bool starts_inside = lane > 0 || e == 0 || head[e - 1] != h;
-    const bool ends_inside = end_lane < 63 || e_last + 1 >= ne || head[e_last + 1] != h;
-    if (starts_inside && ends_inside) {
-      y[h] = make_float2(yh0.x + dhx, yh0.y + dhy);
-    } else {
-      atomicAdd(&y[h].x, dhx);
-      atomicAdd(&y[h].y, dhy);
-    }
-  }
-}
-
-// Vertex-parallel 2-D epoch: one thread per chunk of at most chunk_len
-// consecutive edges of one head walks them in CSR order as the reference's
-// loop does (lib.rs:5180-5241), keeping the head in registers so every later
-// edge and negative of the chunk sees its earlier moves (Gauss-Seidel on the
-// head; a head has one chunk unless its degree exceeds chunk_len, which
-// keeps high-degree hubs from serialising an epoch).  Tail moves are float
-// atomics; the chunk's net head move is added atomically at the end so
-// concurrent moves from other threads are kept.
-__global__ __launch_bounds__(256) void sgd_vertex2_kernel(
-    float2* __restrict__ y, const int32_t* __restrict__ chunk_row,
-    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ row_ptr,
-    const int32_t* __restrict__ tail, const float* __restrict__ eps, float* __restrict__ eons,
-    float* __restrict__ eonns, int64_t n_chunks, int chunk_len, uint32_t n, float ef, float alpha,
-    float a, float b, float rep, float nsr, uint64_t key_epoch, const float* __restrict__ radii,
-    int dens_on, float dens_lambda, float var_shift, int32_t* __restrict__ pos_counts,
-    int32_t* __restrict__ neg_counts) {
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n_chunks) return;
-  const int32_t h = chunk_row[c];
-  const int32_t e0 = chunk_start[c];
-  const int32_t e1 = min(e0 + chunk_len, row_ptr[h + 1]);
-  float2 yh = make_float2(0.f, 0.f), start = yh;
-  bool loaded = false;
-  for (int32_t e = e0; e < e1; ++e) {
-    if (!EdgeSchedule::fire(eons, eps, e, ef)) continue;
-    if (!loaded) {
-      yh = y[h];
-      start = yh;
-      loaded = true;
-    }
-    const int32_t t = tail[e];
-    if (t != h) {
-      float2 yt = y[t];
-      const float dx = yh.x - yt.x, dy = yh.y - yt.y;
-      const float d2 = dx * dx + dy * dy;
-      if (d2 > 0.f) {
-        const float pb = pow_pos(d2, b);
-        const float gc = (-2.f * a * b) * pb * rcp(d2 * (a * pb + 1.f));
-        const float gx = clip4(gc * dx) * alpha, gy = clip4(gc * dy) * alpha;
-        yh.x += gx;
-        yh.y += gy;
-        yt.x -= gx;
-        yt.y -= gy;
-        atomicAdd(&y[t].x, -gx);
-        atomicAdd(&y[t].y, -gy);
-      }
-      if (dens_on) {
-        const float sx = yh.x - yt.x, sy = yh.y - yt.y;
-        const float cur = sqrtf(sx * sx + sy * sy);
-        const float target = fmaxf(expf((radii[h] + radii[t]) * 0.5f), 1e-6f);
-        const float coeff = 0.1f * dens_lambda * (cur - target) / (cur + var_shift);
-        const float inv = 1.f / (cur + 1e-6f);
-        const float gx = clip4(coeff * sx * inv) * alpha, gy = clip4(coeff * sy * inv) * alpha;
-        yh.x -= gx;
-        yh.y -= gy;
-        atomicAdd(&y[t].x, gx);
-        atomicAdd(&y[t].y, gy);
-      }
-    }
-    if (pos_counts) pos_counts[e] += 1;
-    const int nneg = EdgeSchedule::negatives(eonns, eps, e, ef, nsr);
-    if (nneg <= 0) continue;
-    if (neg_counts) neg_counts[e] += nneg;
-    const uint64_t key = mix64(key_epoch ^ static_cast<uint64_t>(e));
-    constexpr int kChunk = 8;
-    for (int s0 = 0; s0 < nneg; s0 += kChunk) {
-      float2 yk[kChunk];
-      int32_t kk[kChunk];
-#pragma unroll
-      for (int j = 0; j < kChunk; ++j) {
-        kk[j] = (s0 + j < nneg) ? static_cast<int32_t>(draw_index(key, static_cast<uint32_t>(s0 + j), n))
-                                : static_cast<int32_t>(h);
-        yk[j] = y[kk[j]];
-      }
-#pragma unroll
-      for (int j = 0; j < kChunk; ++j) {
-        if (kk[j] == h) continue;  // also the padding past nneg
-        const float dx = yh.x - yk[j].x, dy = yh.y - yk[j].y;
-        const float d2 = dx * dx + dy * dy;
-        if (d2 > 0.f) {
-          const float pb = pow_pos(d2, b);
-          const float gc = rep * rcp((0.001f + d2) * (a * pb + 1.f));
-          if (gc > 0.f) {
-            yh.x += clip4(gc * dx) * alpha;
-            yh.y += clip4(gc * dy) * alpha;
-          }
-        }
-      }
-    }
-  }
-  if (loaded) {
-    atomicAdd(&y[h].x, yh.x - start.x);
-    atomicAdd(&y[h].y, yh.y - start.y);
   }
 }
 
@@ -549,6 +359,64 @@ __global__ __launch_bounds__(128) void transform_sgd_kernel(
     }
   }
   for (int c = 0; c < dim; ++c) yq[hq * dim + c] = cur[c];
+}
+
+// rev[e] = position of the reverse edge (tail[e], head[e]) in the tail's
+// row, found by binary search (rows sorted by column), or -1.
+__global__ void reverse_slots_kernel(const int32_t* __restrict__ row_ptr,
+                                     const int32_t* __restrict__ head,
+                                     const int32_t* __restrict__ tail, int64_t ne,
+                                     int32_t* __restrict__ rev) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int32_t h = head[e], t = tail[e];
+  int32_t lo = row_ptr[t], hi = row_ptr[t + 1];
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if (tail[mid] < h)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  rev[e] = (lo < row_ptr[t + 1] && tail[lo] == h) ? lo : -1;
+}
+
+// End of a deferred-tail epoch: every row adds (and clears) the tail moves
+// left in its slots; slots are summed per row over the wave's lanes, then one
+// atomic per (wave, row) segment.
+__global__ __launch_bounds__(256) void apply_deferred_kernel(float2* __restrict__ y,
+                                                             const int32_t* __restrict__ head,
+                                                             float2* __restrict__ tdelta,
+                                                             int64_t ne) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int32_t h = -1;
+  float dx = 0.f, dy = 0.f;
+  if (e < ne) {
+    h = head[e];
+    const float2 d = tdelta[e];
+    if (d.x != 0.f || d.y != 0.f) {
+      dx = d.x;
+      dy = d.y;
+      tdelta[e] = make_float2(0.f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const float ox = __shfl_down(dx, off);
+    const float oy = __shfl_down(dy, off);
+    const int32_t oh = __shfl_down(h, off);
+    if (lane + off < 64 && oh == h) {
+      dx += ox;
+      dy += oy;
+    }
+  }
+  const int32_t prev = __shfl_up(h, 1);
+  const bool seg_start = (lane == 0) || (prev != h);
+  if (seg_start && h >= 0 && (dx != 0.f || dy != 0.f)) {
+    atomicAdd(&y[h].x, dx);
+    atomicAdd(&y[h].y, dy);
+  }
 }
 
 __global__ void copy_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
@@ -626,41 +494,6 @@ __global__ void init_transform_kernel(int64_t m, int dim, const int32_t* __restr
   }
 }
 
-__global__ void chunk_count_kernel(const int32_t* __restrict__ row_ptr, int64_t n, int len,
-                                   int32_t* __restrict__ cnt) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) cnt[i] = (row_ptr[i + 1] - row_ptr[i] + len - 1) / len;
-}
-
-__global__ void chunk_fill_kernel(const int32_t* __restrict__ row_ptr, int64_t n, int len,
-                                  const int32_t* __restrict__ off, int32_t* __restrict__ chunk_row,
-                                  int32_t* __restrict__ chunk_start) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int32_t o = off[i];
-  for (int32_t e = row_ptr[i]; e < row_ptr[i + 1]; e += len, ++o) {
-    chunk_row[o] = static_cast<int32_t>(i);
-    chunk_start[o] = e;
-  }
-}
-
-// Work list of (head, first edge) chunks of at most len edges each.
-int64_t build_chunks(const Graph& g, int len, SgdWorkspace& ws, hipStream_t s) {
-  const int64_t n = g.n;
-  DevBuf<int32_t> cnt(n), off(n);
-  hipLaunchKernelGGL(chunk_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(), n,
-                     len, cnt.get());
-  KERNEL_CHECK();
-  ScanWorkspace sws;
-  const int64_t total = exclusive_scan(cnt.get(), off.get(), n, sws, s);
-  ws.chunk_row.ensure(std::max<int64_t>(total, 1));
-  ws.chunk_start.ensure(std::max<int64_t>(total, 1));
-  hipLaunchKernelGGL(chunk_fill_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(), n,
-                     len, off.get(), ws.chunk_row.get(), ws.chunk_start.get());
-  KERNEL_CHECK();
-  return total;
-}
-
 uint64_t host_mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
@@ -687,47 +520,58 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
   KERNEL_CHECK();
   const float rep = 2.f * args.repulsion_strength * args.b;
   const unsigned blocks = blocks_for(ne, 256);
-  int64_t n_chunks = 0;
-  if (dim == 2 && args.vertex_parallel) n_chunks = build_chunks(g, args.chunk_len, ws, s);
-  const unsigned vblocks = blocks_for(std::max<int64_t>(n_chunks, 1), 256);
+  // Large symmetric graphs: deferred tail moves, one launch per epoch (the
+  // moves of a vertex land at the end of the epoch).  Smaller graphs, where a
+  // vertex is moved many times per epoch relative to its neighbourhood and
+  // launches are cheap, and edge lists that are not the fit's union: atomic
+  // tail moves with the epoch split into kRangeLaunches consecutive edge
+  // ranges, so later rows see the moves of earlier rows as in the
+  // reference's row order.
+  const bool deferred = dim == 2 && args.symmetric && ne >= kDeferredMinEdges;
+  if (deferred) {
+    ws.rev.ensure(ne);
+    ws.tdelta.ensure(ne);
+    hipLaunchKernelGGL(reverse_slots_kernel, dim3(blocks), dim3(256), 0, s, g.row_ptr.get(),
+                       g.head.get(), g.tail.get(), ne, ws.rev.get());
+    KERNEL_CHECK();
+    HIP_CHECK(hipMemsetAsync(ws.tdelta.get(), 0, sizeof(float2) * ne, s));
+  }
   if (stats) HIP_CHECK(hipEventRecord(stats->start, s));
   for (int epoch = 0; epoch < args.n_epochs; ++epoch) {
     const float alpha = args.learning_rate *
                         (1.f - static_cast<float>(epoch) / static_cast<float>(args.n_epochs));
     const uint64_t key_epoch = host_mix64(args.seed + static_cast<uint64_t>(epoch) * 0xD1B54A32D192ED03ULL);
     const int dens_on = (args.radii != nullptr && epoch >= args.dens_start_epoch) ? 1 : 0;
-    if (dim == 2 && args.vertex_parallel)
-      hipLaunchKernelGGL(sgd_vertex2_kernel, dim3(vblocks), dim3(256), 0, s,
-                         reinterpret_cast<float2*>(y), ws.chunk_row.get(), ws.chunk_start.get(),
-                         g.row_ptr.get(), g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(),
-                         n_chunks, args.chunk_len, static_cast<uint32_t>(n),
-                         static_cast<float>(epoch), alpha, args.a, args.b, rep, nsr, key_epoch,
-                         args.radii, dens_on, args.dens_lambda, args.dens_var_shift,
-                         args.pos_counts, args.neg_counts);
-    else if (dim == 2 && args.symmetric && !args.pos_counts && !args.neg_counts)
-      hipLaunchKernelGGL(sgd_epoch2_sym_kernel, dim3(blocks), dim3(256), 0, s,
+    if (deferred) {
+      hipLaunchKernelGGL(sgd_epoch2_kernel<kTailDeferred>, dim3(blocks), dim3(256), 0, s,
                          reinterpret_cast<float2*>(y), g.head.get(), g.tail.get(), ws.eps.get(),
                          ws.eons.get(), ws.eonns.get(), ne, static_cast<uint32_t>(n),
                          static_cast<float>(epoch), alpha, args.a, args.b, rep, nsr, key_epoch,
-                         args.radii, dens_on, args.dens_lambda, args.dens_var_shift);
-    else if (dim == 2 && !args.tail_atomic)
-      hipLaunchKernelGGL(sgd_epoch2_kernel<false>, dim3(blocks), dim3(256), 0, s, reinterpret_cast<float2*>(y),
-                         g.head.get(), g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(), ne,
-                         static_cast<uint32_t>(n), static_cast<float>(epoch), alpha, args.a, args.b,
-                         rep, nsr, key_epoch, args.radii, dens_on, args.dens_lambda,
-                         args.dens_var_shift, args.pos_counts, args.neg_counts);
-    else if (dim == 2)
-      hipLaunchKernelGGL(sgd_epoch2_kernel<true>, dim3(blocks), dim3(256), 0, s, reinterpret_cast<float2*>(y),
-                         g.head.get(), g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(), ne,
-                         static_cast<uint32_t>(n), static_cast<float>(epoch), alpha, args.a, args.b,
-                         rep, nsr, key_epoch, args.radii, dens_on, args.dens_lambda,
-                         args.dens_var_shift, args.pos_counts, args.neg_counts);
-    else
+                         args.radii, dens_on, args.dens_lambda, args.dens_var_shift, ws.rev.get(),
+                         ws.tdelta.get(), args.pos_counts, args.neg_counts, int64_t{0}, ne);
+      hipLaunchKernelGGL(apply_deferred_kernel, dim3(blocks), dim3(256), 0, s,
+                         reinterpret_cast<float2*>(y), g.head.get(), ws.tdelta.get(), ne);
+    } else if (dim == 2) {
+      for (int r = 0; r < kRangeLaunches; ++r) {
+        // range bounds on wave boundaries
+        const int64_t e0 = (ne * r / kRangeLaunches) / 64 * 64;
+        const int64_t e1 = r + 1 == kRangeLaunches ? ne : (ne * (r + 1) / kRangeLaunches) / 64 * 64;
+        if (e1 <= e0) continue;
+        hipLaunchKernelGGL(sgd_epoch2_kernel<kTailAtomic>, dim3(blocks_for(e1 - e0, 256)),
+                           dim3(256), 0, s, reinterpret_cast<float2*>(y), g.head.get(),
+                           g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(), ne,
+                           static_cast<uint32_t>(n), static_cast<float>(epoch), alpha, args.a,
+                           args.b, rep, nsr, key_epoch, args.radii, dens_on, args.dens_lambda,
+                           args.dens_var_shift, nullptr, nullptr, args.pos_counts,
+                           args.neg_counts, e0, e1);
+      }
+    } else {
       hipLaunchKernelGGL(sgd_epoch_kernel<0>, dim3(blocks), dim3(256), 0, s, y, g.head.get(),
                          g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(), ne,
                          static_cast<uint32_t>(n), dim, static_cast<float>(epoch), alpha, args.a,
                          args.b, rep, nsr, key_epoch, args.radii, dens_on, args.dens_lambda,
                          args.dens_var_shift, args.pos_counts, args.neg_counts);
+    }
     KERNEL_CHECK();
   }
   if (stats) {

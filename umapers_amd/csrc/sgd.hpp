@@ -19,16 +19,11 @@ struct SgdArgs {
   int dens_start_epoch = 0;
   float dens_lambda = 0.f;
   float dens_var_shift = 0.1f;
-  // the edge list is a symmetric fuzzy graph (every edge has a reverse of
-  // equal weight): allows the head-only update that needs no tail atomics
+  // the edge list is the fit's fuzzy union: rows sorted by column and every
+  // edge (h, t) has exactly one reverse (t, h).  2-D tail moves are then
+  // deferred to the end of each epoch through the reverse edge's slot instead
+  // of float atomics.
   bool symmetric = false;
-  // tail updates with float atomics (true) or a plain read-modify-write
-  // whose rare lost updates are part of Hogwild (false)
-  bool tail_atomic = true;
-  // one thread per head walking its edges in order (Gauss-Seidel on the
-  // head, as the reference) instead of one thread per edge
-  bool vertex_parallel = false;
-  int chunk_len = 32;  // longest run of one head's edges handled by one thread
   // optional per-edge sample counters (tests of the exact schedule)
   int32_t* pos_counts = nullptr;
   int32_t* neg_counts = nullptr;
@@ -36,7 +31,8 @@ struct SgdArgs {
 
 struct SgdWorkspace {
   DevBuf<float> eps, eons, eonns;
-  DevBuf<int32_t> chunk_row, chunk_start;
+  DevBuf<int32_t> rev;      // reverse-edge slot of every edge (deferred tails)
+  DevBuf<float2> tdelta;    // deferred tail moves, one slot per edge
 };
 
 struct SgdStats {
