@@ -167,6 +167,7 @@ constexpr int kTailAtomic = 0;
 constexpr int kTailDeferred = 1;
 constexpr int64_t kDeferredMinEdges = int64_t{1} << 20;
 constexpr int kRangeLaunches = 8;
+constexpr int kChunkLen = 32;  // longest run of one head's edges walked by one thread
 
 template <int TAIL>
 __global__ __launch_bounds__(256) void sgd_epoch2_kernel(
@@ -297,6 +298,102 @@ __global__ __launch_bounds__(256) void sgd_epoch2_kernel(
   if (seg_start && h >= 0 && (dhx != 0.f || dhy != 0.f)) {
     atomicAdd(&y[h].x, dhx);
     atomicAdd(&y[h].y, dhy);
+  }
+}
+
+// Vertex-parallel 2-D epoch: one thread per chunk of at most chunk_len
+// consecutive edges of one head walks them in CSR order as the reference's
+// loop does (lib.rs:5180-5241), keeping the head in registers so every later
+// edge and negative of the chunk sees its earlier moves (Gauss-Seidel on the
+// head; a head has one chunk unless its degree exceeds chunk_len, which
+// keeps high-degree hubs from serialising an epoch).  Tail moves are float
+// atomics; the chunk's net head move is added atomically at the end so
+// concurrent moves from other threads are kept.
+__global__ __launch_bounds__(256) void sgd_vertex2_kernel(
+    float2* __restrict__ y, const int32_t* __restrict__ chunk_row,
+    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ row_ptr,
+    const int32_t* __restrict__ tail, const float* __restrict__ eps, float* __restrict__ eons,
+    float* __restrict__ eonns, int64_t c_begin, int64_t c_end, int chunk_len, uint32_t n,
+    float ef, float alpha, float a, float b, float rep, float nsr, uint64_t key_epoch,
+    const float* __restrict__ radii, int dens_on, float dens_lambda, float var_shift,
+    int32_t* __restrict__ pos_counts, int32_t* __restrict__ neg_counts) {
+  const int64_t c = c_begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= c_end) return;
+  const int32_t h = chunk_row[c];
+  const int32_t e0 = chunk_start[c];
+  const int32_t e1 = min(e0 + chunk_len, row_ptr[h + 1]);
+  float2 yh = make_float2(0.f, 0.f), start = yh;
+  bool loaded = false;
+  for (int32_t e = e0; e < e1; ++e) {
+    if (!EdgeSchedule::fire(eons, eps, e, ef)) continue;
+    if (!loaded) {
+      yh = y[h];
+      start = yh;
+      loaded = true;
+    }
+    const int32_t t = tail[e];
+    if (t != h) {
+      float2 yt = y[t];
+      const float dx = yh.x - yt.x, dy = yh.y - yt.y;
+      const float d2 = dx * dx + dy * dy;
+      if (d2 > 0.f) {
+        const float pb = pow_pos(d2, b);
+        const float gc = (-2.f * a * b) * pb * rcp(d2 * (a * pb + 1.f));
+        const float gx = clip4(gc * dx) * alpha, gy = clip4(gc * dy) * alpha;
+        yh.x += gx;
+        yh.y += gy;
+        yt.x -= gx;
+        yt.y -= gy;
+        atomicAdd(&y[t].x, -gx);
+        atomicAdd(&y[t].y, -gy);
+      }
+      if (dens_on) {
+        const float sx = yh.x - yt.x, sy = yh.y - yt.y;
+        const float cur = sqrtf(sx * sx + sy * sy);
+        const float target = fmaxf(expf((radii[h] + radii[t]) * 0.5f), 1e-6f);
+        const float coeff = 0.1f * dens_lambda * (cur - target) / (cur + var_shift);
+        const float inv = 1.f / (cur + 1e-6f);
+        const float gx = clip4(coeff * sx * inv) * alpha, gy = clip4(coeff * sy * inv) * alpha;
+        yh.x -= gx;
+        yh.y -= gy;
+        atomicAdd(&y[t].x, gx);
+        atomicAdd(&y[t].y, gy);
+      }
+    }
+    if (pos_counts) pos_counts[e] += 1;
+    const int nneg = EdgeSchedule::negatives(eonns, eps, e, ef, nsr);
+    if (nneg <= 0) continue;
+    if (neg_counts) neg_counts[e] += nneg;
+    const uint64_t key = mix64(key_epoch ^ static_cast<uint64_t>(e));
+    constexpr int kChunk = 8;
+    for (int s0 = 0; s0 < nneg; s0 += kChunk) {
+      float2 yk[kChunk];
+      int32_t kk[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        kk[j] = (s0 + j < nneg) ? static_cast<int32_t>(draw_index(key, static_cast<uint32_t>(s0 + j), n))
+                                : static_cast<int32_t>(h);
+        yk[j] = y[kk[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        if (kk[j] == h) continue;  // also the padding past nneg
+        const float dx = yh.x - yk[j].x, dy = yh.y - yk[j].y;
+        const float d2 = dx * dx + dy * dy;
+        if (d2 > 0.f) {
+          const float pb = pow_pos(d2, b);
+          const float gc = rep * rcp((0.001f + d2) * (a * pb + 1.f));
+          if (gc > 0.f) {
+            yh.x += clip4(gc * dx) * alpha;
+            yh.y += clip4(gc * dy) * alpha;
+          }
+        }
+      }
+    }
+  }
+  if (loaded) {
+    atomicAdd(&y[h].x, yh.x - start.x);
+    atomicAdd(&y[h].y, yh.y - start.y);
   }
 }
 
@@ -494,6 +591,41 @@ __global__ void init_transform_kernel(int64_t m, int dim, const int32_t* __restr
   }
 }
 
+__global__ void chunk_count_kernel(const int32_t* __restrict__ row_ptr, int64_t n, int len,
+                                   int32_t* __restrict__ cnt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = (row_ptr[i + 1] - row_ptr[i] + len - 1) / len;
+}
+
+__global__ void chunk_fill_kernel(const int32_t* __restrict__ row_ptr, int64_t n, int len,
+                                  const int32_t* __restrict__ off, int32_t* __restrict__ chunk_row,
+                                  int32_t* __restrict__ chunk_start) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t o = off[i];
+  for (int32_t e = row_ptr[i]; e < row_ptr[i + 1]; e += len, ++o) {
+    chunk_row[o] = static_cast<int32_t>(i);
+    chunk_start[o] = e;
+  }
+}
+
+// Work list of (head, first edge) chunks of at most len edges each.
+int64_t build_chunks(const Graph& g, int len, SgdWorkspace& ws, hipStream_t s) {
+  const int64_t n = g.n;
+  DevBuf<int32_t> cnt(n), off(n);
+  hipLaunchKernelGGL(chunk_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(), n,
+                     len, cnt.get());
+  KERNEL_CHECK();
+  ScanWorkspace sws;
+  const int64_t total = exclusive_scan(cnt.get(), off.get(), n, sws, s);
+  ws.chunk_row.ensure(std::max<int64_t>(total, 1));
+  ws.chunk_start.ensure(std::max<int64_t>(total, 1));
+  hipLaunchKernelGGL(chunk_fill_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(), n,
+                     len, off.get(), ws.chunk_row.get(), ws.chunk_start.get());
+  KERNEL_CHECK();
+  return total;
+}
+
 uint64_t host_mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
@@ -520,14 +652,18 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
   KERNEL_CHECK();
   const float rep = 2.f * args.repulsion_strength * args.b;
   const unsigned blocks = blocks_for(ne, 256);
-  // Large symmetric graphs: deferred tail moves, one launch per epoch (the
-  // moves of a vertex land at the end of the epoch).  Smaller graphs, where a
-  // vertex is moved many times per epoch relative to its neighbourhood and
-  // launches are cheap, and edge lists that are not the fit's union: atomic
-  // tail moves with the epoch split into kRangeLaunches consecutive edge
-  // ranges, so later rows see the moves of earlier rows as in the
-  // reference's row order.
-  const bool deferred = dim == 2 && args.symmetric && ne >= kDeferredMinEdges;
+  // Large symmetric graphs (the fit's union): deferred tail moves, one launch
+  // per epoch (a vertex's tail moves land at the end of the epoch).  Smaller
+  // graphs, where every vertex moves many times per epoch relative to its
+  // neighbourhood: one thread per head walks its edges in order with the head
+  // in registers (Gauss-Seidel on the head, as the reference's loop), tail
+  // moves atomic.  Large edge lists that are not the fit's union: edge
+  // parallel with atomic tail moves.  The last two split each epoch into
+  // kRangeLaunches launches over consecutive heads, so later rows see the
+  // moves of earlier rows as in the reference's row order.
+  const bool large = ne >= kDeferredMinEdges;
+  const bool deferred = dim == 2 && args.symmetric && large;
+  const bool by_head = dim == 2 && !large;
   if (deferred) {
     ws.rev.ensure(ne);
     ws.tdelta.ensure(ne);
@@ -536,6 +672,7 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
     KERNEL_CHECK();
     HIP_CHECK(hipMemsetAsync(ws.tdelta.get(), 0, sizeof(float2) * ne, s));
   }
+  const int64_t n_chunks = by_head ? build_chunks(g, kChunkLen, ws, s) : 0;
   if (stats) HIP_CHECK(hipEventRecord(stats->start, s));
   for (int epoch = 0; epoch < args.n_epochs; ++epoch) {
     const float alpha = args.learning_rate *
@@ -551,6 +688,18 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
                          ws.tdelta.get(), args.pos_counts, args.neg_counts, int64_t{0}, ne);
       hipLaunchKernelGGL(apply_deferred_kernel, dim3(blocks), dim3(256), 0, s,
                          reinterpret_cast<float2*>(y), g.head.get(), ws.tdelta.get(), ne);
+    } else if (by_head) {
+      for (int r = 0; r < kRangeLaunches; ++r) {
+        const int64_t c0 = n_chunks * r / kRangeLaunches, c1 = n_chunks * (r + 1) / kRangeLaunches;
+        if (c1 <= c0) continue;
+        hipLaunchKernelGGL(sgd_vertex2_kernel, dim3(blocks_for(c1 - c0, 256)), dim3(256), 0, s,
+                           reinterpret_cast<float2*>(y), ws.chunk_row.get(), ws.chunk_start.get(),
+                           g.row_ptr.get(), g.tail.get(), ws.eps.get(), ws.eons.get(),
+                           ws.eonns.get(), c0, c1, kChunkLen, static_cast<uint32_t>(n),
+                           static_cast<float>(epoch), alpha, args.a, args.b, rep, nsr, key_epoch,
+                           args.radii, dens_on, args.dens_lambda, args.dens_var_shift,
+                           args.pos_counts, args.neg_counts);
+      }
     } else if (dim == 2) {
       for (int r = 0; r < kRangeLaunches; ++r) {
         // range bounds on wave boundaries

@@ -33,6 +33,7 @@ struct SgdWorkspace {
   DevBuf<float> eps, eons, eonns;
   DevBuf<int32_t> rev;      // reverse-edge slot of every edge (deferred tails)
   DevBuf<float2> tdelta;    // deferred tail moves, one slot per edge
+  DevBuf<int32_t> chunk_row, chunk_start;  // (head, first edge) of each walked chunk
 };
 
 struct SgdStats {

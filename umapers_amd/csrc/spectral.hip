@@ -170,96 +170,28 @@ __device__ __forceinline__ void zero_range(double* buf, int len) {
     for (int i = threadIdx.x; i < len; i += blockDim.x) buf[i] = 0.0;
 }
 
-// z = y + D^-1/2 W D^-1/2 y over one work item's rows (or z = y when SPMM is
-// false), then the item's share of Z^T Z (or Y^T Z when CROSS) is added to
-// gram[slot].  A group of 9 lanes owns a row: lane j carries columns 2j, 2j+1.
-template <bool SPMM, bool CROSS>
-__global__ __launch_bounds__(256) void iter_mul_gram_kernel(
-    const int32_t* __restrict__ items, const int32_t* __restrict__ prow_ptr,
-    const int32_t* __restrict__ pcol, const double* __restrict__ pnw,
-    const double* __restrict__ yp, double* __restrict__ zp, double* __restrict__ gram,
-    double* __restrict__ zero_buf, int zero_len) {
-  __shared__ double sz[kItemRows][kSub + 1];
-  __shared__ double sy[CROSS ? kItemRows : 1][kSub + 1];
-  zero_range(zero_buf, zero_len);
-  const int item = xcd_item(blockIdx.x, gridDim.x);
-  const int32_t slot = items[item * 3 + 0];
-  const int32_t p0 = items[item * 3 + 1];
-  const int32_t cnt = items[item * 3 + 2] - p0;
-  const int tid = threadIdx.x;
-  const int grp = tid / 9, lj = tid % 9;
-  constexpr int kGroups = 256 / 9;
-  const double2* y2 = reinterpret_cast<const double2*>(yp);
-  double2* z2 = reinterpret_cast<double2*>(zp);
-  if (grp < kGroups) {
-    for (int r = grp; r < cnt; r += kGroups) {
-      const int64_t p = p0 + r;
-      const double2 self = y2[p * (kSub / 2) + lj];
-      double2 acc = self;
-      if (SPMM) {
-        const int32_t e0 = prow_ptr[p], e1 = prow_ptr[p + 1];
-        int32_t e = e0;
-        for (; e + 4 <= e1; e += 4) {
-          const int32_t t0 = pcol[e], t1 = pcol[e + 1], t2 = pcol[e + 2], t3 = pcol[e + 3];
-          const double w0 = pnw[e], w1 = pnw[e + 1], w2 = pnw[e + 2], w3 = pnw[e + 3];
-          const double2 q0 = y2[static_cast<int64_t>(t0) * (kSub / 2) + lj];
-          const double2 q1 = y2[static_cast<int64_t>(t1) * (kSub / 2) + lj];
-          const double2 q2 = y2[static_cast<int64_t>(t2) * (kSub / 2) + lj];
-          const double2 q3 = y2[static_cast<int64_t>(t3) * (kSub / 2) + lj];
-          acc.x += w0 * q0.x;
-          acc.y += w0 * q0.y;
-          acc.x += w1 * q1.x;
-          acc.y += w1 * q1.y;
-          acc.x += w2 * q2.x;
-          acc.y += w2 * q2.y;
-          acc.x += w3 * q3.x;
-          acc.y += w3 * q3.y;
-        }
-        for (; e < e1; ++e) {
-          const double wv = pnw[e];
-          const double2 q = y2[static_cast<int64_t>(pcol[e]) * (kSub / 2) + lj];
-          acc.x += wv * q.x;
-          acc.y += wv * q.y;
-        }
-      }
-      z2[p * (kSub / 2) + lj] = acc;
-      sz[r][2 * lj] = acc.x;
-      sz[r][2 * lj + 1] = acc.y;
-      if (CROSS) {
-        sy[r][2 * lj] = self.x;
-        sy[r][2 * lj + 1] = self.y;
-      }
-    }
-  }
-  __syncthreads();
-  if (tid < kSub * (kSub + 1) / 2) {
-    int pi, pj;
-    pair_of(tid, pi, pj);
-    double acc = 0.0;
-    if (CROSS) {
-      for (int r = 0; r < cnt; ++r) acc += sy[r][pi] * sz[r][pj];
-    } else {
-      for (int r = 0; r < cnt; ++r) acc += sz[r][pi] * sz[r][pj];
-    }
-    atomicAdd(&gram[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
-  }
-}
+// z = y + D^-1/2 W D^-1/2 y, in two kernels so that hub rows (in-degree in
+// the hundreds after the union) do not hold a wave for their whole length:
+// rows with at most kLongRow entries take 9 lanes each (lane j carries
+// columns 2j, 2j+1, 8 neighbour gathers in flight per lane); longer rows are
+// listed once and each takes a block whose 28 lane groups split its entries
+// and sum their partials in LDS.
+constexpr int kLongRow = 64;
 
-// z = y + D^-1/2 W D^-1/2 y without the Gram reduction (the products between
-// two orthonormalisations).  9 lanes per row, 8 neighbour gathers in flight
-// per lane; rows of a component are contiguous, so their gathers stay in L2.
-__global__ __launch_bounds__(256) void spmm_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
-                                                   const int32_t* __restrict__ pcol,
-                                                   const double* __restrict__ pnw,
-                                                   const double* __restrict__ yp,
-                                                   double* __restrict__ zp) {
+__global__ __launch_bounds__(256) void spmm_short_kernel(int64_t rows,
+                                                         const int32_t* __restrict__ prow_ptr,
+                                                         const int32_t* __restrict__ pcol,
+                                                         const double* __restrict__ pnw,
+                                                         const double* __restrict__ yp,
+                                                         double* __restrict__ zp) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t p = gid / 9;
   const int lj = static_cast<int>(gid % 9);
   if (p >= rows) return;
+  const int32_t e0 = prow_ptr[p], e1 = prow_ptr[p + 1];
+  if (e1 - e0 > kLongRow) return;
   const double2* y2 = reinterpret_cast<const double2*>(yp);
   double2 acc = y2[p * (kSub / 2) + lj];
-  const int32_t e0 = prow_ptr[p], e1 = prow_ptr[p + 1];
   int32_t e = e0;
   for (; e + 8 <= e1; e += 8) {
     int32_t t[8];
@@ -285,6 +217,83 @@ __global__ __launch_bounds__(256) void spmm_kernel(int64_t rows, const int32_t* 
     acc.y += wv * q.y;
   }
   reinterpret_cast<double2*>(zp)[p * (kSub / 2) + lj] = acc;
+}
+
+__global__ void long_rows_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
+                                 int32_t* __restrict__ list, int32_t* __restrict__ count) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p < rows && prow_ptr[p + 1] - prow_ptr[p] > kLongRow)
+    list[atomicAdd(count, 1)] = static_cast<int32_t>(p);
+}
+
+__global__ __launch_bounds__(256) void spmm_long_kernel(const int32_t* __restrict__ list,
+                                                        const int32_t* __restrict__ prow_ptr,
+                                                        const int32_t* __restrict__ pcol,
+                                                        const double* __restrict__ pnw,
+                                                        const double* __restrict__ yp,
+                                                        double* __restrict__ zp) {
+  constexpr int kGroups = 256 / 9;
+  __shared__ double2 part[kGroups][kSub / 2];
+  const int64_t p = list[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int grp = tid / 9, lj = tid % 9;
+  const double2* y2 = reinterpret_cast<const double2*>(yp);
+  const int32_t e0 = prow_ptr[p], e1 = prow_ptr[p + 1];
+  if (grp < kGroups) {
+    double2 acc = {0.0, 0.0};
+    for (int32_t e = e0 + grp; e < e1; e += kGroups) {
+      const double wv = pnw[e];
+      const double2 q = y2[static_cast<int64_t>(pcol[e]) * (kSub / 2) + lj];
+      acc.x += wv * q.x;
+      acc.y += wv * q.y;
+    }
+    part[grp][lj] = acc;
+  }
+  __syncthreads();
+  if (tid < kSub / 2) {
+    double2 acc = y2[p * (kSub / 2) + tid];
+    for (int g = 0; g < kGroups; ++g) {
+      acc.x += part[g][tid].x;
+      acc.y += part[g][tid].y;
+    }
+    reinterpret_cast<double2*>(zp)[p * (kSub / 2) + tid] = acc;
+  }
+}
+
+// The item's share of Z^T Z (or Y^T Z when CROSS) added to gram[slot]; the
+// rows are already computed.  zero_buf is cleared on the way (see
+// zero_range).
+template <bool CROSS>
+__global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ items,
+                                                   const double* __restrict__ yp,
+                                                   const double* __restrict__ zp,
+                                                   double* __restrict__ gram,
+                                                   double* __restrict__ zero_buf, int zero_len) {
+  __shared__ double sz[kItemRows][kSub + 1];
+  __shared__ double sy[CROSS ? kItemRows : 1][kSub + 1];
+  zero_range(zero_buf, zero_len);
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int32_t slot = items[item * 3 + 0];
+  const int32_t p0 = items[item * 3 + 1];
+  const int32_t cnt = items[item * 3 + 2] - p0;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < cnt * kSub; q += 256) {
+    const int r = q / kSub, c = q % kSub;
+    sz[r][c] = zp[(static_cast<int64_t>(p0) + r) * kSub + c];
+    if (CROSS) sy[r][c] = yp[(static_cast<int64_t>(p0) + r) * kSub + c];
+  }
+  __syncthreads();
+  if (tid < kSub * (kSub + 1) / 2) {
+    int pi, pj;
+    pair_of(tid, pi, pj);
+    double acc = 0.0;
+    if (CROSS) {
+      for (int r = 0; r < cnt; ++r) acc += sy[r][pi] * sz[r][pj];
+    } else {
+      for (int r = 0; r < cnt; ++r) acc += sz[r][pi] * sz[r][pj];
+    }
+    atomicAdd(&gram[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
+  }
 }
 
 // Cholesky of gram[slot] (upper R, G = R^T R) in LDS, then dst = src R^-1 for
@@ -819,26 +828,42 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
       KERNEL_CHECK();
     };
     // start block: Z = Y, g1 = Y^T Y
-    hipLaunchKernelGGL((iter_mul_gram_kernel<false, false>), dim3(n_items), dim3(256), 0, s, it_items,
-                       prow_ptr.get(), pcol.get(), pnw.get(), yb, zb, g1, g2, glen);
+    HIP_CHECK(hipMemcpyAsync(zb, yb, sizeof(double) * rows * kSub, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1, g2,
+                       glen);
     KERNEL_CHECK();
     orth_into_y();
+    // rows too long for the 9-lane path
+    DevBuf<int32_t> long_list(rows), long_count(1);
+    HIP_CHECK(hipMemsetAsync(long_count.get(), 0, 4, s));
+    hipLaunchKernelGGL(long_rows_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                       prow_ptr.get(), long_list.get(), long_count.get());
+    KERNEL_CHECK();
+    int32_t n_long = 0;
+    HIP_CHECK(hipMemcpyAsync(&n_long, long_count.get(), 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    auto product = [&](const double* src, double* dst) {
+      hipLaunchKernelGGL(spmm_short_kernel, dim3(blocks_for(rows * 9, 256)), dim3(256), 0, s, rows,
+                         prow_ptr.get(), pcol.get(), pnw.get(), src, dst);
+      if (n_long > 0)
+        hipLaunchKernelGGL(spmm_long_kernel, dim3(n_long), dim3(256), 0, s, long_list.get(),
+                           prow_ptr.get(), pcol.get(), pnw.get(), src, dst);
+      KERNEL_CHECK();
+    };
     for (int it = 0; it < kIters; ++it) {
+      product(yb, zb);
       if ((it + 1) % kOrthEvery == 0 || it + 1 == kIters) {
-        hipLaunchKernelGGL((iter_mul_gram_kernel<true, false>), dim3(n_items), dim3(256), 0, s, it_items,
-                           prow_ptr.get(), pcol.get(), pnw.get(), yb, zb, g1, g2, glen);
+        hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1,
+                           g2, glen);
         KERNEL_CHECK();
         orth_into_y();
       } else {
-        hipLaunchKernelGGL(spmm_kernel, dim3(blocks_for(rows * 9, 256)), dim3(256), 0, s, rows,
-                           prow_ptr.get(), pcol.get(), pnw.get(), yb, zb);
-        KERNEL_CHECK();
         std::swap(yb, zb);
       }
     }
     // Ritz matrix Y^T (I + N) Y, upper triangle
-    hipLaunchKernelGGL((iter_mul_gram_kernel<true, true>), dim3(n_items), dim3(256), 0, s, it_items,
-                       prow_ptr.get(), pcol.get(), pnw.get(), yb, zb, g1,
+    product(yb, zb);
+    hipLaunchKernelGGL(gram_kernel<true>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1,
                        static_cast<double*>(nullptr), 0);
     KERNEL_CHECK();
     std::vector<double> ritz(static_cast<size_t>(n_slots) * kSub * kSub);
