@@ -99,3 +99,25 @@ def test_components_match_oracle(gpu, oracle, mixture):
     nc_o, lab_o = oracle.connected_components(mixture.shape[0], h, t)
     nc_g, lab_g = capi.components(mixture.shape[0], h, t, w)
     assert nc_g == nc_o and np.array_equal(lab_g, lab_o)
+
+
+def test_fuzzy_union_hub_rows(gpu, oracle):
+    # Plant two hubs in a real kNN graph: every row also points at row 0
+    # (in-degree ~20000, the serial path) and 2500 rows point at row 1
+    # (in-degree ~2500, the block path of up to 16384 entries); the remaining
+    # rows with more than 64 union entries take the block path of up to 1024.
+    x = datasets.gaussian_mixture(20000, 16, 8, 3.0, 41, 1.0)
+    idx, dist = oracle.knn(x, 15)
+    idx = idx.copy()
+    for r in range(2, idx.shape[0]):
+        if 0 not in idx[r]:
+            idx[r, 14] = 0
+        if r < 2502 and 1 not in idx[r]:
+            idx[r, 13] = 1
+    sig, rho = oracle.smooth_knn_dist(dist, 15.0, 1.0, include_self=True)
+    h_o, t_o, w_o = oracle.fuzzy_graph(idx, dist, sig, rho, 1.0, 0)
+    h_g, t_g, w_g = capi.fuzzy_graph(idx, dist, sig, rho, 1.0, 0)
+    indeg = np.bincount(idx[:, 1:].ravel(), minlength=idx.shape[0])
+    assert indeg[0] > 16384 and 1024 < indeg[1] <= 16384
+    assert np.array_equal(h_g, h_o) and np.array_equal(t_g, t_o)
+    assert np.all(np.abs(w_g - w_o) <= 1e-5 * w_o)

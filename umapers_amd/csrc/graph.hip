@@ -301,22 +301,32 @@ __device__ void union_row_serial(const int32_t* __restrict__ idx, const float* _
 // loaded one per lane with key 2*col + (incoming), sorted by a register
 // bitonic network, and adjacent equal columns are combined with the
 // arithmetic of lib.rs:3611-3646; surviving entries are compacted in column
-// order.  Rows with more than 64 entries take union_row_serial.
+// order.  Rows with more than 64 entries are listed for union_long_kernel
+// (up to kUnionBig entries) or take union_row_serial.
+constexpr int kUnionMid = 1024;   // long rows handled with a small LDS footprint
+constexpr int kUnionBig = 16384;  // longest row sorted in LDS
 __global__ __launch_bounds__(256) void union_rows_kernel(
     const int32_t* __restrict__ idx, const float* __restrict__ wdir, int64_t n, int k,
     const int32_t* __restrict__ in_off, int32_t* __restrict__ in_src, float* __restrict__ in_w,
     float mix, int32_t* __restrict__ tmp_col, float* __restrict__ tmp_w,
     int32_t* __restrict__ ucount, uint32_t* __restrict__ wmax_bits,
-    int32_t* __restrict__ serial_c, float* __restrict__ serial_w) {
+    int32_t* __restrict__ serial_c, float* __restrict__ serial_w, int32_t* __restrict__ mid_list,
+    int32_t* __restrict__ big_list, int32_t* __restrict__ long_count) {
   const int lane = threadIdx.x & 63;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= n) return;
   const int32_t b0 = in_off[i], b1 = in_off[i + 1];
   const int m_in = b1 - b0;
   if (k + m_in > 64) {
-    if (lane == 0)
-      union_row_serial(idx, wdir, i, k, b0, b1, in_src, in_w, mix, tmp_col, tmp_w, ucount,
-                       wmax_bits, serial_c + i * k, serial_w + i * k);
+    if (lane == 0) {
+      if (k + m_in <= kUnionMid)
+        mid_list[atomicAdd(&long_count[0], 1)] = static_cast<int32_t>(i);
+      else if (k + m_in <= kUnionBig)
+        big_list[atomicAdd(&long_count[1], 1)] = static_cast<int32_t>(i);
+      else
+        union_row_serial(idx, wdir, i, k, b0, b1, in_src, in_w, mix, tmp_col, tmp_w, ucount,
+                         wmax_bits, serial_c + i * k, serial_w + i * k);
+    }
     return;
   }
   int key = INT_MAX;
@@ -378,6 +388,127 @@ __global__ __launch_bounds__(256) void union_rows_kernel(
     const int cnt = __popcll(bal);
     ucount[i] = cnt;
     if (cnt > 0) atomicMax(wmax_bits, __float_as_uint(mx));
+  }
+}
+
+// Block-per-row union for rows with more than 64 entries: the same keys as
+// union_rows_kernel sorted by a bitonic network in LDS over the next power of
+// two of the row's length (at most cap), equal columns combined, survivors
+// compacted by a block prefix sum.  Dynamic LDS: cap * 8 bytes + 1 KiB.
+__global__ __launch_bounds__(256) void union_long_kernel(
+    const int32_t* __restrict__ list, int cap, const int32_t* __restrict__ idx,
+    const float* __restrict__ wdir, int k, const int32_t* __restrict__ in_off,
+    const int32_t* __restrict__ in_src, const float* __restrict__ in_w, float mix,
+    int32_t* __restrict__ tmp_col, float* __restrict__ tmp_w, int32_t* __restrict__ ucount,
+    uint32_t* __restrict__ wmax_bits) {
+  extern __shared__ __align__(16) unsigned char lds_u[];
+  int* skey = reinterpret_cast<int*>(lds_u);
+  float* sw = reinterpret_cast<float*>(lds_u + 4 * cap);
+  int* part = reinterpret_cast<int*>(lds_u + 8 * cap);  // 256 partial counts
+  const int tid = threadIdx.x;
+  const int64_t i = list[blockIdx.x];
+  const int32_t b0 = in_off[i], b1 = in_off[i + 1];
+  const int len = k + (b1 - b0);
+  int lp = 64;
+  while (lp < len) lp <<= 1;
+  for (int q = tid; q < lp; q += 256) {
+    int key = INT_MAX;
+    float w = 0.f;
+    if (q < k) {
+      const float wv = wdir[i * k + q];
+      const int32_t j = idx[i * k + q];
+      if (wv > 0.f && j != i) {
+        key = 2 * j;
+        w = wv;
+      }
+    } else if (q < len) {
+      key = 2 * in_src[b0 + q - k] + 1;
+      w = in_w[b0 + q - k];
+    }
+    skey[q] = key;
+    sw[q] = w;
+  }
+  __syncthreads();
+  for (int size = 2; size <= lp; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = tid; t < lp / 2; t += 256) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const int kl = skey[lo], kh = skey[hi];
+        if ((kl > kh) == up) {
+          skey[lo] = kh;
+          skey[hi] = kl;
+          const float wt = sw[lo];
+          sw[lo] = sw[hi];
+          sw[hi] = wt;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // survivors: first entry of each column with a positive symmetric weight;
+  // thread tid scans the contiguous run [tid * per, (tid + 1) * per)
+  const int per = lp / 256 > 0 ? lp / 256 : 1;
+  const int q0 = tid * per;
+  auto sym_at = [&](int q, float& out) {
+    const int key = skey[q];
+    if (key == INT_MAX) return false;
+    const int col = key >> 1;
+    if (q > 0 && skey[q - 1] != INT_MAX && (skey[q - 1] >> 1) == col) return false;
+    float wij = 0.f, wji = 0.f;
+    if ((key & 1) == 0) {
+      wij = sw[q];
+      if (q + 1 < lp && skey[q + 1] != INT_MAX && (skey[q + 1] >> 1) == col) wji = sw[q + 1];
+    } else {
+      wji = sw[q];
+    }
+    const float prod = fmul(wij, wji);
+    out = fadd(fmul(mix, fsub(fadd(wij, wji), prod)), fmul(fsub(1.f, mix), prod));
+    return out > 0.f;
+  };
+  int mine = 0;
+  float mx = 0.f;
+  for (int q = q0; q < q0 + per && q < lp; ++q) {
+    float v;
+    if (sym_at(q, v)) {
+      ++mine;
+      mx = fmaxf(mx, v);
+    }
+  }
+  part[tid] = mine;
+  __syncthreads();
+  // exclusive scan of the 256 partial counts (thread 0; 256 adds)
+  __shared__ int total;
+  if (tid == 0) {
+    int run = 0;
+    for (int t = 0; t < 256; ++t) {
+      const int c = part[t];
+      part[t] = run;
+      run += c;
+    }
+    total = run;
+  }
+  __syncthreads();
+  int pos = part[tid];
+  const int64_t base = i * k + b0;
+  for (int q = q0; q < q0 + per && q < lp; ++q) {
+    float v;
+    if (sym_at(q, v)) {
+      tmp_col[base + pos] = skey[q] >> 1;
+      tmp_w[base + pos] = v;
+      ++pos;
+    }
+  }
+  // row maximum
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  __shared__ float wmx[4];
+  if ((tid & 63) == 0) wmx[tid >> 6] = mx;
+  __syncthreads();
+  if (tid == 0) {
+    ucount[i] = total;
+    const float m4 = fmaxf(fmaxf(wmx[0], wmx[1]), fmaxf(wmx[2], wmx[3]));
+    if (total > 0) atomicMax(wmax_bits, __float_as_uint(m4));
   }
 }
 
@@ -633,11 +764,32 @@ void fuzzy_graph_device(const int32_t* idx, const float* dist, const float* sigm
   if (k > 64) throw_invalid("n_neighbors must be <= 64 on the GPU path");
   ws.serial_c.ensure(nk);
   ws.serial_w.ensure(nk);
+  ws.long_rows.ensure(2 * n + 2);
+  int32_t* mid_list = ws.long_rows.get();
+  int32_t* big_list = ws.long_rows.get() + n;
+  int32_t* long_count = ws.long_rows.get() + 2 * n;
+  HIP_CHECK(hipMemsetAsync(long_count, 0, 8, s));
   hipLaunchKernelGGL(union_rows_kernel, dim3(blocks_for(n, 4)), dim3(256), 0, s, idx,
                      ws.wdir.get(), n, k, ws.in_off.get(), ws.in_src.get(), ws.in_w.get(), mix,
                      ws.tmp_col.get(), ws.tmp_w.get(), ws.ucount.get(), ws.umax.get(),
-                     ws.serial_c.get(), ws.serial_w.get());
+                     ws.serial_c.get(), ws.serial_w.get(), mid_list, big_list, long_count);
   KERNEL_CHECK();
+  int32_t n_long[2] = {0, 0};
+  HIP_CHECK(hipMemcpyAsync(n_long, long_count, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  auto long_rows = [&](const int32_t* list, int count, int cap) {
+    if (count == 0) return;
+    const size_t bytes = static_cast<size_t>(cap) * 8 + 1024;
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(union_long_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(bytes)));
+    hipLaunchKernelGGL(union_long_kernel, dim3(count), dim3(256), bytes, s, list, cap, idx,
+                       ws.wdir.get(), k, ws.in_off.get(), ws.in_src.get(), ws.in_w.get(), mix,
+                       ws.tmp_col.get(), ws.tmp_w.get(), ws.ucount.get(), ws.umax.get());
+    KERNEL_CHECK();
+  };
+  long_rows(mid_list, n_long[0], kUnionMid);
+  long_rows(big_list, n_long[1], kUnionBig);
   hipLaunchKernelGGL(tmp_offsets_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                      ws.in_off.get(), n, k, ws.tmp_off.get());
   KERNEL_CHECK();

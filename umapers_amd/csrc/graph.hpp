@@ -35,6 +35,7 @@ struct GraphWorkspace {
   DevBuf<float> serial_w;
   DevBuf<float> tmp_w;
   DevBuf<int32_t> flag;
+  DevBuf<int32_t> long_rows;  // rows with more than 64 union entries, and their counts
   DevBuf<double> dsum;
   DevBuf<uint32_t> umax;
   ScanWorkspace scan;
