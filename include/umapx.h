@@ -206,6 +206,28 @@ int32_t umapx_optimize_layout(float* y, int64_t n, int32_t dim, const int64_t* h
                               uint64_t seed, const float* dens_radii, int32_t dens_start_epoch,
                               float dens_lambda, float dens_var_shift, int32_t* pos_counts,
                               int32_t* neg_counts);
+/* ------------------------------------------------------------------------
+ * Learned transform (the reference's nearest analog is ParametricUmapModel,
+ * rust_umap/src/parametric.rs:89-534; the encoder and its loss are this
+ * library's own, see csrc/mlp.hip).
+ * ------------------------------------------------------------------------ */
+
+/* Number of f32 parameters of the encoder d -> hidden -> hidden -> dim; the
+ * flat layout is W1 [dp x hidden], b1, W2 [hidden x hidden], b2,
+ * W3 [hidden x dim], b3 with dp = d rounded up to a multiple of 32. */
+int32_t umapx_mlp_param_count(int32_t d, int32_t hidden, int32_t dim, int64_t* count);
+/* One forward / backward pass of the encoder's training loss on a given
+ * batch: rows are n_edges heads, n_edges tails, then n_edges * n_neg
+ * negatives (negative t of edge e at row 2 n_edges + e n_neg + t); x_rows are
+ * raw features (standardised with mean / inv_std), y_rows the fitted
+ * embedding.  Writes the parameter gradients, the loss and the outputs z. */
+int32_t umapx_mlp_batch_grad(int32_t d, int32_t hidden, int32_t dim, const float* params,
+                             const float* mean, const float* inv_std, const float* x_rows,
+                             const float* y_rows, int32_t n_edges, int32_t n_neg, float a,
+                             float b, float mse_weight, float* grads, float* loss, float* z_out);
+/* Training loss of the learned transform at its first and last step. */
+int32_t umapx_get_mlp_loss(const umapx_handle* h, double* first, double* last, int32_t* steps);
+
 /* local_radii_from_knn / embedding_radii_from_knn (lib.rs:5057-5108). */
 int32_t umapx_local_radii(const int64_t* idx, const float* dist, int64_t n, int32_t k,
                           float var_shift, float* out);

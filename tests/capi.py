@@ -119,3 +119,21 @@ def embedding_radii(y, idx, vs=0.1):
     L.check(L.lib().umapx_embedding_radii(L.ptr(y), y.shape[0], y.shape[1], L.ptr(idx, L._i64p),
                                           idx.shape[1], vs, L.ptr(out)))
     return out
+
+
+def mlp_param_count(d, h, dim):
+    out = ctypes.c_int64()
+    L.check(L.lib().umapx_mlp_param_count(d, h, dim, ctypes.byref(out)))
+    return out.value
+
+
+def mlp_batch_grad(params, mean, inv_std, x_rows, y_rows, d, h, dim, n_edges, n_neg, a, b, lam):
+    params, mean, inv_std = _f32(params), _f32(mean), _f32(inv_std)
+    x_rows, y_rows = _f32(x_rows), _f32(y_rows)
+    grads = np.empty_like(params)
+    z = np.empty((x_rows.shape[0], dim), np.float32)
+    loss = ctypes.c_float()
+    L.check(L.lib().umapx_mlp_batch_grad(d, h, dim, L.ptr(params), L.ptr(mean), L.ptr(inv_std),
+                                         L.ptr(x_rows), L.ptr(y_rows), n_edges, n_neg, a, b, lam,
+                                         L.ptr(grads), ctypes.pointer(loss), L.ptr(z)))
+    return loss.value, grads, z

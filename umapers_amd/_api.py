@@ -193,6 +193,12 @@ class _Core:
         L.check(self._lib.umapx_get_timings(self._h, ctypes.byref(t)))
         return t.as_dict()
 
+    def mlp_loss(self) -> tuple[float, float, int]:
+        first, last, steps = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+        L.check(self._lib.umapx_get_mlp_loss(self._h, ctypes.byref(first), ctypes.byref(last),
+                                             ctypes.byref(steps)))
+        return first.value, last.value, steps.value
+
     def transform_into(self, arr: np.ndarray, out: np.ndarray, mode: str | None = None) -> None:
         fn = {None: self._lib.umapx_transform, "plain": self._lib.umapx_transform_plain,
               "learned": self._lib.umapx_transform_learned}[mode]
@@ -475,6 +481,11 @@ class Umap:
     def transform_learned(self, query: Any) -> np.ndarray:
         """One pass through the learned encoder (trained on first use)."""
         return self._core.transform(_as_f32_matrix(query, "query"), "learned")
+
+    def learned_transform_loss(self) -> tuple[float, float, int]:
+        """Training loss of the learned encoder at its first and last step, and
+        the number of steps (raises when it has not been trained)."""
+        return self._core.mlp_loss()
 
     def inverse_transform(self, embedded_query: Any, *, out: np.ndarray | None = None):
         raise NotImplementedError("inverse_transform is not available on the GPU path")

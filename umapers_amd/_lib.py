@@ -133,6 +133,15 @@ SIGNATURES = {
     "umapx_get_ab": (ctypes.c_int32, [_vp, _f32p, _f32p]),
     "umapx_get_radii": (ctypes.c_int32, [_vp, _f32p, _f32p]),
     "umapx_find_ab_params": (ctypes.c_int32, [ctypes.c_float, ctypes.c_float, _f32p, _f32p]),
+    "umapx_mlp_param_count": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.POINTER(ctypes.c_int64)]),
+    "umapx_mlp_batch_grad": (ctypes.c_int32, [
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p,
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p,
+        _f32p, _f32p]),
+    "umapx_get_mlp_loss": (ctypes.c_int32, [_vp, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_int32)]),
     "umapx_knn": (ctypes.c_int32, [_f32p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i64p, _f32p]),
     "umapx_knn_to_reference": (ctypes.c_int32, [

@@ -661,6 +661,37 @@ int32_t umapx_transform_device(umapx_handle* h, const float* d_q, int64_t m, int
   });
 }
 
+int32_t umapx_mlp_param_count(int32_t d, int32_t hidden, int32_t dim, int64_t* count) {
+  return guarded([&] {
+    if (!count) throw_invalid("count must not be null");
+    if (d < 1 || hidden < 64 || hidden % 64 != 0 || dim < 1 || dim > 8)
+      throw_invalid("need d >= 1, hidden a positive multiple of 64 and 1 <= dim <= 8");
+    *count = MlpShape::make(d, hidden, dim).total;
+  });
+}
+
+int32_t umapx_mlp_batch_grad(int32_t d, int32_t hidden, int32_t dim, const float* params,
+                             const float* mean, const float* inv_std, const float* x_rows,
+                             const float* y_rows, int32_t n_edges, int32_t n_neg, float a,
+                             float b, float mse_weight, float* grads, float* loss, float* z_out) {
+  return guarded([&] {
+    if (!params || !mean || !inv_std || !x_rows || !y_rows || !grads || !loss)
+      throw_invalid("null argument");
+    mlp_batch_grad(d, hidden, dim, params, mean, inv_std, x_rows, y_rows, n_edges, n_neg, a, b,
+                   mse_weight, grads, loss, z_out);
+  });
+}
+
+int32_t umapx_get_mlp_loss(const umapx_handle* h, double* first, double* last, int32_t* steps) {
+  return guarded([&] {
+    require_handle(h);
+    if (!h->mlp.trained) throw Error(kNotFitted, "the learned transform has not been trained");
+    if (first) *first = h->mlp.first_loss;
+    if (last) *last = h->mlp.last_loss;
+    if (steps) *steps = h->mlp.steps_done;
+  });
+}
+
 int32_t umapx_transform_learned_device(umapx_handle* h, const float* d_q, int64_t m, int64_t d,
                                        float* d_out) {
   return guarded([&] {

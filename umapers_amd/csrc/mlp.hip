@@ -1,14 +1,709 @@
+// Learned transform on gfx950: MLP encoder training and inference.
+//
+// Network: x (standardised, BF16, features padded to dp) -> H1 = relu(x W1 +
+// b1) -> H2 = relu(H1 W2 + b2) -> z = H2 W3 + b3.  The two wide layers run on
+// one hand-written BF16 MFMA GEMM (32x32x16, FP32 accumulation, 64x64 block
+// tiles staged through LDS) with fused epilogues: bias + ReLU in the forward
+// pass, the ReLU mask in the backward pass, and split-K atomic accumulation
+// for the weight gradients.  The narrow output layer and the loss gradient
+// are VALU kernels.  FP32 master weights are updated by Adam; the GEMMs read
+// their BF16 copies.
+//
+// Loss per batch of B edges (i, j) drawn with probability proportional to
+// their fuzzy-graph weight, each with n_neg uniformly drawn negatives k:
+//   L = (1/B) sum_e [ log(1 + a s_ij^b) + sum_k log(1 + 1 / (a (s_ik + eps)^b)) ]
+//       + lambda / S * sum_rows |z - y|^2
+// with s = |z_u - z_v|^2, eps = 1e-3, S = B (2 + n_neg) rows and y the fitted
+// embedding of each row.  The first term is UMAP's cross-entropy with
+// q = 1 / (1 + a s^b) (attraction -log q, repulsion -log(1 - q)); the second
+// keeps the encoder in the coordinates of the plain fit, as the reference's
+// parametric model regresses on its teacher (parametric.rs:391-393).
 #include "mlp.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
 
 namespace umapx {
 
-void mlp_train(MlpModel&, const float*, int64_t, int64_t, const float*, int, const Graph&, float,
-               float, const umapx_params&, hipStream_t) {
-  throw_invalid("learned transform is not available in this build");
+MlpShape MlpShape::make(int d, int h, int dim) {
+  MlpShape s;
+  s.d = d;
+  s.dp = (d + 31) / 32 * 32;
+  s.h = h;
+  s.dim = dim;
+  int64_t o = 0;
+  s.off_w1 = o;
+  o += static_cast<int64_t>(s.dp) * h;
+  s.off_b1 = o;
+  o += h;
+  s.off_w2 = o;
+  o += static_cast<int64_t>(h) * h;
+  s.off_b2 = o;
+  o += h;
+  s.off_w3 = o;
+  o += static_cast<int64_t>(h) * dim;
+  s.off_b3 = o;
+  o += dim;
+  s.total = o;
+  return s;
 }
 
-void mlp_forward_rows(const MlpModel&, const float*, int64_t, float*, hipStream_t) {
-  throw_invalid("learned transform is not available in this build");
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxDim = 8;
+constexpr float kRepulsionEps = 1e-3f;
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) {
+  return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+__device__ __forceinline__ uint16_t lane16(const u32x4& v, int i) {
+  return static_cast<uint16_t>(v[i >> 1] >> (16 * (i & 1)));
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+uint64_t mix64_host(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double uniform01(uint64_t key) {
+  return static_cast<double>(mix64(key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+constexpr int kBM = 64, kBN = 64, kBK = 32, kLdsK = kBK + 8;
+enum { kEpiBiasRelu = 0, kEpiMask = 1, kEpiAtomic = 2 };
+
+// C = A B, A [M x K] and B [K x N] in BF16, FP32 accumulation on MFMA
+// 32x32x16.  A_T: A is stored [K x M] (row stride lda); B_T: B is stored
+// [N x K].  Each of the 4 waves owns a 32x32 quarter of the 64x64 block tile;
+// K advances 32 at a time through LDS, where both operands sit with K
+// contiguous so each lane's MFMA fragment is one 16-byte read.  blockIdx.z
+// selects the K range [z * k_chunk, (z + 1) * k_chunk).  Requires M, N, K
+// and k_chunk to be multiples of 8 (every 16-byte vector lies inside them).
+template <bool A_T, bool B_T, int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(
+    const uint16_t* __restrict__ A, int lda, const uint16_t* __restrict__ B, int ldb, int M,
+    int N, int K, int k_chunk, const float* __restrict__ bias, const uint16_t* __restrict__ mask,
+    uint16_t* __restrict__ Cb, float* __restrict__ Cf, int ldc) {
+  __shared__ __align__(16) uint16_t As[kBM][kLdsK];
+  __shared__ __align__(16) uint16_t Bs[kBN][kLdsK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+  const int kb = blockIdx.z * k_chunk;
+  const int ke = min(K, kb + k_chunk);
+  floatx16 acc;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+  const u32x4 zero = {0u, 0u, 0u, 0u};
+  for (int k0 = kb; k0 < ke; k0 += kBK) {
+    if (!A_T) {
+      const int r = tid >> 2, kv = (tid & 3) * 8;
+      const int gm = m0 + r, gk = k0 + kv;
+      u32x4 v = zero;
+      if (gm < M && gk < ke) v = *reinterpret_cast<const u32x4*>(A + static_cast<int64_t>(gm) * lda + gk);
+      *reinterpret_cast<u32x4*>(&As[r][kv]) = v;
+    } else {
+      const int kk = tid >> 3, mv = (tid & 7) * 8;
+      const int gk = k0 + kk, gm = m0 + mv;
+      u32x4 v = zero;
+      if (gk < ke && gm < M) v = *reinterpret_cast<const u32x4*>(A + static_cast<int64_t>(gk) * lda + gm);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) As[mv + i][kk] = lane16(v, i);
+    }
+    if (!B_T) {
+      const int kk = tid >> 3, nv = (tid & 7) * 8;
+      const int gk = k0 + kk, gn = n0 + nv;
+      u32x4 v = zero;
+      if (gk < ke && gn < N) v = *reinterpret_cast<const u32x4*>(B + static_cast<int64_t>(gk) * ldb + gn);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Bs[nv + i][kk] = lane16(v, i);
+    } else {
+      const int r = tid >> 2, kv = (tid & 3) * 8;
+      const int gn = n0 + r, gk = k0 + kv;
+      u32x4 v = zero;
+      if (gn < N && gk < ke) v = *reinterpret_cast<const u32x4*>(B + static_cast<int64_t>(gn) * ldb + gk);
+      *reinterpret_cast<u32x4*>(&Bs[r][kv]) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 16) {
+      const u32x4 af = *reinterpret_cast<const u32x4*>(&As[wm * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+      const u32x4 bf = *reinterpret_cast<const u32x4*>(&Bs[wn * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, af),
+                                                    __builtin_bit_cast(bf16x8, bf), acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int row = m0 + wm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+    const int col = n0 + wn * 32 + (lane & 31);
+    if (row >= M || col >= N) continue;
+    const int64_t o = static_cast<int64_t>(row) * ldc + col;
+    if (EPI == kEpiBiasRelu) {
+      Cb[o] = to_bf16(fmaxf(acc[i] + bias[col], 0.f));
+    } else if (EPI == kEpiMask) {
+      const uint16_t mv = mask[o];
+      Cb[o] = ((mv & 0x8000u) == 0 && (mv & 0x7FFFu) != 0) ? to_bf16(acc[i]) : uint16_t{0};
+    } else {
+      atomicAdd(&Cf[o], acc[i]);
+    }
+  }
+}
+
+template <bool A_T, bool B_T, int EPI>
+void gemm(const uint16_t* A, int lda, const uint16_t* B, int ldb, int M, int N, int K, int splits,
+          const float* bias, const uint16_t* mask, uint16_t* Cb, float* Cf, int ldc, hipStream_t s) {
+  const int k_chunk = ((K + splits - 1) / splits + kBK - 1) / kBK * kBK;
+  const dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM, (K + k_chunk - 1) / k_chunk);
+  hipLaunchKernelGGL((gemm_bf16_kernel<A_T, B_T, EPI>), grid, dim3(256), 0, s, A, lda, B, ldb, M,
+                     N, K, k_chunk, bias, mask, Cb, Cf, ldc);
+  KERNEL_CHECK();
+}
+
+// Standardised, BF16, zero-padded copy of the listed rows (row index < 0, or
+// feature >= d, gives 0); the fitted embedding of each row alongside.
+__global__ void gather_rows_kernel(const float* __restrict__ x, int d, int dp,
+                                   const int32_t* __restrict__ rows, int64_t first_row,
+                                   int64_t count, const float* __restrict__ mean,
+                                   const float* __restrict__ inv_std, uint16_t* __restrict__ xb,
+                                   const float* __restrict__ y, int dim, float* __restrict__ yt) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count * dp) return;
+  const int64_t s = t / dp;
+  const int f = static_cast<int>(t % dp);
+  const int64_t r = rows ? static_cast<int64_t>(rows[s]) : first_row + s;
+  float v = 0.f;
+  if (r >= 0 && f < d) v = (x[r * d + f] - mean[f]) * inv_std[f];
+  xb[t] = to_bf16(v);
+  if (yt && f < dim) yt[s * dim + f] = (r >= 0) ? y[r * dim + f] : 0.f;
+}
+
+// z = H2 W3 + b3, one wave per row (W3 from its BF16 copy).
+__global__ __launch_bounds__(256) void out_layer_kernel(const uint16_t* __restrict__ h2,
+                                                        int64_t rows, int h, int dim,
+                                                        const uint16_t* __restrict__ w3h,
+                                                        const float* __restrict__ b3,
+                                                        float* __restrict__ z) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (r >= rows) return;
+  float acc[kMaxDim];
+#pragma unroll
+  for (int c = 0; c < kMaxDim; ++c) acc[c] = 0.f;
+  for (int j = lane; j < h; j += 64) {
+    const float hv = from_bf16(h2[r * h + j]);
+#pragma unroll
+    for (int c = 0; c < kMaxDim; ++c)
+      if (c < dim) acc[c] += hv * from_bf16(w3h[j * dim + c]);
+  }
+#pragma unroll
+  for (int c = 0; c < kMaxDim; ++c)
+    for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o);
+  if (lane < dim) {
+    float v = 0.f;
+#pragma unroll
+    for (int c = 0; c < kMaxDim; ++c)
+      if (c == lane) v = acc[c];
+    z[r * dim + lane] = v + b3[lane];
+  }
+}
+
+// dL/dz for every row of the batch and the batch loss (one thread per edge;
+// every row belongs to exactly one edge, so no atomics on dz).
+__global__ void loss_grad_kernel(const float* __restrict__ z, const float* __restrict__ yt,
+                                 int n_edges, int n_neg, int dim, int s_real, float a, float b,
+                                 float lambda, float* __restrict__ dz, double* __restrict__ loss) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (e < n_edges) {
+    const float inv_b = 1.f / static_cast<float>(n_edges);
+    const float mse_c = 2.f * lambda / static_cast<float>(s_real);
+    const int64_t hi = e, ti = static_cast<int64_t>(n_edges) + e;
+    float zi[kMaxDim], gi[kMaxDim];
+    for (int c = 0; c < dim; ++c) {
+      zi[c] = z[hi * dim + c];
+      gi[c] = 0.f;
+    }
+    auto mse = [&](int64_t row, const float* zr, float* g) {
+      float sq = 0.f;
+      for (int c = 0; c < dim; ++c) {
+        const float df = zr[c] - yt[row * dim + c];
+        g[c] += mse_c * df;
+        sq += df * df;
+      }
+      return static_cast<double>(lambda) * sq / s_real;
+    };
+    // attraction
+    {
+      float zj[kMaxDim], gj[kMaxDim];
+      float s = 0.f;
+      for (int c = 0; c < dim; ++c) {
+        zj[c] = z[ti * dim + c];
+        gj[c] = 0.f;
+        const float df = zi[c] - zj[c];
+        s += df * df;
+      }
+      if (s > 0.f) {
+        const float pb = powf(s, b);
+        const float P = a * pb;
+        l += log1p(static_cast<double>(P)) * inv_b;
+        const float dlds = a * b * (pb / s) / (1.f + P);
+        for (int c = 0; c < dim; ++c) {
+          const float g = 2.f * dlds * (zi[c] - zj[c]) * inv_b;
+          gi[c] += g;
+          gj[c] -= g;
+        }
+      }
+      l += mse(ti, zj, gj);
+      for (int c = 0; c < dim; ++c) dz[ti * dim + c] = gj[c];
+    }
+    // repulsion
+    for (int t = 0; t < n_neg; ++t) {
+      const int64_t ki = 2 * static_cast<int64_t>(n_edges) + static_cast<int64_t>(e) * n_neg + t;
+      float zk[kMaxDim], gk[kMaxDim];
+      float s = 0.f;
+      for (int c = 0; c < dim; ++c) {
+        zk[c] = z[ki * dim + c];
+        gk[c] = 0.f;
+        const float df = zi[c] - zk[c];
+        s += df * df;
+      }
+      const float se = s + kRepulsionEps;
+      const float P = a * powf(se, b);
+      l += log1p(1.0 / static_cast<double>(P)) * inv_b;
+      const float dlds = -b / (se * (1.f + P));
+      for (int c = 0; c < dim; ++c) {
+        const float g = 2.f * dlds * (zi[c] - zk[c]) * inv_b;
+        gi[c] += g;
+        gk[c] -= g;
+      }
+      l += mse(ki, zk, gk);
+      for (int c = 0; c < dim; ++c) dz[ki * dim + c] = gk[c];
+    }
+    l += mse(hi, zi, gi);
+    for (int c = 0; c < dim; ++c) dz[hi * dim + c] = gi[c];
+  }
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+  if (loss && (threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(loss, l);
+}
+
+// dH2 = (dz W3^T) masked by H2 > 0, in BF16.
+__global__ void out_layer_back_kernel(const float* __restrict__ dz, const uint16_t* __restrict__ h2,
+                                      int64_t rows, int h, int dim,
+                                      const uint16_t* __restrict__ w3h, uint16_t* __restrict__ dh2) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= rows * h) return;
+  const int64_t r = t / h;
+  const int j = static_cast<int>(t % h);
+  float v = 0.f;
+  if (from_bf16(h2[t]) > 0.f)
+    for (int c = 0; c < dim; ++c) v += dz[r * dim + c] * from_bf16(w3h[j * dim + c]);
+  dh2[t] = to_bf16(v);
+}
+
+// Column sums over a block of rows, added atomically: out[j] += sum_r m[r][j]
+// (BF16 input), and, for the output layer, w[j][c] += sum_r h2[r][j] dz[r][c]
+// and bias[c] += sum_r dz[r][c].
+__global__ __launch_bounds__(256) void colsum_bf16_kernel(const uint16_t* __restrict__ m,
+                                                          int64_t rows, int cols, int rows_per_block,
+                                                          float* __restrict__ out) {
+  const int j = blockIdx.y * 256 + threadIdx.x;
+  if (j >= cols) return;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+  const int64_t r1 = min(rows, r0 + rows_per_block);
+  float s = 0.f;
+  for (int64_t r = r0; r < r1; ++r) s += from_bf16(m[r * cols + j]);
+  if (s != 0.f) atomicAdd(&out[j], s);
+}
+
+__global__ __launch_bounds__(256) void out_layer_wgrad_kernel(const uint16_t* __restrict__ h2,
+                                                              const float* __restrict__ dz,
+                                                              int64_t rows, int h, int dim,
+                                                              int rows_per_block,
+                                                              float* __restrict__ dw3,
+                                                              float* __restrict__ db3) {
+  const int j = blockIdx.y * 256 + threadIdx.x;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+  const int64_t r1 = min(rows, r0 + rows_per_block);
+  float acc[kMaxDim];
+  for (int c = 0; c < kMaxDim; ++c) acc[c] = 0.f;
+  if (j < h) {
+    for (int64_t r = r0; r < r1; ++r) {
+      const float hv = from_bf16(h2[r * h + j]);
+      for (int c = 0; c < dim; ++c) acc[c] += hv * dz[r * dim + c];
+    }
+    for (int c = 0; c < dim; ++c)
+      if (acc[c] != 0.f) atomicAdd(&dw3[j * dim + c], acc[c]);
+  }
+  if (blockIdx.y == 0 && threadIdx.x < dim) {
+    float s = 0.f;
+    for (int64_t r = r0; r < r1; ++r) s += dz[r * dim + threadIdx.x];
+    if (s != 0.f) atomicAdd(&db3[threadIdx.x], s);
+  }
+}
+
+// Adam on the FP32 masters (bias-corrected step), then the BF16 copies.
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, int64_t n, float lr_t,
+                            float b1, float b2, float eps, uint16_t* __restrict__ ph) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float pi = p[i] - lr_t * mi / (sqrtf(vi) + eps);
+  p[i] = pi;
+  ph[i] = to_bf16(pi);
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ p, int64_t n, uint16_t* __restrict__ ph) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) ph[i] = to_bf16(p[i]);
+}
+
+// He-uniform weights U(-sqrt(6 / fan_in), +) from a counter hash; zero
+// biases and zero rows of W1 past the real input width.
+__global__ void init_params_kernel(float* __restrict__ p, int64_t n, MlpShape sh, uint64_t seed) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v = 0.f;
+  float fan = 0.f;
+  if (i >= sh.off_w1 && i < sh.off_b1) {
+    const int64_t row = (i - sh.off_w1) / sh.h;
+    if (row < sh.d) fan = static_cast<float>(sh.d);
+  } else if (i >= sh.off_w2 && i < sh.off_b2) {
+    fan = static_cast<float>(sh.h);
+  } else if (i >= sh.off_w3 && i < sh.off_b3) {
+    fan = static_cast<float>(sh.h);
+  }
+  if (fan > 0.f) {
+    const float lim = sqrtf(6.f / fan);
+    v = static_cast<float>(2.0 * uniform01(seed ^ mix64(static_cast<uint64_t>(i) + 1)) - 1.0) * lim;
+  }
+  p[i] = v;
+}
+
+// Batch rows: edge e draws u ~ U(0, total) and takes the first edge whose
+// inclusive weight prefix exceeds u; its head and tail fill slots e and
+// B + e, and its negatives (uniform over the n rows) slots 2B + e n_neg + t.
+__global__ void sample_batch_kernel(const double* __restrict__ cdf, int64_t ne,
+                                    const int32_t* __restrict__ head,
+                                    const int32_t* __restrict__ tail, int n_edges, int n_neg,
+                                    int64_t n, uint64_t key, int32_t* __restrict__ rows) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_edges) return;
+  const double total = cdf[ne - 1];
+  const double u = uniform01(key ^ mix64(2 * static_cast<uint64_t>(e) + 1)) * total;
+  int64_t lo = 0, hi = ne - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (cdf[mid] > u)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  rows[e] = head[lo];
+  rows[n_edges + e] = tail[lo];
+  for (int t = 0; t < n_neg; ++t) {
+    const uint64_t h = mix64(key ^ mix64(2 * (static_cast<uint64_t>(e) * n_neg + t) + 2));
+    rows[2 * static_cast<int64_t>(n_edges) + static_cast<int64_t>(e) * n_neg + t] =
+        static_cast<int32_t>(((h >> 32) * static_cast<uint64_t>(n)) >> 32);
+  }
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Column mean and 1/std of the training rows (f64 sums); a zero-variance
+// column gets 1/std = 1.
+__global__ void col_moments_kernel(const float* __restrict__ x, int64_t n, int d,
+                                   double* __restrict__ sums) {
+  const int f = blockIdx.y * 256 + threadIdx.x;
+  if (f >= d) return;
+  double s = 0.0, s2 = 0.0;
+  for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
+    const double v = x[r * d + f];
+    s += v;
+    s2 += v * v;
+  }
+  atomicAdd(&sums[f], s);
+  atomicAdd(&sums[d + f], s2);
+}
+__global__ void col_finish_kernel(const double* __restrict__ sums, int64_t n, int d,
+                                  float* __restrict__ mean, float* __restrict__ inv_std) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= d) return;
+  const double mu = sums[f] / static_cast<double>(n);
+  const double var = std::max(sums[d + f] / static_cast<double>(n) - mu * mu, 0.0);
+  mean[f] = static_cast<float>(mu);
+  inv_std[f] = var > 1e-24 ? static_cast<float>(1.0 / std::sqrt(var)) : 1.f;
+}
+
+// Device buffers of one forward / backward pass over s_pad rows.
+struct Pass {
+  DevBuf<uint16_t> xb, h1, h2, dh2, dh1;
+  DevBuf<float> yt, z, dz;
+  DevBuf<double> loss;
+  void ensure(const MlpShape& sh, int64_t s_pad) {
+    xb.ensure(s_pad * sh.dp);
+    h1.ensure(s_pad * sh.h);
+    h2.ensure(s_pad * sh.h);
+    dh2.ensure(s_pad * sh.h);
+    dh1.ensure(s_pad * sh.h);
+    yt.ensure(s_pad * sh.dim);
+    z.ensure(s_pad * sh.dim);
+    dz.ensure(s_pad * sh.dim);
+    loss.ensure(1);
+  }
+};
+
+void forward(const MlpShape& sh, const float* params, const uint16_t* wh, Pass& ps, int64_t rows,
+             hipStream_t s) {
+  const int r = static_cast<int>(rows);
+  gemm<false, false, kEpiBiasRelu>(ps.xb.get(), sh.dp, wh + sh.off_w1, sh.h, r, sh.h, sh.dp, 1,
+                                   params + sh.off_b1, nullptr, ps.h1.get(), nullptr, sh.h, s);
+  gemm<false, false, kEpiBiasRelu>(ps.h1.get(), sh.h, wh + sh.off_w2, sh.h, r, sh.h, sh.h, 1,
+                                   params + sh.off_b2, nullptr, ps.h2.get(), nullptr, sh.h, s);
+  hipLaunchKernelGGL(out_layer_kernel, dim3(static_cast<unsigned>((rows + 3) / 4)), dim3(256), 0, s,
+                     ps.h2.get(), rows, sh.h, sh.dim, wh + sh.off_w3, params + sh.off_b3, ps.z.get());
+  KERNEL_CHECK();
+}
+
+// Gradients of the loss into grads (zeroed here), given dz.
+void backward(const MlpShape& sh, const uint16_t* wh, Pass& ps, int64_t rows, float* grads,
+              hipStream_t s) {
+  HIP_CHECK(hipMemsetAsync(grads, 0, sizeof(float) * sh.total, s));
+  const int r = static_cast<int>(rows);
+  constexpr int kRowsPerBlock = 256;
+  const unsigned rb = static_cast<unsigned>((rows + kRowsPerBlock - 1) / kRowsPerBlock);
+  const unsigned cb = static_cast<unsigned>((sh.h + 255) / 256);
+  hipLaunchKernelGGL(out_layer_wgrad_kernel, dim3(rb, cb), dim3(256), 0, s, ps.h2.get(), ps.dz.get(),
+                     rows, sh.h, sh.dim, kRowsPerBlock, grads + sh.off_w3, grads + sh.off_b3);
+  hipLaunchKernelGGL(out_layer_back_kernel, dim3(blocks_for(rows * sh.h, 256)), dim3(256), 0, s,
+                     ps.dz.get(), ps.h2.get(), rows, sh.h, sh.dim, wh + sh.off_w3, ps.dh2.get());
+  hipLaunchKernelGGL(colsum_bf16_kernel, dim3(rb, cb), dim3(256), 0, s, ps.dh2.get(), rows, sh.h,
+                     kRowsPerBlock, grads + sh.off_b2);
+  KERNEL_CHECK();
+  const int splits = std::max(1, r / 1024);
+  // dW2 = H1^T dH2
+  gemm<true, false, kEpiAtomic>(ps.h1.get(), sh.h, ps.dh2.get(), sh.h, sh.h, sh.h, r, splits,
+                                nullptr, nullptr, nullptr, grads + sh.off_w2, sh.h, s);
+  // dH1 = (dH2 W2^T) masked by H1 > 0
+  gemm<false, true, kEpiMask>(ps.dh2.get(), sh.h, wh + sh.off_w2, sh.h, r, sh.h, sh.h, 1, nullptr,
+                              ps.h1.get(), ps.dh1.get(), nullptr, sh.h, s);
+  hipLaunchKernelGGL(colsum_bf16_kernel, dim3(rb, cb), dim3(256), 0, s, ps.dh1.get(), rows, sh.h,
+                     kRowsPerBlock, grads + sh.off_b1);
+  KERNEL_CHECK();
+  // dW1 = X^T dH1
+  gemm<true, false, kEpiAtomic>(ps.xb.get(), sh.dp, ps.dh1.get(), sh.h, sh.dp, sh.h, r, splits,
+                                nullptr, nullptr, nullptr, grads + sh.off_w1, sh.h, s);
+}
+
+int64_t pad64(int64_t v) { return (v + 63) / 64 * 64; }
+
+void check_shape(int d, int h, int dim) {
+  if (dim < 1 || dim > kMaxDim) throw_invalid("the learned transform supports n_components <= 8");
+  if (h < 64 || h % 64 != 0) throw_invalid("mlp_hidden must be a positive multiple of 64");
+  if (d < 1) throw_invalid("input rows must have at least one feature");
+}
+
+}  // namespace
+
+void mlp_train(MlpModel& m, const float* x, int64_t n, int64_t d, const float* y, int dim,
+               const Graph& g, float a, float b, const umapx_params& p, hipStream_t s) {
+  maybe_inject_fault("mlp_train");
+  check_shape(static_cast<int>(d), p.mlp_hidden, dim);
+  if (p.mlp_steps < 1 || p.mlp_batch < 1 || p.mlp_negatives < 0)
+    throw_invalid("mlp_steps and mlp_batch must be >= 1 and mlp_negatives >= 0");
+  if (!(p.mlp_learning_rate > 0.f) || !(p.mlp_embedding_weight >= 0.f))
+    throw_invalid("mlp_learning_rate must be > 0 and mlp_embedding_weight >= 0");
+  if (g.n_edges < 1) throw_invalid("the fitted graph has no edges to train the learned transform on");
+  const MlpShape sh = MlpShape::make(static_cast<int>(d), p.mlp_hidden, dim);
+  m.shape = sh;
+  m.params.ensure(sh.total);
+  m.grads.ensure(sh.total);
+  m.adam_m.ensure(sh.total);
+  m.adam_v.ensure(sh.total);
+  m.wh.ensure(sh.total);
+  m.mean.ensure(d);
+  m.inv_std.ensure(d);
+  const uint64_t seed = p.random_seed ^ 0x6D6C7045ACE0F00DULL;
+  hipLaunchKernelGGL(init_params_kernel, dim3(blocks_for(sh.total, 256)), dim3(256), 0, s,
+                     m.params.get(), sh.total, sh, seed);
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(sh.total, 256)), dim3(256), 0, s,
+                     m.params.get(), sh.total, m.wh.get());
+  HIP_CHECK(hipMemsetAsync(m.adam_m.get(), 0, sizeof(float) * sh.total, s));
+  HIP_CHECK(hipMemsetAsync(m.adam_v.get(), 0, sizeof(float) * sh.total, s));
+  {
+    DevBuf<double> sums(2 * d);
+    HIP_CHECK(hipMemsetAsync(sums.get(), 0, sizeof(double) * 2 * d, s));
+    hipLaunchKernelGGL(col_moments_kernel,
+                       dim3(static_cast<unsigned>(std::min<int64_t>(n, 512)),
+                            static_cast<unsigned>((d + 255) / 256)),
+                       dim3(256), 0, s, x, n, static_cast<int>(d), sums.get());
+    hipLaunchKernelGGL(col_finish_kernel, dim3(blocks_for(d, 256)), dim3(256), 0, s, sums.get(), n,
+                       static_cast<int>(d), m.mean.get(), m.inv_std.get());
+    KERNEL_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  // inclusive prefix sums of the edge weights, in f64 on the host (once)
+  const int64_t ne = g.n_edges;
+  DevBuf<double> cdf(ne);
+  {
+    std::vector<float> w(ne);
+    HIP_CHECK(hipMemcpyAsync(w.data(), g.weight.get(), 4 * ne, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<double> c(ne);
+    double run = 0.0;
+    for (int64_t i = 0; i < ne; ++i) {
+      run += static_cast<double>(w[i]);
+      c[i] = run;
+    }
+    HIP_CHECK(hipMemcpyAsync(cdf.get(), c.data(), 8 * ne, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  const int B = p.mlp_batch, nn = p.mlp_negatives;
+  const int64_t s_real = static_cast<int64_t>(B) * (2 + nn);
+  const int64_t s_pad = pad64(s_real);
+  Pass ps;
+  ps.ensure(sh, s_pad);
+  DevBuf<int32_t> rows(s_pad);
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(s_pad, 256)), dim3(256), 0, s, rows.get(),
+                     s_pad, -1);
+  HIP_CHECK(hipMemsetAsync(ps.dz.get(), 0, sizeof(float) * s_pad * dim, s));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  std::vector<double> loss_log;
+  for (int step = 0; step < p.mlp_steps; ++step) {
+    const uint64_t key = mix64_host(seed + 0x9E3779B97F4A7C15ULL * (static_cast<uint64_t>(step) + 1));
+    hipLaunchKernelGGL(sample_batch_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, s, cdf.get(),
+                       ne, g.head.get(), g.tail.get(), B, nn, n, key, rows.get());
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_for(s_pad * sh.dp, 256)), dim3(256), 0, s, x,
+                       static_cast<int>(d), sh.dp, rows.get(), int64_t{0}, s_pad, m.mean.get(),
+                       m.inv_std.get(), ps.xb.get(), y, dim, ps.yt.get());
+    KERNEL_CHECK();
+    forward(sh, m.params.get(), m.wh.get(), ps, s_pad, s);
+    const bool log_loss = step == 0 || step + 1 == p.mlp_steps;
+    if (log_loss) HIP_CHECK(hipMemsetAsync(ps.loss.get(), 0, sizeof(double), s));
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, s, ps.z.get(),
+                       ps.yt.get(), B, nn, dim, static_cast<int>(s_real), a, b,
+                       p.mlp_embedding_weight, ps.dz.get(), log_loss ? ps.loss.get() : nullptr);
+    KERNEL_CHECK();
+    backward(sh, m.wh.get(), ps, s_pad, m.grads.get(), s);
+    const double t1 = static_cast<double>(step) + 1.0;
+    const float lr_t = static_cast<float>(p.mlp_learning_rate * std::sqrt(1.0 - std::pow(b2, t1)) /
+                                          (1.0 - std::pow(b1, t1)));
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(sh.total, 256)), dim3(256), 0, s,
+                       m.params.get(), m.grads.get(), m.adam_m.get(), m.adam_v.get(), sh.total,
+                       lr_t, b1, b2, eps, m.wh.get());
+    KERNEL_CHECK();
+    if (log_loss) {
+      double l = 0.0;
+      HIP_CHECK(hipMemcpyAsync(&l, ps.loss.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      if (step == 0) m.first_loss = l;
+      m.last_loss = l;
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+  m.steps_done = p.mlp_steps;
+  m.trained = true;
+}
+
+void mlp_forward_rows(const MlpModel& m, const float* x, int64_t rows, float* out, hipStream_t s) {
+  if (!m.trained) throw Error(kNotFitted, "the learned transform has not been trained");
+  const MlpShape& sh = m.shape;
+  constexpr int64_t kChunk = 65536;
+  Pass ps;
+  ps.ensure(sh, std::min<int64_t>(pad64(rows), kChunk));
+  for (int64_t r0 = 0; r0 < rows; r0 += kChunk) {
+    const int64_t cnt = std::min(kChunk, rows - r0);
+    const int64_t cpad = pad64(cnt);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_for(cpad * sh.dp, 256)), dim3(256), 0, s, x,
+                       sh.d, sh.dp, static_cast<const int32_t*>(nullptr), r0, cnt, m.mean.get(),
+                       m.inv_std.get(), ps.xb.get(), static_cast<const float*>(nullptr), sh.dim,
+                       static_cast<float*>(nullptr));
+    KERNEL_CHECK();
+    if (cpad > cnt)
+      HIP_CHECK(hipMemsetAsync(ps.xb.get() + cnt * sh.dp, 0, sizeof(uint16_t) * (cpad - cnt) * sh.dp, s));
+    forward(sh, m.params.get(), m.wh.get(), ps, cpad, s);
+    HIP_CHECK(hipMemcpyAsync(out + r0 * sh.dim, ps.z.get(), sizeof(float) * cnt * sh.dim,
+                             hipMemcpyDeviceToDevice, s));
+  }
+}
+
+void mlp_batch_grad(int d, int h, int dim, const float* params, const float* mean,
+                    const float* inv_std, const float* x_rows, const float* y_rows, int n_edges,
+                    int n_neg, float a, float b, float mse_weight, float* grads, float* loss,
+                    float* z_out) {
+  check_shape(d, h, dim);
+  if (n_edges < 1 || n_neg < 0) throw_invalid("n_edges must be >= 1 and n_neg >= 0");
+  const MlpShape sh = MlpShape::make(d, h, dim);
+  const int64_t s_real = static_cast<int64_t>(n_edges) * (2 + n_neg);
+  const int64_t s_pad = pad64(s_real);
+  hipStream_t s = nullptr;
+  HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{s};
+  DevBuf<float> dp(sh.total), dg(sh.total), dmean(d), dinv(d), dx(s_real * d), dy(s_real * dim);
+  DevBuf<uint16_t> dwh(sh.total);
+  HIP_CHECK(hipMemcpyAsync(dp.get(), params, 4 * sh.total, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dmean.get(), mean, 4 * d, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dinv.get(), inv_std, 4 * d, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dx.get(), x_rows, 4 * s_real * d, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dy.get(), y_rows, 4 * s_real * dim, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(sh.total, 256)), dim3(256), 0, s, dp.get(),
+                     sh.total, dwh.get());
+  Pass ps;
+  ps.ensure(sh, s_pad);
+  DevBuf<int32_t> rows(s_pad);
+  {
+    std::vector<int32_t> r(s_pad, -1);
+    for (int64_t i = 0; i < s_real; ++i) r[i] = static_cast<int32_t>(i);
+    HIP_CHECK(hipMemcpyAsync(rows.get(), r.data(), 4 * s_pad, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_for(s_pad * sh.dp, 256)), dim3(256), 0, s,
+                       dx.get(), d, sh.dp, rows.get(), int64_t{0}, s_pad, dmean.get(), dinv.get(),
+                       ps.xb.get(), dy.get(), dim, ps.yt.get());
+    KERNEL_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  HIP_CHECK(hipMemsetAsync(ps.dz.get(), 0, sizeof(float) * s_pad * dim, s));
+  HIP_CHECK(hipMemsetAsync(ps.loss.get(), 0, sizeof(double), s));
+  forward(sh, dp.get(), dwh.get(), ps, s_pad, s);
+  hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(n_edges, 256)), dim3(256), 0, s, ps.z.get(),
+                     ps.yt.get(), n_edges, n_neg, dim, static_cast<int>(s_real), a, b, mse_weight,
+                     ps.dz.get(), ps.loss.get());
+  KERNEL_CHECK();
+  backward(sh, dwh.get(), ps, s_pad, dg.get(), s);
+  double l = 0.0;
+  HIP_CHECK(hipMemcpyAsync(grads, dg.get(), 4 * sh.total, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&l, ps.loss.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  if (z_out) HIP_CHECK(hipMemcpyAsync(z_out, ps.z.get(), 4 * s_real * dim, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  *loss = static_cast<float>(l);
 }
 
 }  // namespace umapx

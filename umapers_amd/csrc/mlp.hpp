@@ -1,7 +1,13 @@
 // Learned transform: an MLP encoder d -> H -> H -> n_components (ReLU), BF16
 // weights and activations with FP32 accumulation and FP32 master weights,
-// trained on the UMAP cross-entropy over the fitted fuzzy graph with HIP
-// forward / backward kernels.
+// trained on the UMAP cross-entropy over the fitted fuzzy graph plus a
+// squared error to the fitted embedding, with HIP forward / backward kernels
+// (BF16 MFMA GEMMs for the two wide layers) and Adam.
+//
+// The reference's analog is ParametricUmapModel (rust_umap/src/parametric.rs:
+// a d -> H (tanh) -> out network regressed on a teacher UMAP embedding,
+// :89-534); this encoder keeps the teacher target as the squared-error term
+// and adds the UMAP cross-entropy over the teacher's fuzzy graph.
 #pragma once
 
 #include "../../include/umapx.h"
@@ -10,21 +16,35 @@
 
 namespace umapx {
 
+// Flat parameter layout (f32): W1 [dp x h], b1 [h], W2 [h x h], b2 [h],
+// W3 [h x dim], b3 [dim]; dp = d rounded up to a multiple of 32 (rows of W1
+// past d stay zero).
+struct MlpShape {
+  int d = 0, dp = 0, h = 0, dim = 0;
+  int64_t off_w1 = 0, off_b1 = 0, off_w2 = 0, off_b2 = 0, off_w3 = 0, off_b3 = 0, total = 0;
+  static MlpShape make(int d, int h, int dim);
+};
+
 struct MlpModel {
   bool trained = false;
-  int d = 0, h = 0, out = 0;
-  // FP32 master weights, row-major [in x out]
-  DevBuf<float> w1, b1, w2, b2, w3, b3;
-  // BF16 copies used by the forward / backward GEMMs
-  DevBuf<uint16_t> w1h, w2h, w3h;
-  // input standardisation
-  DevBuf<float> mean, inv_std;
-  double last_loss = 0.0;
+  MlpShape shape;
+  DevBuf<float> params, grads, adam_m, adam_v;
+  DevBuf<uint16_t> wh;  // BF16 copies of W1, W2, W3 (same offsets as params)
+  DevBuf<float> mean, inv_std;  // input standardisation over the training rows
+  double last_loss = 0.0, first_loss = 0.0;
   int steps_done = 0;
 };
 
 void mlp_train(MlpModel& m, const float* x, int64_t n, int64_t d, const float* y, int dim,
                const Graph& g, float a, float b, const umapx_params& p, hipStream_t s);
 void mlp_forward_rows(const MlpModel& m, const float* x, int64_t rows, float* out, hipStream_t s);
+
+// One forward / backward pass on a given batch (host arrays), for gradient
+// checks: rows are laid out as B heads, B tails, then B * n_neg negatives
+// (negative t of edge e at 2B + e * n_neg + t).
+void mlp_batch_grad(int d, int h, int dim, const float* params, const float* mean,
+                    const float* inv_std, const float* x_rows, const float* y_rows, int n_edges,
+                    int n_neg, float a, float b, float mse_weight, float* grads, float* loss,
+                    float* z_out);
 
 }  // namespace umapx
