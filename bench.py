@@ -15,10 +15,14 @@ in HBM, through the C ABI (umapx_fit_transform_device).
                    on 1000 sampled rows (self included)
   transform_rows_per_sec   plain transform (kNN into the 200k training rows +
                    per-query SGD) of 20,000 fresh rows, device resident
+  learned_transform  the 100,000 x 192 learned-transform configuration: fit +
+                   encoder training, encoder rows/s over 1M fresh rows, and the
+                   plain transform on 10,000 of them as judge (per-row
+                   distance, trustworthiness of both)
   roofline         the dominant kernel of the fit (SGD epochs: algorithmic
                    bytes 4E + 56 A + 8 N per epoch against 8 TB/s HBM), with
                    the kNN distance-tile kernel (2 n^2 d FLOP against the BF16
-                   dense MFMA peak) reported as roofline_knn
+                   dense MFMA peak) reported as roofline_other
   cpu_baseline     the C++ restatement of the reference's CPU path on a
                    sub-sample of the same rows, scaled to 200k rows
 
@@ -78,6 +82,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=50000,
                     help="rows of the CPU-baseline sample (0 skips that leg)")
     ap.add_argument("--transform-rows", type=int, default=20000)
+    ap.add_argument("--mlp-queries", type=int, default=1_000_000,
+                    help="rows sent through the learned encoder (0 skips the learned-transform leg)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -166,7 +172,8 @@ def main() -> int:
     knn_flops = 2.0 * n * n * d
     sgd_gbs = sgd_bytes / (sgd_ms * 1e-3) / 1e9
     knn_tfs = knn_flops / (knn_ms * 1e-3) / 1e12
-    roof_sgd = {"kernel": "sgd_epoch2_kernel", "bound": "hbm", "achieved": round(sgd_gbs, 1),
+    roof_sgd = {"kernel": "sgd epoch (sgd_epoch2_kernel<1> + apply_deferred_kernel)",
+                "bound": "hbm", "achieved": round(sgd_gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sgd_gbs / HBM_PEAK_GBS, 4),
                 "traffic": None, "launches": epochs,
                 "avg_launch_us": round(sgd_ms * 1e3 / epochs, 2),
@@ -211,6 +218,62 @@ def main() -> int:
         tt.append(time.perf_counter() - t0)
     xform_rps = m / float(np.median(tt))
     log(f"bench: plain transform {m} rows in {np.median(tt) * 1e3:.1f} ms")
+
+    # ---- learned transform: 100000 x 192 fit, encoder over fresh queries ----
+    mlp = None
+    if args.mlp_queries > 0:
+        m_q = args.mlp_queries
+        log(f"bench: learned transform leg (100000 x 192 fit, {m_q} queries)")
+        x5, q5 = datasets.c5(n_fit=100_000, n_query=m_q)
+        p5 = L.default_params()
+        p5.n_neighbors = k
+        p5.transform_mode = L.TRANSFORM_LEARNED
+        p5.random_seed = 42
+        h5 = ctypes.c_void_p()
+        L.check(lib.umapx_create(ctypes.byref(p5), ctypes.byref(h5)))
+        t0 = time.perf_counter()
+        y5 = np.empty((x5.shape[0], 2), np.float32)
+        L.check(lib.umapx_fit_transform(h5, L.ptr(x5), x5.shape[0], x5.shape[1], L.ptr(y5)))
+        fit5 = time.perf_counter() - t0
+        first, last, steps = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+        L.check(lib.umapx_get_mlp_loss(h5, ctypes.byref(first), ctypes.byref(last), ctypes.byref(steps)))
+        q5_d = torch.from_numpy(q5).to(dev)
+        yq5_d = torch.empty((m_q, 2), dtype=torch.float32, device=dev)
+        args_l = (h5, ctypes.c_void_p(q5_d.data_ptr()), m_q, q5.shape[1], ctypes.c_void_p(yq5_d.data_ptr()))
+        L.check(lib.umapx_transform_learned_device(*args_l))
+        torch.cuda.synchronize()
+        tl = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            L.check(lib.umapx_transform_learned_device(*args_l))
+            torch.cuda.synchronize()
+            tl.append(time.perf_counter() - t0)
+        mlp_sec = float(np.median(tl))
+        judge = np.ascontiguousarray(q5[:10000])
+        y_mlp = yq5_d[:10000].cpu().numpy()
+        y_plain = np.empty((judge.shape[0], 2), np.float32)
+        t0 = time.perf_counter()
+        L.check(lib.umapx_transform_plain(h5, L.ptr(judge), judge.shape[0], judge.shape[1], L.ptr(y_plain)))
+        plain_sec = time.perf_counter() - t0
+        per_row = np.linalg.norm(y_mlp - y_plain, axis=1)
+        extent = float(np.ptp(y5, axis=0).max())
+        flops_row = 2.0 * (192 * 256 + 256 * 256 + 256 * 2)
+        mlp = {"workload": "c5: 100000x192 fit (20 clusters, seed 31), encoder 192-256-256-2 "
+                           f"trained {steps.value} steps x 4096 edges x 5 negatives; {m_q} fresh "
+                           "queries (seed 32), device resident; plain judge on the first 10000",
+               "fit_sec_incl_training": round(fit5, 3),
+               "train_loss_first_last": [round(first.value, 4), round(last.value, 4)],
+               "learned_rows_per_sec": round(m_q / mlp_sec, 1),
+               "learned_tflops": round(flops_row * m_q / mlp_sec / 1e12, 2),
+               "plain_judge_rows_per_sec": round(judge.shape[0] / plain_sec, 1),
+               "per_row_dist_median": round(float(np.median(per_row)), 4),
+               "per_row_dist_p90": round(float(np.percentile(per_row, 90)), 4),
+               "embedding_extent": round(extent, 3),
+               "trust15_learned": round(trustworthiness(judge, y_mlp, k), 5),
+               "trust15_plain": round(trustworthiness(judge, y_plain, k), 5)}
+        log(f"bench: learned transform {m_q / mlp_sec:.0f} rows/s vs plain {judge.shape[0] / plain_sec:.0f} rows/s")
+        lib.umapx_destroy(h5)
+        del q5_d, yq5_d
 
     # ---- CPU baseline: the C++ restatement on a sub-sample ------------------
     cpu = None
@@ -260,6 +323,7 @@ def main() -> int:
         "trustworthiness_15": round(trust, 5),
         "knn_preservation_15": round(kpres, 5),
         "transform_rows_per_sec": round(xform_rps, 1),
+        "learned_transform": mlp,
         "stage_sec": {key: round(prof[key], 4) for key in L.TIMING_KEYS},
         "fp32_knn_kernel_ms": round(float(np.median([t["knn_kernel_ms"] for t in tm32])), 3),
         "roofline": roof,
