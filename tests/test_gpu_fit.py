@@ -10,10 +10,12 @@ pytestmark = pytest.mark.gpu
 
 def test_c1_fit_parity_with_oracle(gpu, oracle):
     # i.i.d. Gaussian rows have no structure, so single-seed trustworthiness
-    # moves by ~0.005; the parity bar (0.01) is applied to 3-seed means
+    # moves by ~0.006 (the init planes agree to cosine 1.0; the SGD walk
+    # amplifies last-bit differences); the parity bar (0.01) is applied to
+    # 10-seed means, whose difference has a spread of about 0.002
     x = datasets.c1()
     tg, to, kg, ko = [], [], [], []
-    for seed in (42, 7, 11):
+    for seed in range(10):
         y = Umap(n_epochs=200, ann_mode="exact", random_seed=seed).fit_transform(x)
         yo = oracle.fit(x, oracle.OracleParams(n_epochs=200, random_seed=seed)).embedding
         assert y.shape == (1000, 2) and y.dtype == np.float32 and np.isfinite(y).all()

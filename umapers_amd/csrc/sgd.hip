@@ -3,15 +3,26 @@
 // The reference runs one Gauss-Seidel pass over the head-sorted edges per
 // epoch on a single thread (optimize_layout_training_flat,
 // rust_umap/src/lib.rs:5137-5318; densMAP loop :5324-5441).  Here each epoch is
-// one edge-parallel launch (Hogwild): a thread per edge checks the edge's f32
-// schedule, applies the attraction to its head (summed per head across the
-// wave) and to its tail (float atomics, or for the fit's symmetric graph a
-// move deferred to the end of the epoch), then draws its negatives from a
-// counter-based hash of (seed, epoch, edge, draw) and accumulates the
-// repulsion on a register copy of the head.  The per-edge
+// one edge-parallel launch (Hogwild): the edges' f32 schedules are checked and
+// the firing edges compacted per wave; a thread per firing edge applies the
+// attraction to its head (summed per head across the wave) and to its tail
+// (float atomics, or for the fit's symmetric graph by the reverse edge's own
+// thread), then draws its negatives from a counter-based hash of (seed,
+// epoch, edge, draw) and accumulates the repulsion on a register copy of the
+// head.  The per-edge
 // schedule (epoch_of_next_sample, epoch_of_next_negative_sample) uses the
 // reference's exact f32 operations, so every edge fires and draws negatives
 // the same number of times as in the reference.
+//
+// Moves that reach one vertex within a launch are summed from the positions
+// the launch started from, while the reference applies them one after
+// another, each from the position the previous one left.  The two behave
+// alike while a vertex has few edges, but a hub (a row hundreds of entries
+// long, common in high-dimensional kNN graphs) sums hundreds of pulls
+// toward the same neighbourhood, overshoots it by that factor and is thrown
+// out of its cluster.  Every move onto a vertex with more than kHubEnds edge
+// ends is therefore scaled by kHubEnds / (its edge ends), so its summed step
+// is that of a vertex with kHubEnds ends; other vertices move unscaled.
 #include "sgd.hpp"
 
 #include <algorithm>
@@ -70,7 +81,8 @@ __global__ __launch_bounds__(256) void sgd_epoch_kernel(
     const float* __restrict__ eps, float* __restrict__ eons, float* __restrict__ eonns,
     int64_t ne, uint32_t n, int dim_rt, float ef, float alpha, float a, float b, float rep,
     float nsr, uint64_t key_epoch, const float* __restrict__ radii, int dens_on, float dens_lambda,
-    float var_shift, int32_t* __restrict__ pos_counts, int32_t* __restrict__ neg_counts) {
+    float var_shift, const float* __restrict__ mscale, int32_t* __restrict__ pos_counts,
+    int32_t* __restrict__ neg_counts) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   if (!EdgeSchedule::fire(eons, eps, e, ef)) return;
@@ -85,6 +97,7 @@ __global__ __launch_bounds__(256) void sgd_epoch_kernel(
   }
   float dh[MAXD];
   for (int c = 0; c < dim; ++c) dh[c] = 0.f;
+  const float st = mscale ? mscale[t] : 1.f;
   if (h != t) {
     float d2 = 0.f;
     for (int c = 0; c < dim; ++c) {
@@ -99,7 +112,7 @@ __global__ __launch_bounds__(256) void sgd_epoch_kernel(
         yh[c] += g;
         yt[c] -= g;
         dh[c] += g;
-        atomicAdd(&y[static_cast<int64_t>(t) * dim + c], -g);
+        atomicAdd(&y[static_cast<int64_t>(t) * dim + c], -g * st);
       }
     }
     if (dens_on) {
@@ -117,7 +130,7 @@ __global__ __launch_bounds__(256) void sgd_epoch_kernel(
         const float g = clip4(coeff * dir) * alpha;
         yh[c] -= g;
         dh[c] -= g;
-        atomicAdd(&y[static_cast<int64_t>(t) * dim + c], g);
+        atomicAdd(&y[static_cast<int64_t>(t) * dim + c], g * st);
       }
     }
   }
@@ -148,156 +161,182 @@ __global__ __launch_bounds__(256) void sgd_epoch_kernel(
       }
     }
   }
+  const float sh = mscale ? mscale[h] : 1.f;
   for (int c = 0; c < dim; ++c)
-    if (dh[c] != 0.f) atomicAdd(&y[static_cast<int64_t>(h) * dim + c], dh[c]);
+    if (dh[c] != 0.f) atomicAdd(&y[static_cast<int64_t>(h) * dim + c], dh[c] * sh);
 }
 
-// 2-D specialisation (lib.rs:5172-5245).  All lanes stay alive to the end so
-// that the head deltas of a wave can be summed per head (edges are sorted by
-// head, so equal heads are adjacent lanes) before a single atomic per head
-// segment; the negatives of an edge are gathered with independent loads.
-// Tail moves: kTailAtomic adds them with float atomics at once; kTailDeferred
-// (symmetric graphs) stores each edge's tail move in the slot of its reverse
-// edge rev[e], which lies in the tail's own row, and apply_deferred_kernel
-// adds every row's slots to its vertex at the end of the epoch.  Random
-// float atomics on the tails dominate the epoch otherwise; the deferred form
-// replaces them with one plain store per firing edge and one atomic per
-// (wave, row) segment.  A launch covers edges [e_begin, e_end).
-constexpr int kTailAtomic = 0;
-constexpr int kTailDeferred = 1;
-constexpr int64_t kDeferredMinEdges = int64_t{1} << 20;
+// 2-D specialisation (lib.rs:5172-5245) with the firing edges compacted.  A
+// wave owns kWaveEdges consecutive edges: it first runs their schedules (one
+// edge per lane per pass) and lists the firing ones in LDS in edge order,
+// then works through the list 64 at a time, so that the heavy part (the
+// attraction and the negatives) runs on full waves although only a fraction
+// of the edges fire in an epoch.  Heads are sorted, so equal heads sit in
+// adjacent lanes in every round: head deltas are summed over equal-head lane
+// runs and added with one atomic per run.
+//
+// Tail moves: with MIRROR (the fit's symmetric union, where the reverse edge
+// (t, h) exists and fires in the same epochs, having the same weight and
+// hence the same schedule) every edge replays the pair's two moves, row
+// min(h, t) first and row max(h, t) second as in the reference's row order,
+// from its own reads of both rows, and keeps only its head's share: the
+// reverse edge's thread keeps the other.  No thread then writes a row it does
+// not own.  Otherwise the tail move is added with float atomics.  A launch
+// covers edges [e_begin, e_end).
+constexpr int64_t kMirrorMinEdges = int64_t{1} << 20;
 constexpr int kRangeLaunches = 8;
 constexpr int kChunkLen = 32;  // longest run of one head's edges walked by one thread
+constexpr int kPerLane = 2;
+constexpr int kWaveEdges = 64 * kPerLane;
+constexpr int kEpochWaves = 4;  // waves per block of the epoch kernel
 
-template <int TAIL>
-__global__ __launch_bounds__(256) void sgd_epoch2_kernel(
+template <bool MIRROR>
+__global__ __launch_bounds__(64 * kEpochWaves) void sgd_epoch2_kernel(
     float2* __restrict__ y, const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
-    const float* __restrict__ eps, float* __restrict__ eons, float* __restrict__ eonns, int64_t ne,
+    const float* __restrict__ eps, float* __restrict__ eons, float* __restrict__ eonns,
     uint32_t n, float ef, float alpha, float a, float b, float rep, float nsr, uint64_t key_epoch,
     const float* __restrict__ radii, int dens_on, float dens_lambda, float var_shift,
-    const int32_t* __restrict__ rev, float2* __restrict__ tdelta,
-    int32_t* __restrict__ pos_counts, int32_t* __restrict__ neg_counts, int64_t e_begin,
-    int64_t e_end) {
-  const int64_t e = e_begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const float* __restrict__ mscale, int32_t* __restrict__ pos_counts,
+    int32_t* __restrict__ neg_counts, int64_t e_begin, int64_t e_end) {
+  __shared__ uint16_t list[kEpochWaves][kWaveEdges];
   const int lane = threadIdx.x & 63;
-  bool active = false;
-  int32_t h = -1;
-  float dhx = 0.f, dhy = 0.f;
-  if (e < e_end) {
-    h = head[e];
-    active = EdgeSchedule::fire(eons, eps, e, ef);
+  const int wv = threadIdx.x >> 6;
+  const int64_t wbase =
+      e_begin + (static_cast<int64_t>(blockIdx.x) * kEpochWaves + wv) * kWaveEdges;
+  const uint64_t below = (uint64_t{1} << lane) - 1;
+  int cnt = 0;
+#pragma unroll
+  for (int r = 0; r < kPerLane; ++r) {
+    const int64_t e = wbase + r * 64 + lane;
+    const bool fire = e < e_end && EdgeSchedule::fire(eons, eps, e, ef);
+    const uint64_t mask = __ballot(fire);
+    if (fire) list[wv][cnt + __popcll(mask & below)] = static_cast<uint16_t>(r * 64 + lane);
+    cnt += __popcll(mask);
   }
-  if (active) {
-    const int32_t t = tail[e];
-    const int32_t slot = TAIL == kTailDeferred ? rev[e] : -1;
-    float2 yh = y[h];
-    float2 yt = y[t];
-    // attraction (lib.rs:5196-5212) and density term (lib.rs:5110-5134) of
-    // edge (ia -> ib) on register copies, accumulating both endpoint moves
-    auto pull = [&](float2& ya, float2& yb, int32_t ia, int32_t ib, float2& da, float2& db) {
-      const float dx = ya.x - yb.x, dy = ya.y - yb.y;
-      const float d2 = dx * dx + dy * dy;
-      if (d2 > 0.f) {
-        const float pb = pow_pos(d2, b);
-        const float gc = (-2.f * a * b) * pb * rcp(d2 * (a * pb + 1.f));
-        const float gx = clip4(gc * dx) * alpha, gy = clip4(gc * dy) * alpha;
-        ya.x += gx;
-        ya.y += gy;
-        yb.x -= gx;
-        yb.y -= gy;
-        da.x += gx;
-        da.y += gy;
-        db.x -= gx;
-        db.y -= gy;
-      }
-      if (dens_on) {
-        const float sx = ya.x - yb.x, sy = ya.y - yb.y;
-        const float cur = sqrtf(sx * sx + sy * sy);
-        const float target = fmaxf(expf((radii[ia] + radii[ib]) * 0.5f), 1e-6f);
-        const float coeff = 0.1f * dens_lambda * (cur - target) / (cur + var_shift);
-        const float inv = 1.f / (cur + 1e-6f);
-        const float gx = clip4(coeff * sx * inv) * alpha, gy = clip4(coeff * sy * inv) * alpha;
-        ya.x -= gx;
-        ya.y -= gy;
-        yb.x += gx;
-        yb.y += gy;
-        da.x -= gx;
-        da.y -= gy;
-        db.x += gx;
-        db.y += gy;
-      }
-    };
-    if (h != t) {
-      float2 dh = make_float2(0.f, 0.f), dt = make_float2(0.f, 0.f);
-      if (slot < 0) {
-        pull(yh, yt, h, t, dh, dt);
-        if (dt.x != 0.f || dt.y != 0.f) {
-          atomicAdd(&y[t].x, dt.x);
-          atomicAdd(&y[t].y, dt.y);
-        }
-      } else if (h < t) {
-        // The reverse edge (t, h) fires in the same epochs (equal weight,
-        // equal schedule).  The reference moves the pair twice per firing,
-        // the second time from the positions the first move left (row h
-        // before row t); with the tail move deferred, that order is replayed
-        // here for both edges and the edge (t, h) itself skips attraction.
-        pull(yh, yt, h, t, dh, dt);
-        pull(yt, yh, t, h, dt, dh);
-        if (dt.x != 0.f || dt.y != 0.f) tdelta[slot] = dt;
-      }
-      dhx += dh.x;
-      dhy += dh.y;
+  // the list is written and read by lanes of one wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  // attraction (lib.rs:5196-5212) and density term (lib.rs:5110-5134) of
+  // edge (ia -> ib) on register copies, accumulating both endpoint moves
+  auto pull = [&](float2& ya, float2& yb, int32_t ia, int32_t ib, float2& da, float2& db) {
+    const float dx = ya.x - yb.x, dy = ya.y - yb.y;
+    const float d2 = dx * dx + dy * dy;
+    if (d2 > 0.f) {
+      const float pb = pow_pos(d2, b);
+      const float gc = (-2.f * a * b) * pb * rcp(d2 * (a * pb + 1.f));
+      const float gx = clip4(gc * dx) * alpha, gy = clip4(gc * dy) * alpha;
+      ya.x += gx;
+      ya.y += gy;
+      yb.x -= gx;
+      yb.y -= gy;
+      da.x += gx;
+      da.y += gy;
+      db.x -= gx;
+      db.y -= gy;
     }
-    if (pos_counts) pos_counts[e] += 1;
-    const int nneg = EdgeSchedule::negatives(eonns, eps, e, ef, nsr);
-    if (nneg > 0) {
-      if (neg_counts) neg_counts[e] += nneg;
-      const uint64_t key = mix64(key_epoch ^ static_cast<uint64_t>(e));
-      constexpr int kChunk = 8;
-      for (int s0 = 0; s0 < nneg; s0 += kChunk) {
-        float2 yk[kChunk];
-        int32_t kk[kChunk];
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          kk[j] = (s0 + j < nneg) ? static_cast<int32_t>(draw_index(key, static_cast<uint32_t>(s0 + j), n)) : h;
-          yk[j] = y[kk[j]];
+    if (dens_on) {
+      const float sx = ya.x - yb.x, sy = ya.y - yb.y;
+      const float cur = sqrtf(sx * sx + sy * sy);
+      const float target = fmaxf(expf((radii[ia] + radii[ib]) * 0.5f), 1e-6f);
+      const float coeff = 0.1f * dens_lambda * (cur - target) / (cur + var_shift);
+      const float inv = 1.f / (cur + 1e-6f);
+      const float gx = clip4(coeff * sx * inv) * alpha, gy = clip4(coeff * sy * inv) * alpha;
+      ya.x -= gx;
+      ya.y -= gy;
+      yb.x += gx;
+      yb.y += gy;
+      da.x -= gx;
+      da.y -= gy;
+      db.x += gx;
+      db.y += gy;
+    }
+  };
+
+  for (int base = 0; base < cnt; base += 64) {
+    const int j = base + lane;
+    int32_t h = -1;
+    float dhx = 0.f, dhy = 0.f;
+    if (j < cnt) {
+      const int64_t e = wbase + list[wv][j];
+      h = head[e];
+      const int32_t t = tail[e];
+      float2 yh = y[h];
+      if (h != t) {
+        float2 yt = y[t];
+        float2 dh = make_float2(0.f, 0.f), dt = make_float2(0.f, 0.f);
+        if (MIRROR) {
+          if (h < t) {
+            pull(yh, yt, h, t, dh, dt);
+            pull(yt, yh, t, h, dt, dh);
+          } else {
+            pull(yt, yh, t, h, dt, dh);
+            pull(yh, yt, h, t, dh, dt);
+          }
+        } else {
+          pull(yh, yt, h, t, dh, dt);
+          if (dt.x != 0.f || dt.y != 0.f) {
+            const float st = mscale ? mscale[t] : 1.f;
+            atomicAdd(&y[t].x, dt.x * st);
+            atomicAdd(&y[t].y, dt.y * st);
+          }
         }
+        dhx += dh.x;
+        dhy += dh.y;
+      }
+      if (pos_counts) pos_counts[e] += 1;
+      const int nneg = EdgeSchedule::negatives(eonns, eps, e, ef, nsr);
+      if (nneg > 0) {
+        if (neg_counts) neg_counts[e] += nneg;
+        const uint64_t key = mix64(key_epoch ^ static_cast<uint64_t>(e));
+        constexpr int kChunk = 8;
+        for (int s0 = 0; s0 < nneg; s0 += kChunk) {
+          float2 yk[kChunk];
+          int32_t kk[kChunk];
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          if (kk[j] == h) continue;  // also the padding past nneg
-          const float dx = yh.x - yk[j].x, dy = yh.y - yk[j].y;
-          const float d2 = dx * dx + dy * dy;
-          if (d2 > 0.f) {
-            const float pb = pow_pos(d2, b);
-            const float gc = rep * rcp((0.001f + d2) * (a * pb + 1.f));
-            if (gc > 0.f) {
-              const float gx = clip4(gc * dx) * alpha, gy = clip4(gc * dy) * alpha;
-              yh.x += gx;
-              yh.y += gy;
-              dhx += gx;
-              dhy += gy;
+          for (int q = 0; q < kChunk; ++q) {
+            kk[q] = (s0 + q < nneg) ? static_cast<int32_t>(draw_index(key, static_cast<uint32_t>(s0 + q), n)) : h;
+            yk[q] = y[kk[q]];
+          }
+#pragma unroll
+          for (int q = 0; q < kChunk; ++q) {
+            if (kk[q] == h) continue;  // also the padding past nneg
+            const float dx = yh.x - yk[q].x, dy = yh.y - yk[q].y;
+            const float d2 = dx * dx + dy * dy;
+            if (d2 > 0.f) {
+              const float pb = pow_pos(d2, b);
+              const float gc = rep * rcp((0.001f + d2) * (a * pb + 1.f));
+              if (gc > 0.f) {
+                const float gx = clip4(gc * dx) * alpha, gy = clip4(gc * dy) * alpha;
+                yh.x += gx;
+                yh.y += gy;
+                dhx += gx;
+                dhy += gy;
+              }
             }
           }
         }
       }
     }
-  }
-  // segmented suffix sums of the head deltas over equal-head lane runs
+    // segmented suffix sums of the head deltas over equal-head lane runs
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const float ox = __shfl_down(dhx, off);
-    const float oy = __shfl_down(dhy, off);
-    const int32_t oh = __shfl_down(h, off);
-    if (lane + off < 64 && oh == h) {
-      dhx += ox;
-      dhy += oy;
+    for (int off = 1; off < 64; off <<= 1) {
+      const float ox = __shfl_down(dhx, off);
+      const float oy = __shfl_down(dhy, off);
+      const int32_t oh = __shfl_down(h, off);
+      if (lane + off < 64 && oh == h) {
+        dhx += ox;
+        dhy += oy;
+      }
     }
-  }
-  const int32_t prev = __shfl_up(h, 1);
-  const bool seg_start = (lane == 0) || (prev != h);
-  if (seg_start && h >= 0 && (dhx != 0.f || dhy != 0.f)) {
-    atomicAdd(&y[h].x, dhx);
-    atomicAdd(&y[h].y, dhy);
+    const int32_t prev = __shfl_up(h, 1);
+    const bool seg_start = (lane == 0) || (prev != h);
+    if (seg_start && h >= 0 && (dhx != 0.f || dhy != 0.f)) {
+      const float sh = mscale ? mscale[h] : 1.f;
+      atomicAdd(&y[h].x, dhx * sh);
+      atomicAdd(&y[h].y, dhy * sh);
+    }
   }
 }
 
@@ -316,7 +355,8 @@ __global__ __launch_bounds__(256) void sgd_vertex2_kernel(
     float* __restrict__ eonns, int64_t c_begin, int64_t c_end, int chunk_len, uint32_t n,
     float ef, float alpha, float a, float b, float rep, float nsr, uint64_t key_epoch,
     const float* __restrict__ radii, int dens_on, float dens_lambda, float var_shift,
-    int32_t* __restrict__ pos_counts, int32_t* __restrict__ neg_counts) {
+    const float* __restrict__ mscale, int32_t* __restrict__ pos_counts,
+    int32_t* __restrict__ neg_counts) {
   const int64_t c = c_begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= c_end) return;
   const int32_t h = chunk_row[c];
@@ -334,6 +374,7 @@ __global__ __launch_bounds__(256) void sgd_vertex2_kernel(
     const int32_t t = tail[e];
     if (t != h) {
       float2 yt = y[t];
+      const float st = mscale ? mscale[t] : 1.f;
       const float dx = yh.x - yt.x, dy = yh.y - yt.y;
       const float d2 = dx * dx + dy * dy;
       if (d2 > 0.f) {
@@ -344,8 +385,8 @@ __global__ __launch_bounds__(256) void sgd_vertex2_kernel(
         yh.y += gy;
         yt.x -= gx;
         yt.y -= gy;
-        atomicAdd(&y[t].x, -gx);
-        atomicAdd(&y[t].y, -gy);
+        atomicAdd(&y[t].x, -gx * st);
+        atomicAdd(&y[t].y, -gy * st);
       }
       if (dens_on) {
         const float sx = yh.x - yt.x, sy = yh.y - yt.y;
@@ -356,8 +397,8 @@ __global__ __launch_bounds__(256) void sgd_vertex2_kernel(
         const float gx = clip4(coeff * sx * inv) * alpha, gy = clip4(coeff * sy * inv) * alpha;
         yh.x -= gx;
         yh.y -= gy;
-        atomicAdd(&y[t].x, gx);
-        atomicAdd(&y[t].y, gy);
+        atomicAdd(&y[t].x, gx * st);
+        atomicAdd(&y[t].y, gy * st);
       }
     }
     if (pos_counts) pos_counts[e] += 1;
@@ -392,8 +433,9 @@ __global__ __launch_bounds__(256) void sgd_vertex2_kernel(
     }
   }
   if (loaded) {
-    atomicAdd(&y[h].x, yh.x - start.x);
-    atomicAdd(&y[h].y, yh.y - start.y);
+    const float sh = mscale ? mscale[h] : 1.f;
+    atomicAdd(&y[h].x, (yh.x - start.x) * sh);
+    atomicAdd(&y[h].y, (yh.y - start.y) * sh);
   }
 }
 
@@ -458,62 +500,25 @@ __global__ __launch_bounds__(128) void transform_sgd_kernel(
   for (int c = 0; c < dim; ++c) yq[hq * dim + c] = cur[c];
 }
 
-// rev[e] = position of the reverse edge (tail[e], head[e]) in the tail's
-// row, found by binary search (rows sorted by column), or -1.
-__global__ void reverse_slots_kernel(const int32_t* __restrict__ row_ptr,
-                                     const int32_t* __restrict__ head,
-                                     const int32_t* __restrict__ tail, int64_t ne,
-                                     int32_t* __restrict__ rev) {
+// Edge ends per vertex (as head plus as tail), then the hub move scale.
+constexpr int kHubEnds = 64;
+
+__global__ void edge_ends_kernel(const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
+                                 int64_t ne, int32_t* __restrict__ ends) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= ne) return;
-  const int32_t h = head[e], t = tail[e];
-  int32_t lo = row_ptr[t], hi = row_ptr[t + 1];
-  while (lo < hi) {
-    const int32_t mid = lo + ((hi - lo) >> 1);
-    if (tail[mid] < h)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  rev[e] = (lo < row_ptr[t + 1] && tail[lo] == h) ? lo : -1;
+  atomicAdd(&ends[head[e]], 1);
+  atomicAdd(&ends[tail[e]], 1);
 }
 
-// End of a deferred-tail epoch: every row adds (and clears) the tail moves
-// left in its slots; slots are summed per row over the wave's lanes, then one
-// atomic per (wave, row) segment.
-__global__ __launch_bounds__(256) void apply_deferred_kernel(float2* __restrict__ y,
-                                                             const int32_t* __restrict__ head,
-                                                             float2* __restrict__ tdelta,
-                                                             int64_t ne) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  int32_t h = -1;
-  float dx = 0.f, dy = 0.f;
-  if (e < ne) {
-    h = head[e];
-    const float2 d = tdelta[e];
-    if (d.x != 0.f || d.y != 0.f) {
-      dx = d.x;
-      dy = d.y;
-      tdelta[e] = make_float2(0.f, 0.f);
-    }
-  }
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const float ox = __shfl_down(dx, off);
-    const float oy = __shfl_down(dy, off);
-    const int32_t oh = __shfl_down(h, off);
-    if (lane + off < 64 && oh == h) {
-      dx += ox;
-      dy += oy;
-    }
-  }
-  const int32_t prev = __shfl_up(h, 1);
-  const bool seg_start = (lane == 0) || (prev != h);
-  if (seg_start && h >= 0 && (dx != 0.f || dy != 0.f)) {
-    atomicAdd(&y[h].x, dx);
-    atomicAdd(&y[h].y, dy);
-  }
+__global__ void hub_scale_kernel(const int32_t* __restrict__ ends, int64_t n,
+                                 float* __restrict__ mscale, int32_t* __restrict__ hubs) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int32_t c = ends[v];
+  const bool hub = c > kHubEnds;
+  mscale[v] = hub ? static_cast<float>(kHubEnds) / static_cast<float>(c) : 1.f;
+  if (hub) atomicAdd(hubs, 1);
 }
 
 __global__ void copy_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
@@ -652,42 +657,46 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
   KERNEL_CHECK();
   const float rep = 2.f * args.repulsion_strength * args.b;
   const unsigned blocks = blocks_for(ne, 256);
-  // Large symmetric graphs (the fit's union): deferred tail moves, one launch
-  // per epoch (a vertex's tail moves land at the end of the epoch).  Smaller
-  // graphs, where every vertex moves many times per epoch relative to its
+  // Large symmetric graphs (the fit's union): every edge replays its pair's
+  // moves and keeps its head's share, one launch per epoch.  Smaller graphs,
+  // where every vertex moves many times per epoch relative to its
   // neighbourhood: one thread per head walks its edges in order with the head
   // in registers (Gauss-Seidel on the head, as the reference's loop), tail
   // moves atomic.  Large edge lists that are not the fit's union: edge
   // parallel with atomic tail moves.  The last two split each epoch into
   // kRangeLaunches launches over consecutive heads, so later rows see the
   // moves of earlier rows as in the reference's row order.
-  const bool large = ne >= kDeferredMinEdges;
-  const bool deferred = dim == 2 && args.symmetric && large;
+  const bool large = ne >= kMirrorMinEdges;
+  const bool mirror = dim == 2 && args.symmetric && large;
   const bool by_head = dim == 2 && !large;
-  if (deferred) {
-    ws.rev.ensure(ne);
-    ws.tdelta.ensure(ne);
-    hipLaunchKernelGGL(reverse_slots_kernel, dim3(blocks), dim3(256), 0, s, g.row_ptr.get(),
-                       g.head.get(), g.tail.get(), ne, ws.rev.get());
-    KERNEL_CHECK();
-    HIP_CHECK(hipMemsetAsync(ws.tdelta.get(), 0, sizeof(float2) * ne, s));
-  }
+  const unsigned epoch_blocks = blocks_for(ne, kEpochWaves * kWaveEdges);
   const int64_t n_chunks = by_head ? build_chunks(g, kChunkLen, ws, s) : 0;
+  // hub move scales (nullptr when no vertex has more than kHubEnds ends)
+  ws.ends.ensure(n + 1);
+  ws.mscale.ensure(n);
+  HIP_CHECK(hipMemsetAsync(ws.ends.get(), 0, sizeof(int32_t) * (n + 1), s));
+  hipLaunchKernelGGL(edge_ends_kernel, dim3(blocks), dim3(256), 0, s, g.head.get(), g.tail.get(),
+                     ne, ws.ends.get());
+  hipLaunchKernelGGL(hub_scale_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, ws.ends.get(), n,
+                     ws.mscale.get(), ws.ends.get() + n);
+  KERNEL_CHECK();
+  int32_t n_hubs = 0;
+  HIP_CHECK(hipMemcpyAsync(&n_hubs, ws.ends.get() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  const float* mscale = n_hubs > 0 ? ws.mscale.get() : nullptr;
   if (stats) HIP_CHECK(hipEventRecord(stats->start, s));
   for (int epoch = 0; epoch < args.n_epochs; ++epoch) {
     const float alpha = args.learning_rate *
                         (1.f - static_cast<float>(epoch) / static_cast<float>(args.n_epochs));
     const uint64_t key_epoch = host_mix64(args.seed + static_cast<uint64_t>(epoch) * 0xD1B54A32D192ED03ULL);
     const int dens_on = (args.radii != nullptr && epoch >= args.dens_start_epoch) ? 1 : 0;
-    if (deferred) {
-      hipLaunchKernelGGL(sgd_epoch2_kernel<kTailDeferred>, dim3(blocks), dim3(256), 0, s,
+    if (mirror) {
+      hipLaunchKernelGGL(sgd_epoch2_kernel<true>, dim3(epoch_blocks), dim3(64 * kEpochWaves), 0, s,
                          reinterpret_cast<float2*>(y), g.head.get(), g.tail.get(), ws.eps.get(),
-                         ws.eons.get(), ws.eonns.get(), ne, static_cast<uint32_t>(n),
+                         ws.eons.get(), ws.eonns.get(), static_cast<uint32_t>(n),
                          static_cast<float>(epoch), alpha, args.a, args.b, rep, nsr, key_epoch,
-                         args.radii, dens_on, args.dens_lambda, args.dens_var_shift, ws.rev.get(),
-                         ws.tdelta.get(), args.pos_counts, args.neg_counts, int64_t{0}, ne);
-      hipLaunchKernelGGL(apply_deferred_kernel, dim3(blocks), dim3(256), 0, s,
-                         reinterpret_cast<float2*>(y), g.head.get(), ws.tdelta.get(), ne);
+                         args.radii, dens_on, args.dens_lambda, args.dens_var_shift, mscale,
+                         args.pos_counts, args.neg_counts, int64_t{0}, ne);
     } else if (by_head) {
       for (int r = 0; r < kRangeLaunches; ++r) {
         const int64_t c0 = n_chunks * r / kRangeLaunches, c1 = n_chunks * (r + 1) / kRangeLaunches;
@@ -697,21 +706,24 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
                            g.row_ptr.get(), g.tail.get(), ws.eps.get(), ws.eons.get(),
                            ws.eonns.get(), c0, c1, kChunkLen, static_cast<uint32_t>(n),
                            static_cast<float>(epoch), alpha, args.a, args.b, rep, nsr, key_epoch,
-                           args.radii, dens_on, args.dens_lambda, args.dens_var_shift,
+                           args.radii, dens_on, args.dens_lambda, args.dens_var_shift, mscale,
                            args.pos_counts, args.neg_counts);
       }
     } else if (dim == 2) {
       for (int r = 0; r < kRangeLaunches; ++r) {
         // range bounds on wave boundaries
-        const int64_t e0 = (ne * r / kRangeLaunches) / 64 * 64;
-        const int64_t e1 = r + 1 == kRangeLaunches ? ne : (ne * (r + 1) / kRangeLaunches) / 64 * 64;
+        const int64_t e0 = (ne * r / kRangeLaunches) / kWaveEdges * kWaveEdges;
+        const int64_t e1 = r + 1 == kRangeLaunches
+                               ? ne
+                               : (ne * (r + 1) / kRangeLaunches) / kWaveEdges * kWaveEdges;
         if (e1 <= e0) continue;
-        hipLaunchKernelGGL(sgd_epoch2_kernel<kTailAtomic>, dim3(blocks_for(e1 - e0, 256)),
-                           dim3(256), 0, s, reinterpret_cast<float2*>(y), g.head.get(),
-                           g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(), ne,
-                           static_cast<uint32_t>(n), static_cast<float>(epoch), alpha, args.a,
-                           args.b, rep, nsr, key_epoch, args.radii, dens_on, args.dens_lambda,
-                           args.dens_var_shift, nullptr, nullptr, args.pos_counts,
+        hipLaunchKernelGGL(sgd_epoch2_kernel<false>,
+                           dim3(blocks_for(e1 - e0, kEpochWaves * kWaveEdges)),
+                           dim3(64 * kEpochWaves), 0, s, reinterpret_cast<float2*>(y),
+                           g.head.get(), g.tail.get(), ws.eps.get(), ws.eons.get(),
+                           ws.eonns.get(), static_cast<uint32_t>(n), static_cast<float>(epoch),
+                           alpha, args.a, args.b, rep, nsr, key_epoch, args.radii, dens_on,
+                           args.dens_lambda, args.dens_var_shift, mscale, args.pos_counts,
                            args.neg_counts, e0, e1);
       }
     } else {
@@ -719,7 +731,7 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
                          g.tail.get(), ws.eps.get(), ws.eons.get(), ws.eonns.get(), ne,
                          static_cast<uint32_t>(n), dim, static_cast<float>(epoch), alpha, args.a,
                          args.b, rep, nsr, key_epoch, args.radii, dens_on, args.dens_lambda,
-                         args.dens_var_shift, args.pos_counts, args.neg_counts);
+                         args.dens_var_shift, mscale, args.pos_counts, args.neg_counts);
     }
     KERNEL_CHECK();
   }

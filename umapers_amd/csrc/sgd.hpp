@@ -19,10 +19,9 @@ struct SgdArgs {
   int dens_start_epoch = 0;
   float dens_lambda = 0.f;
   float dens_var_shift = 0.1f;
-  // the edge list is the fit's fuzzy union: rows sorted by column and every
-  // edge (h, t) has exactly one reverse (t, h).  2-D tail moves are then
-  // deferred to the end of each epoch through the reverse edge's slot instead
-  // of float atomics.
+  // the edge list is the fit's fuzzy union: every edge (h, t) has exactly one
+  // reverse (t, h) of equal weight.  A 2-D tail move is then made by the
+  // reverse edge's own thread instead of a float atomic.
   bool symmetric = false;
   // optional per-edge sample counters (tests of the exact schedule)
   int32_t* pos_counts = nullptr;
@@ -31,9 +30,9 @@ struct SgdArgs {
 
 struct SgdWorkspace {
   DevBuf<float> eps, eons, eonns;
-  DevBuf<int32_t> rev;      // reverse-edge slot of every edge (deferred tails)
-  DevBuf<float2> tdelta;    // deferred tail moves, one slot per edge
   DevBuf<int32_t> chunk_row, chunk_start;  // (head, first edge) of each walked chunk
+  DevBuf<int32_t> ends;     // edge ends per vertex (+ hub count)
+  DevBuf<float> mscale;     // per-vertex move scale (hubs below 1)
 };
 
 struct SgdStats {

@@ -5,6 +5,9 @@
 // twice-iterated modified Gram-Schmidt (rust_umap/src/lib.rs:4161-4326), then a
 // Ritz eigen-solve.  Here all components of at least 128 rows iterate together
 // on the GPU:
+//   - the block lives in f32 (the product's traffic is one neighbour-row
+//     gather per graph entry, so storage width is its cost); Gram matrices,
+//     Cholesky factors, triangular solves and the Ritz problem are f64;
 //   - the shifted normalised adjacency product is a CSR row gather;
 //   - orthonormalisation is CholeskyQR2 per component (Gram matrix by a
 //     segmented reduction over the component's rows, an 18x18 Cholesky, a
@@ -33,8 +36,9 @@ constexpr int kIters = 256;         // SPECTRAL_ITERATIVE_MAX_ITERS
 // after each one.  For an upper-triangular R with a positive diagonal,
 // orth(A R^-1) = orth(A), so in exact arithmetic the iterates equal the
 // reference's; with the operator's spectrum in [0, 2] four unnormalised
-// products raise the block's condition number by at most (2 / lambda_18)^4,
-// far inside what CholeskyQR2 handles in f64.
+// products raise the block's condition number kappa by at most
+// (2 / lambda_18)^4, far inside what CholeskyQR2 handles in f64, and storing
+// the block in f32 moves its weakest direction by about kappa * 2^-24.
 constexpr int kOrthEvery = 4;
 constexpr int kItemRows = 128;      // rows per work item of the iteration kernels
 constexpr double kOrthoEps = 1e-12; // SPECTRAL_ORTHO_EPS
@@ -115,7 +119,7 @@ __global__ void prow_fill_kernel(const int32_t* __restrict__ perm, int64_t rows,
                                  const int32_t* __restrict__ tail, const float* __restrict__ w,
                                  const double* __restrict__ inv, const int32_t* __restrict__ pos_of,
                                  const int32_t* __restrict__ prow_ptr, int32_t* __restrict__ pcol,
-                                 double* __restrict__ pnw) {
+                                 float* __restrict__ pnw) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= rows) return;
   const int32_t v = perm[p];
@@ -124,7 +128,7 @@ __global__ void prow_fill_kernel(const int32_t* __restrict__ perm, int64_t rows,
   for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e, ++o) {
     const int32_t t = tail[e];
     pcol[o] = pos_of[t];
-    pnw[o] = (t == v) ? 0.0 : static_cast<double>(fmaxf(w[e], 0.f)) * iv * inv[t];
+    pnw[o] = (t == v) ? 0.f : static_cast<float>(static_cast<double>(fmaxf(w[e], 0.f)) * iv * inv[t]);
   }
 }
 
@@ -134,16 +138,17 @@ __global__ void start_block_kernel(const int32_t* __restrict__ perm, int64_t row
                                    const int32_t* __restrict__ slot_of_pos,
                                    const int32_t* __restrict__ local_of_pos,
                                    const uint64_t* __restrict__ seeds,
-                                   const double* __restrict__ deg, double* __restrict__ yp) {
+                                   const double* __restrict__ deg, float* __restrict__ yp) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= rows) return;
   const int32_t v = perm[p];
   const uint64_t key = seeds[slot_of_pos[p]] ^ 0x7A5B3D91CE42A117ULL;
   const uint64_t lr = static_cast<uint64_t>(local_of_pos[p]);
-  double* row = yp + p * kSub;
-  row[0] = sqrt(deg[v]);
+  float* row = yp + p * kSub;
+  row[0] = static_cast<float>(sqrt(deg[v]));
   for (int c = 1; c < kSub; ++c)
-    row[c] = 2.0 * uniform01(key + lr * kSub + static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ULL) - 1.0;
+    row[c] = static_cast<float>(
+        2.0 * uniform01(key + lr * kSub + static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ULL) - 1.0);
 }
 
 // Work items are laid out component by component.  Hardware hands block b to
@@ -170,107 +175,153 @@ __device__ __forceinline__ void zero_range(double* buf, int len) {
     for (int i = threadIdx.x; i < len; i += blockDim.x) buf[i] = 0.0;
 }
 
-// z = y + D^-1/2 W D^-1/2 y, in two kernels so that hub rows (in-degree in
-// the hundreds after the union) do not hold a wave for their whole length:
-// rows with at most kLongRow entries take 9 lanes each (lane j carries
-// columns 2j, 2j+1, 8 neighbour gathers in flight per lane); longer rows are
-// listed once and each takes a block whose 28 lane groups split its entries
-// and sum their partials in LDS.
-constexpr int kLongRow = 64;
+// z = y + D^-1/2 W D^-1/2 y.  Work is cut into pieces of at most kSeg
+// entries so that hub rows (hundreds of entries after the union) do not hold
+// a wave for their whole length: a row with at most kSeg entries is one
+// piece that writes its z row directly; a longer row is split into kSeg-entry
+// pieces whose partial sums go to a scratch table (slot = piece index), and a
+// second kernel adds each long row's partials in piece order, so the result
+// does not depend on scheduling.  Every piece takes 9 lanes (lane j carries
+// columns 2j, 2j+1).
+//
+// The block is stored in f32 (72 B a row): the product is a gather of one
+// neighbour row per graph entry, so the row width sets the traffic.  The
+// normalised weights are formed in f64 and rounded once; the Gram matrices
+// and the Cholesky factors below stay in f64.
+constexpr int kSeg = 64;
 
-__global__ __launch_bounds__(256) void spmm_short_kernel(int64_t rows,
-                                                         const int32_t* __restrict__ prow_ptr,
-                                                         const int32_t* __restrict__ pcol,
-                                                         const double* __restrict__ pnw,
-                                                         const double* __restrict__ yp,
-                                                         double* __restrict__ zp) {
+// A wave takes 7 pieces (lanes 9g .. 9g+8 for piece g; lane 63 idles).  The
+// piece's column indices and weights are first staged in LDS by coalesced
+// loads of its 9 lanes; each lane then walks the staged entries (LDS
+// broadcast reads) and gathers its two columns of every neighbour row, 8
+// gathers in flight.  Staging keeps the vector memory pipeline for the
+// neighbour rows, which bound the kernel.
+constexpr int kPiecesPerWave = 7;
+constexpr int kSpmmWaves = 4;
+
+__global__ __launch_bounds__(64 * kSpmmWaves) void spmm_kernel(
+    int64_t n_pieces, const int32_t* __restrict__ piece_row,
+    const int32_t* __restrict__ piece_start, const int32_t* __restrict__ prow_ptr,
+    const int32_t* __restrict__ pcol, const float* __restrict__ pnw,
+    const float* __restrict__ yp, float* __restrict__ zp, float2* __restrict__ part) {
+  __shared__ int2 staged[kSpmmWaves][kPiecesPerWave][kSeg];
+  const int wv = threadIdx.x >> 6;
+  const int wl = threadIdx.x & 63;
+  const int g = wl / 9;
+  const int lj = wl - 9 * g;
+  const int64_t i = (static_cast<int64_t>(blockIdx.x) * kSpmmWaves + wv) * kPiecesPerWave + g;
+  const bool valid = g < kPiecesPerWave && i < n_pieces;
+  int32_t p = 0, e0 = 0, cnt = 0;
+  bool whole = true;
+  if (valid) {
+    p = piece_row[i];
+    e0 = piece_start[i];
+    const int32_t r0 = prow_ptr[p], r1 = prow_ptr[p + 1];
+    whole = r1 - r0 <= kSeg;
+    cnt = whole ? r1 - e0 : min(kSeg, r1 - e0);
+    for (int k = lj; k < cnt; k += 9)
+      staged[wv][g][k] = make_int2(pcol[e0 + k], __float_as_int(pnw[e0 + k]));
+  }
+  // the entries are written and read by lanes of one wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (!valid) return;
+  const float2* y2 = reinterpret_cast<const float2*>(yp);
+  const int2* st = staged[wv][g];
+  float2 acc = whole ? y2[static_cast<int64_t>(p) * (kSub / 2) + lj] : make_float2(0.f, 0.f);
+  int k = 0;
+  for (; k + 8 <= cnt; k += 8) {
+    int2 c[8];
+    float2 q[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c[j] = st[k + j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) q[j] = y2[static_cast<int64_t>(c[j].x) * (kSub / 2) + lj];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float w = __int_as_float(c[j].y);
+      acc.x = fmaf(w, q[j].x, acc.x);
+      acc.y = fmaf(w, q[j].y, acc.y);
+    }
+  }
+  for (; k < cnt; ++k) {
+    const int2 c = st[k];
+    const float2 q = y2[static_cast<int64_t>(c.x) * (kSub / 2) + lj];
+    const float w = __int_as_float(c.y);
+    acc.x = fmaf(w, q.x, acc.x);
+    acc.y = fmaf(w, q.y, acc.y);
+  }
+  if (whole)
+    reinterpret_cast<float2*>(zp)[static_cast<int64_t>(p) * (kSub / 2) + lj] = acc;
+  else
+    part[i * (kSub / 2) + lj] = acc;
+}
+
+// z row of every long row: its own y row plus its pieces' partials in order.
+__global__ void spmm_combine_kernel(int64_t n_long, const int32_t* __restrict__ long_list,
+                                    const int32_t* __restrict__ prow_ptr,
+                                    const int32_t* __restrict__ piece_off,
+                                    const float2* __restrict__ part,
+                                    const float* __restrict__ yp, float* __restrict__ zp) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t p = gid / 9;
+  const int64_t i = gid / 9;
   const int lj = static_cast<int>(gid % 9);
-  if (p >= rows) return;
-  const int32_t e0 = prow_ptr[p], e1 = prow_ptr[p + 1];
-  if (e1 - e0 > kLongRow) return;
-  const double2* y2 = reinterpret_cast<const double2*>(yp);
-  double2 acc = y2[p * (kSub / 2) + lj];
-  int32_t e = e0;
-  for (; e + 8 <= e1; e += 8) {
-    int32_t t[8];
-    double w[8];
-    double2 q[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      t[j] = pcol[e + j];
-      w[j] = pnw[e + j];
-    }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) q[j] = y2[static_cast<int64_t>(t[j]) * (kSub / 2) + lj];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      acc.x += w[j] * q[j].x;
-      acc.y += w[j] * q[j].y;
-    }
+  if (i >= n_long) return;
+  const int64_t p = long_list[i];
+  const int32_t np = (prow_ptr[p + 1] - prow_ptr[p] + kSeg - 1) / kSeg;
+  float2 acc = reinterpret_cast<const float2*>(yp)[p * (kSub / 2) + lj];
+  for (int32_t q = piece_off[p]; q < piece_off[p] + np; ++q) {
+    const float2 v = part[static_cast<int64_t>(q) * (kSub / 2) + lj];
+    acc.x += v.x;
+    acc.y += v.y;
   }
-  for (; e < e1; ++e) {
-    const double wv = pnw[e];
-    const double2 q = y2[static_cast<int64_t>(pcol[e]) * (kSub / 2) + lj];
-    acc.x += wv * q.x;
-    acc.y += wv * q.y;
-  }
-  reinterpret_cast<double2*>(zp)[p * (kSub / 2) + lj] = acc;
+  reinterpret_cast<float2*>(zp)[p * (kSub / 2) + lj] = acc;
 }
 
 __global__ void long_rows_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
                                  int32_t* __restrict__ list, int32_t* __restrict__ count) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p < rows && prow_ptr[p + 1] - prow_ptr[p] > kLongRow)
+  if (p < rows && prow_ptr[p + 1] - prow_ptr[p] > kSeg)
     list[atomicAdd(count, 1)] = static_cast<int32_t>(p);
 }
 
-__global__ __launch_bounds__(256) void spmm_long_kernel(const int32_t* __restrict__ list,
-                                                        const int32_t* __restrict__ prow_ptr,
-                                                        const int32_t* __restrict__ pcol,
-                                                        const double* __restrict__ pnw,
-                                                        const double* __restrict__ yp,
-                                                        double* __restrict__ zp) {
-  constexpr int kGroups = 256 / 9;
-  __shared__ double2 part[kGroups][kSub / 2];
-  const int64_t p = list[blockIdx.x];
-  const int tid = threadIdx.x;
-  const int grp = tid / 9, lj = tid % 9;
-  const double2* y2 = reinterpret_cast<const double2*>(yp);
-  const int32_t e0 = prow_ptr[p], e1 = prow_ptr[p + 1];
-  if (grp < kGroups) {
-    double2 acc = {0.0, 0.0};
-    for (int32_t e = e0 + grp; e < e1; e += kGroups) {
-      const double wv = pnw[e];
-      const double2 q = y2[static_cast<int64_t>(pcol[e]) * (kSub / 2) + lj];
-      acc.x += wv * q.x;
-      acc.y += wv * q.y;
-    }
-    part[grp][lj] = acc;
-  }
-  __syncthreads();
-  if (tid < kSub / 2) {
-    double2 acc = y2[p * (kSub / 2) + tid];
-    for (int g = 0; g < kGroups; ++g) {
-      acc.x += part[g][tid].x;
-      acc.y += part[g][tid].y;
-    }
-    reinterpret_cast<double2*>(zp)[p * (kSub / 2) + tid] = acc;
+__global__ void piece_count_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
+                                   int32_t* __restrict__ cnt) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const int32_t len = prow_ptr[p + 1] - prow_ptr[p];
+  cnt[p] = len <= kSeg ? 1 : (len + kSeg - 1) / kSeg;
+}
+
+__global__ void piece_fill_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
+                                  const int32_t* __restrict__ piece_off,
+                                  int32_t* __restrict__ piece_row,
+                                  int32_t* __restrict__ piece_start) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  int32_t q = piece_off[p];
+  const int32_t r0 = prow_ptr[p], r1 = prow_ptr[p + 1];
+  piece_row[q] = static_cast<int32_t>(p);
+  piece_start[q] = r0;
+  for (int32_t e = r0 + kSeg; e < r1; e += kSeg) {
+    ++q;
+    piece_row[q] = static_cast<int32_t>(p);
+    piece_start[q] = e;
   }
 }
 
-// The item's share of Z^T Z (or Y^T Z when CROSS) added to gram[slot]; the
-// rows are already computed.  zero_buf is cleared on the way (see
-// zero_range).
+// The item's share of Z^T Z (or Y^T Z when CROSS) added to gram[slot], with
+// f64 products of the stored f32 values; the rows are already computed.
+// zero_buf is cleared on the way (see zero_range).
 template <bool CROSS>
 __global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ items,
-                                                   const double* __restrict__ yp,
-                                                   const double* __restrict__ zp,
+                                                   const float* __restrict__ yp,
+                                                   const float* __restrict__ zp,
                                                    double* __restrict__ gram,
                                                    double* __restrict__ zero_buf, int zero_len) {
-  __shared__ double sz[kItemRows][kSub + 1];
-  __shared__ double sy[CROSS ? kItemRows : 1][kSub + 1];
+  __shared__ float sz[kItemRows][kSub + 1];
+  __shared__ float sy[CROSS ? kItemRows : 1][kSub + 1];
   zero_range(zero_buf, zero_len);
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int32_t slot = items[item * 3 + 0];
@@ -288,26 +339,29 @@ __global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ i
     pair_of(tid, pi, pj);
     double acc = 0.0;
     if (CROSS) {
-      for (int r = 0; r < cnt; ++r) acc += sy[r][pi] * sz[r][pj];
+      for (int r = 0; r < cnt; ++r)
+        acc = fma(static_cast<double>(sy[r][pi]), static_cast<double>(sz[r][pj]), acc);
     } else {
-      for (int r = 0; r < cnt; ++r) acc += sz[r][pi] * sz[r][pj];
+      for (int r = 0; r < cnt; ++r)
+        acc = fma(static_cast<double>(sz[r][pi]), static_cast<double>(sz[r][pj]), acc);
     }
     atomicAdd(&gram[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
   }
 }
 
 // Cholesky of gram[slot] (upper R, G = R^T R) in LDS, then dst = src R^-1 for
-// the item's rows, and (GRAM) the item's share of dst^T dst into gram_out.
-// A first-pass pivot <= 1e-12 (or non-finite) flags the slot as failed: it
-// equals the residual norm Gram-Schmidt checks in lib.rs:4186-4190.
+// the item's rows (solved in f64, stored f32), and (GRAM) the item's share of
+// dst^T dst into gram_out.  A first-pass pivot <= 1e-12 (or non-finite) flags
+// the slot as failed: it equals the residual norm Gram-Schmidt checks in
+// lib.rs:4186-4190.
 template <bool GRAM>
 __global__ __launch_bounds__(256) void iter_orth_kernel(
     const int32_t* __restrict__ items, const double* __restrict__ gram_in,
-    const double* __restrict__ src, double* __restrict__ dst, double* __restrict__ gram_out,
+    const float* __restrict__ src, float* __restrict__ dst, double* __restrict__ gram_out,
     int32_t* __restrict__ fail, double* __restrict__ zero_buf, int zero_len) {
   __shared__ double G[kSub][kSub];
   __shared__ double R[kSub][kSub + 1];
-  __shared__ double sz[GRAM ? kItemRows : 1][kSub + 1];
+  __shared__ float sz[GRAM ? kItemRows : 1][kSub + 1];
   __shared__ int bad;
   zero_range(zero_buf, zero_len);
   const int item = xcd_item(blockIdx.x, gridDim.x);
@@ -344,7 +398,7 @@ __global__ __launch_bounds__(256) void iter_orth_kernel(
   if (tid == 0 && bad && fail) fail[slot] = 1;
   if (tid < cnt) {
     const int64_t p = p0 + tid;
-    const double* row = src + p * kSub;
+    const float* row = src + p * kSub;
     double z[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) {
@@ -353,11 +407,12 @@ __global__ __launch_bounds__(256) void iter_orth_kernel(
       for (int k = 0; k < j; ++k) s -= z[k] * R[k][j];
       z[j] = s / R[j][j];
     }
-    double* out = dst + p * kSub;
+    float* out = dst + p * kSub;
 #pragma unroll
     for (int j = 0; j < kSub; ++j) {
-      out[j] = z[j];
-      if (GRAM) sz[tid][j] = z[j];
+      const float f = static_cast<float>(z[j]);
+      out[j] = f;
+      if (GRAM) sz[tid][j] = f;
     }
   }
   if (GRAM) {
@@ -366,32 +421,48 @@ __global__ __launch_bounds__(256) void iter_orth_kernel(
       int pi, pj;
       pair_of(tid, pi, pj);
       double acc = 0.0;
-      for (int r = 0; r < cnt; ++r) acc += sz[r][pi] * sz[r][pj];
+      for (int r = 0; r < cnt; ++r)
+        acc = fma(static_cast<double>(sz[r][pi]), static_cast<double>(sz[r][pj]), acc);
       atomicAdd(&gram_out[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
     }
   }
 }
 
-// coords = Y V_slot (kSub x dim) in f64, stored f32; per-slot max |coord|.
-__global__ void project_kernel(const int32_t* __restrict__ perm, int64_t rows,
-                               const int32_t* __restrict__ slot_of_pos,
-                               const double* __restrict__ y, const double* __restrict__ vmat,
-                               int dim, float* __restrict__ out, float* __restrict__ maxabs) {
+// coords = Y V_slot (kSub x dim) in f64, stored f32; per-slot max |coord|
+// (reduced over the lanes of a wave that share a slot, then one atomic).
+__global__ __launch_bounds__(256) void project_kernel(const int32_t* __restrict__ perm, int64_t rows,
+                                                      const int32_t* __restrict__ slot_of_pos,
+                                                      const float* __restrict__ y,
+                                                      const double* __restrict__ vmat, int dim,
+                                                      float* __restrict__ out,
+                                                      float* __restrict__ maxabs) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= rows) return;
-  const int32_t v = perm[p];
-  const int32_t slot = slot_of_pos[p];
-  const double* row = y + p * kSub;
-  const double* vm = vmat + static_cast<int64_t>(slot) * kSub * dim;
+  const int lane = threadIdx.x & 63;
+  int32_t slot = -1;
   float mx = 0.f;
-  for (int c = 0; c < dim; ++c) {
-    double s = 0.0;
-    for (int b = 0; b < kSub; ++b) s += row[b] * vm[b * dim + c];
-    const float f = static_cast<float>(s);
-    out[static_cast<int64_t>(v) * dim + c] = f;
-    mx = fmaxf(mx, fabsf(f));
+  if (p < rows) {
+    const int32_t v = perm[p];
+    slot = slot_of_pos[p];
+    const float* row = y + p * kSub;
+    const double* vm = vmat + static_cast<int64_t>(slot) * kSub * dim;
+    for (int c = 0; c < dim; ++c) {
+      double s = 0.0;
+      for (int b = 0; b < kSub; ++b) s += static_cast<double>(row[b]) * vm[b * dim + c];
+      const float f = static_cast<float>(s);
+      out[static_cast<int64_t>(v) * dim + c] = f;
+      mx = fmaxf(mx, fabsf(f));
+    }
   }
-  atomicMax(reinterpret_cast<unsigned int*>(&maxabs[slot]), __float_as_uint(mx));
+  // positions are laid out slot by slot, so equal slots are adjacent lanes
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const float o = __shfl_down(mx, off);
+    const int32_t os = __shfl_down(slot, off);
+    if (lane + off < 64 && os == slot) mx = fmaxf(mx, o);
+  }
+  const int32_t prev = __shfl_up(slot, 1);
+  if (slot >= 0 && (lane == 0 || prev != slot))
+    atomicMax(reinterpret_cast<unsigned int*>(&maxabs[slot]), __float_as_uint(mx));
 }
 
 // y = coords * scale[slot] + N(0, 1e-4) + anchor[slot]  (noisy_scale_coords /
@@ -804,7 +875,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     const int32_t pe32 = static_cast<int32_t>(pe);
     HIP_CHECK(hipMemcpyAsync(prow_ptr.get() + rows, &pe32, 4, hipMemcpyHostToDevice, s));
     DevBuf<int32_t> pcol(std::max<int64_t>(pe, 1));
-    DevBuf<double> pnw(std::max<int64_t>(pe, 1));
+    DevBuf<float> pnw(std::max<int64_t>(pe, 1));
     hipLaunchKernelGGL(prow_fill_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
                        g.row_ptr.get(), g.tail.get(), g.weight.get(), ws.inv.get(), ws.label.get(),
                        prow_ptr.get(), pcol.get(), pnw.get());
@@ -816,8 +887,8 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     double* g2 = ws.gram.get() + glen;
     HIP_CHECK(hipMemsetAsync(ws.fail.get(), 0, 4 * n_slots, s));
     HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * 2 * glen, s));
-    double* yb = ws.y.get();
-    double* zb = ws.z.get();
+    float* yb = ws.y.get();
+    float* zb = ws.z.get();
     const int32_t* it_items = ws.items.get();
     // Orthonormalise Z into Y (CholeskyQR2): g1 holds Z^T Z on entry.
     auto orth_into_y = [&]() {
@@ -828,26 +899,37 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
       KERNEL_CHECK();
     };
     // start block: Z = Y, g1 = Y^T Y
-    HIP_CHECK(hipMemcpyAsync(zb, yb, sizeof(double) * rows * kSub, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(zb, yb, sizeof(float) * rows * kSub, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1, g2,
                        glen);
     KERNEL_CHECK();
     orth_into_y();
-    // rows too long for the 9-lane path
-    DevBuf<int32_t> long_list(rows), long_count(1);
+    // pieces of at most kSeg entries in row order; rows longer than kSeg
+    DevBuf<int32_t> long_list(rows), long_count(1), piece_cnt(rows), piece_off(rows);
     HIP_CHECK(hipMemsetAsync(long_count.get(), 0, 4, s));
     hipLaunchKernelGGL(long_rows_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
                        prow_ptr.get(), long_list.get(), long_count.get());
+    hipLaunchKernelGGL(piece_count_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                       prow_ptr.get(), piece_cnt.get());
+    KERNEL_CHECK();
+    const int64_t n_pieces = exclusive_scan(piece_cnt.get(), piece_off.get(), rows, scan_ws, s);
+    DevBuf<int32_t> piece_row(n_pieces), piece_start(n_pieces);
+    DevBuf<float2> part(static_cast<size_t>(n_pieces) * (kSub / 2));
+    hipLaunchKernelGGL(piece_fill_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                       prow_ptr.get(), piece_off.get(), piece_row.get(), piece_start.get());
     KERNEL_CHECK();
     int32_t n_long = 0;
     HIP_CHECK(hipMemcpyAsync(&n_long, long_count.get(), 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    auto product = [&](const double* src, double* dst) {
-      hipLaunchKernelGGL(spmm_short_kernel, dim3(blocks_for(rows * 9, 256)), dim3(256), 0, s, rows,
-                         prow_ptr.get(), pcol.get(), pnw.get(), src, dst);
+    auto product = [&](const float* src, float* dst) {
+      hipLaunchKernelGGL(spmm_kernel, dim3(blocks_for(n_pieces, kSpmmWaves * kPiecesPerWave)),
+                         dim3(64 * kSpmmWaves), 0, s,
+                         n_pieces, piece_row.get(), piece_start.get(), prow_ptr.get(), pcol.get(),
+                         pnw.get(), src, dst, part.get());
       if (n_long > 0)
-        hipLaunchKernelGGL(spmm_long_kernel, dim3(n_long), dim3(256), 0, s, long_list.get(),
-                           prow_ptr.get(), pcol.get(), pnw.get(), src, dst);
+        hipLaunchKernelGGL(spmm_combine_kernel, dim3(blocks_for(static_cast<int64_t>(n_long) * 9, 256)),
+                           dim3(256), 0, s, static_cast<int64_t>(n_long), long_list.get(),
+                           prow_ptr.get(), piece_off.get(), part.get(), src, dst);
       KERNEL_CHECK();
     };
     for (int it = 0; it < kIters; ++it) {

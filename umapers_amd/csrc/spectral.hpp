@@ -8,7 +8,8 @@ namespace umapx {
 
 struct SpectralWorkspace {
   DevBuf<int32_t> label, flag, perm, comp_of, items, slot_of;
-  DevBuf<double> deg, inv, y, z, gram, rmat, vmat, cent;
+  DevBuf<double> deg, inv, gram, rmat, vmat, cent;
+  DevBuf<float> y, z;  // the iterated block and its product, [rows][18] f32
   DevBuf<uint64_t> seeds;
   DevBuf<float> maxabs, scale, anchor;
   DevBuf<int32_t> fail;
