@@ -38,6 +38,15 @@ void check_hip(hipError_t e, const char* what, const char* file, int line) {
   throw Error(kDeviceError, msg);
 }
 
+void retain_pool_memory() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  hipMemPool_t pool;
+  HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t keep = UINT64_MAX;
+  HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+}
+
 void maybe_inject_fault(const char* stage) {
   static const char* env = std::getenv("UMAPX_FAULT_INJECT");
   if (env && std::strcmp(env, stage) == 0)
@@ -208,12 +217,25 @@ struct StageClock {
 using namespace umapx;
 
 struct umapx_handle {
+  // Declared first so that it is destroyed last: the buffers below are
+  // stream-ordered allocations on this stream and are freed on it.
+  struct StreamOwner {
+    hipStream_t s = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~StreamOwner() {
+      if (s) (void)hipStreamSynchronize(s);
+      for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } owner;
+  hipStream_t& stream = owner.s;
+  hipEvent_t (&ev)[4] = owner.ev;
   umapx_params p{};
   float a = 0.f, b = 0.f;
   bool fitted = false;
   int64_t n = 0, d = 0;
   int k_fit = 0;
-  hipStream_t stream = nullptr;
   DevBuf<float> x, y, sig, rho, rad_orig, rad_emb, knn_dist;
   DevBuf<int32_t> knn_idx;
   KnnWorkspace kws;
@@ -224,18 +246,13 @@ struct umapx_handle {
   MlpModel mlp;
   umapx_timings timings{};
   bool count_schedule = false;  // set by the profile entry point
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
   void ensure_stream() {
     if (!stream) {
       HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
       for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+      retain_pool_memory();
     }
-  }
-  ~umapx_handle() {
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
   }
 
   std::pair<float, float> resolve_ab() {
@@ -250,6 +267,7 @@ struct umapx_handle {
                   const float* pre_dist, bool validated_on_host, double t_start) {
     ensure_stream();
     hipStream_t s = stream;
+    AllocStreamScope alloc_scope(s);
     umapx_timings t{};
     StageClock clk(s);
     if (!validated_on_host) validate_data_device(d_x, n_, d_, s);
@@ -410,6 +428,7 @@ struct umapx_handle {
   void transform_plain_device(const float* d_q, int64_t m, float* d_out) {
     ensure_stream();
     hipStream_t s = stream;
+    AllocStreamScope alloc_scope(s);
     const int k = static_cast<int>(std::min<int64_t>(p.n_neighbors, n));
     const int32_t n_epochs = p.n_epochs > 0 ? std::max(p.n_epochs, 1) : (m <= 10000 ? 100 : 30);
     DevBuf<int32_t> idx(static_cast<size_t>(m) * k);
@@ -459,6 +478,7 @@ struct umapx_handle {
   }
 
   void run_transform(const float* d_q, int64_t m, float* d_out, int mode) {
+    AllocStreamScope alloc_scope(stream);
     if (mode == UMAPX_TRANSFORM_LEARNED) {
       if (!mlp.trained) {
         mlp_train(mlp, x.get(), n, d, y.get(), p.n_components, graph, a, b, p, stream);

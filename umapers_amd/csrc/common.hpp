@@ -45,26 +45,54 @@ void check_hip(hipError_t e, const char* what, const char* file, int line);
 // check error propagation through the C-ABI).
 void maybe_inject_fault(const char* stage);
 
-// RAII device buffer.
+// Stream on which DevBuf allocations of this thread are stream-ordered
+// (hipMallocAsync / hipFreeAsync from the device's default memory pool),
+// set by AllocStreamScope while a handle works on its own stream; with no
+// scope active, DevBuf uses hipMalloc / hipFree.  Stream-ordered buffers are
+// freed without a device-wide synchronisation, so a fit's temporaries do not
+// stall the queue between stages.
+inline hipStream_t& alloc_stream() {
+  static thread_local hipStream_t s = nullptr;
+  return s;
+}
+
+struct AllocStreamScope {
+  hipStream_t prev;
+  explicit AllocStreamScope(hipStream_t s) : prev(alloc_stream()) { alloc_stream() = s; }
+  ~AllocStreamScope() { alloc_stream() = prev; }
+  AllocStreamScope(const AllocStreamScope&) = delete;
+  AllocStreamScope& operator=(const AllocStreamScope&) = delete;
+};
+
+// Keeps memory freed to the current device's default pool for reuse instead
+// of returning it to the driver at synchronisation points.
+void retain_pool_memory();
+
+// RAII device buffer.  A buffer allocated under an AllocStreamScope is freed
+// on the same stream, which must outlive it.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  hipStream_t s = nullptr;  // allocation stream (stream-ordered), or nullptr
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s) {
     o.p = nullptr;
     o.n = 0;
+    o.s = nullptr;
   }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
       p = o.p;
       n = o.n;
+      s = o.s;
       o.p = nullptr;
       o.n = 0;
+      o.s = nullptr;
     }
     return *this;
   }
@@ -73,7 +101,11 @@ struct DevBuf {
     release();
     if (count == 0) return;
     maybe_inject_fault("alloc");
-    HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    s = alloc_stream();
+    if (s)
+      HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), s));
+    else
+      HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
     n = count;
   }
   // Grow-only reallocation (contents are not kept).
@@ -81,9 +113,15 @@ struct DevBuf {
     if (count > n) alloc(count);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (s)
+        (void)hipFreeAsync(p, s);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    s = nullptr;
   }
   T* get() const { return p; }
   size_t size() const { return n; }

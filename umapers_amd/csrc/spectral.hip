@@ -9,11 +9,11 @@
 //     gather per graph entry, so storage width is its cost); Gram matrices,
 //     Cholesky factors, triangular solves and the Ritz problem are f64;
 //   - the shifted normalised adjacency product is a CSR row gather;
-//   - orthonormalisation is CholeskyQR2 per component (Gram matrix by a
-//     segmented reduction over the component's rows, an 18x18 Cholesky, a
-//     triangular solve per row, done twice).  In exact arithmetic it returns
-//     the same Q as Gram-Schmidt (the QR factor with a positive diagonal is
-//     unique) and its first-pass pivots are the residual norms Gram-Schmidt
+//   - orthonormalisation is CholeskyQR per component (Gram matrix by a
+//     segmented reduction over the component's rows, an 18x18 Cholesky and
+//     its inverse, then every row times R^-1).  In exact arithmetic it
+//     returns the same Q as Gram-Schmidt (the QR factor with a positive
+//     diagonal is unique) and its pivots are the residual norms Gram-Schmidt
 //     checks against 1e-12;
 //   - the 18x18 Ritz problems are solved on the host.
 // Smaller components take the reference's dense Laplacian path on the host,
@@ -37,7 +37,7 @@ constexpr int kIters = 256;         // SPECTRAL_ITERATIVE_MAX_ITERS
 // orth(A R^-1) = orth(A), so in exact arithmetic the iterates equal the
 // reference's; with the operator's spectrum in [0, 2] four unnormalised
 // products raise the block's condition number kappa by at most
-// (2 / lambda_18)^4, far inside what CholeskyQR2 handles in f64, and storing
+// (2 / lambda_18)^4, far inside what CholeskyQR handles in f64, and storing
 // the block in f32 moves its weakest direction by about kappa * 2^-24.
 constexpr int kOrthEvery = 4;
 constexpr int kItemRows = 128;      // rows per work item of the iteration kernels
@@ -168,11 +168,6 @@ __device__ __forceinline__ void pair_of(int tid, int& pi, int& pj) {
   }
   pi = i;
   pj = i + t;
-}
-
-__device__ __forceinline__ void zero_range(double* buf, int len) {
-  if (buf && blockIdx.x == 0)
-    for (int i = threadIdx.x; i < len; i += blockDim.x) buf[i] = 0.0;
 }
 
 // z = y + D^-1/2 W D^-1/2 y.  Work is cut into pieces of at most kSeg
@@ -313,16 +308,13 @@ __global__ void piece_fill_kernel(int64_t rows, const int32_t* __restrict__ prow
 
 // The item's share of Z^T Z (or Y^T Z when CROSS) added to gram[slot], with
 // f64 products of the stored f32 values; the rows are already computed.
-// zero_buf is cleared on the way (see zero_range).
 template <bool CROSS>
 __global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ items,
                                                    const float* __restrict__ yp,
                                                    const float* __restrict__ zp,
-                                                   double* __restrict__ gram,
-                                                   double* __restrict__ zero_buf, int zero_len) {
+                                                   double* __restrict__ gram) {
   __shared__ float sz[kItemRows][kSub + 1];
   __shared__ float sy[CROSS ? kItemRows : 1][kSub + 1];
-  zero_range(zero_buf, zero_len);
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int32_t slot = items[item * 3 + 0];
   const int32_t p0 = items[item * 3 + 1];
@@ -349,34 +341,28 @@ __global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ i
   }
 }
 
-// Cholesky of gram[slot] (upper R, G = R^T R) in LDS, then dst = src R^-1 for
-// the item's rows (solved in f64, stored f32), and (GRAM) the item's share of
-// dst^T dst into gram_out.  A first-pass pivot <= 1e-12 (or non-finite) flags
-// the slot as failed: it equals the residual norm Gram-Schmidt checks in
-// lib.rs:4186-4190.
-template <bool GRAM>
-__global__ __launch_bounds__(256) void iter_orth_kernel(
-    const int32_t* __restrict__ items, const double* __restrict__ gram_in,
-    const float* __restrict__ src, float* __restrict__ dst, double* __restrict__ gram_out,
-    int32_t* __restrict__ fail, double* __restrict__ zero_buf, int zero_len) {
+// Cholesky factor of every slot's Gram matrix (upper R, G = R^T R) and its
+// inverse, one block per slot; gram[slot] is cleared for the next
+// accumulation.  A pivot <= 1e-12 (or non-finite) flags the slot as failed:
+// it equals the residual norm Gram-Schmidt checks in lib.rs:4186-4190.
+__global__ __launch_bounds__(64) void chol_kernel(double* __restrict__ gram,
+                                                  double* __restrict__ rinv,
+                                                  int32_t* __restrict__ fail) {
   __shared__ double G[kSub][kSub];
   __shared__ double R[kSub][kSub + 1];
-  __shared__ float sz[GRAM ? kItemRows : 1][kSub + 1];
+  __shared__ double Ri[kSub][kSub + 1];
   __shared__ int bad;
-  zero_range(zero_buf, zero_len);
-  const int item = xcd_item(blockIdx.x, gridDim.x);
-  const int32_t slot = items[item * 3 + 0];
-  const int32_t p0 = items[item * 3 + 1];
-  const int32_t cnt = items[item * 3 + 2] - p0;
   const int tid = threadIdx.x;
-  const double* g = gram_in + static_cast<int64_t>(slot) * kSub * kSub;
-  for (int q = tid; q < kSub * kSub; q += 256) {
+  double* g = gram + static_cast<int64_t>(blockIdx.x) * kSub * kSub;
+  for (int q = tid; q < kSub * kSub; q += 64) {
     const int i = q / kSub, j = q % kSub;
     G[i][j] = i <= j ? g[i * kSub + j] : g[j * kSub + i];
     R[i][j] = 0.0;
+    Ri[i][j] = 0.0;
   }
   if (tid == 0) bad = 0;
   __syncthreads();
+  for (int q = tid; q < kSub * kSub; q += 64) g[q] = 0.0;
   for (int j = 0; j < kSub; ++j) {
     if (tid == 0) {
       double piv = G[j][j];
@@ -395,36 +381,53 @@ __global__ __launch_bounds__(256) void iter_orth_kernel(
     }
     __syncthreads();
   }
-  if (tid == 0 && bad && fail) fail[slot] = 1;
-  if (tid < cnt) {
-    const int64_t p = p0 + tid;
-    const float* row = src + p * kSub;
-    double z[kSub];
-#pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      double s = row[j];
-#pragma unroll
-      for (int k = 0; k < j; ++k) s -= z[k] * R[k][j];
-      z[j] = s / R[j][j];
-    }
-    float* out = dst + p * kSub;
-#pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      const float f = static_cast<float>(z[j]);
-      out[j] = f;
-      if (GRAM) sz[tid][j] = f;
+  // column c of R^-1 by back substitution (each thread owns one column)
+  if (tid < kSub) {
+    const int c = tid;
+    Ri[c][c] = 1.0 / R[c][c];
+    for (int i = c - 1; i >= 0; --i) {
+      double v = 0.0;
+      for (int k = i + 1; k <= c; ++k) v += R[i][k] * Ri[k][c];
+      Ri[i][c] = -v / R[i][i];
     }
   }
-  if (GRAM) {
-    __syncthreads();
-    if (tid < kSub * (kSub + 1) / 2) {
-      int pi, pj;
-      pair_of(tid, pi, pj);
-      double acc = 0.0;
-      for (int r = 0; r < cnt; ++r)
-        acc = fma(static_cast<double>(sz[r][pi]), static_cast<double>(sz[r][pj]), acc);
-      atomicAdd(&gram_out[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
-    }
+  __syncthreads();
+  double* out = rinv + static_cast<int64_t>(blockIdx.x) * kSub * kSub;
+  for (int q = tid; q < kSub * kSub; q += 64) out[q] = Ri[q / kSub][q % kSub];
+  if (tid == 0 && bad) fail[blockIdx.x] = 1;
+}
+
+// dst = src R^-1 for the item's rows (f64 products, stored f32).  One
+// CholeskyQR pass suffices here: the Gram matrix is formed in f64 from the
+// stored f32 rows, so the factor is accurate to about kappa^2 * 2^-53, far
+// below the 2^-24 of storing the result; a second pass could not take the
+// block's orthogonality past that storage rounding.
+__global__ __launch_bounds__(kItemRows) void rotate_kernel(const int32_t* __restrict__ items,
+                                                           const double* __restrict__ rinv,
+                                                           const float* __restrict__ src,
+                                                           float* __restrict__ dst) {
+  __shared__ double Ri[kSub][kSub];
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int32_t slot = items[item * 3 + 0];
+  const int32_t p0 = items[item * 3 + 1];
+  const int32_t cnt = items[item * 3 + 2] - p0;
+  const int tid = threadIdx.x;
+  const double* r = rinv + static_cast<int64_t>(slot) * kSub * kSub;
+  for (int q = tid; q < kSub * kSub; q += kItemRows) Ri[q / kSub][q % kSub] = r[q];
+  __syncthreads();
+  if (tid >= cnt) return;
+  const int64_t p = p0 + tid;
+  const float* row = src + p * kSub;
+  double x[kSub];
+#pragma unroll
+  for (int k = 0; k < kSub; ++k) x[k] = row[k];
+  float* out = dst + p * kSub;
+#pragma unroll
+  for (int j = 0; j < kSub; ++j) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k <= j; ++k) v = fma(x[k], Ri[k][j], v);
+    out[j] = static_cast<float>(v);
   }
 }
 
@@ -857,7 +860,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     ws.inv.ensure(n);
     ws.y.ensure(static_cast<size_t>(rows) * kSub);
     ws.z.ensure(static_cast<size_t>(rows) * kSub);
-    ws.gram.ensure(static_cast<size_t>(2) * n_slots * kSub * kSub);
+    ws.gram.ensure(static_cast<size_t>(n_slots) * kSub * kSub);
     ws.fail.ensure(n_slots);
     ws.label.ensure(n);  // reused as pos_of
     hipLaunchKernelGGL(degree_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(),
@@ -884,24 +887,23 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     KERNEL_CHECK();
     const int glen = n_slots * kSub * kSub;
     double* g1 = ws.gram.get();
-    double* g2 = ws.gram.get() + glen;
     HIP_CHECK(hipMemsetAsync(ws.fail.get(), 0, 4 * n_slots, s));
-    HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * 2 * glen, s));
+    HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * glen, s));
     float* yb = ws.y.get();
     float* zb = ws.z.get();
     const int32_t* it_items = ws.items.get();
-    // Orthonormalise Z into Y (CholeskyQR2): g1 holds Z^T Z on entry.
+    ws.rmat.ensure(static_cast<size_t>(glen));
+    // Orthonormalise Z into Y (CholeskyQR): g1 holds Z^T Z on entry and is
+    // cleared for the next accumulation.
     auto orth_into_y = [&]() {
-      hipLaunchKernelGGL(iter_orth_kernel<true>, dim3(n_items), dim3(256), 0, s, it_items, g1, zb, zb,
-                         g2, ws.fail.get(), static_cast<double*>(nullptr), 0);
-      hipLaunchKernelGGL(iter_orth_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, g2, zb, yb,
-                         static_cast<double*>(nullptr), static_cast<int32_t*>(nullptr), g1, glen);
+      hipLaunchKernelGGL(chol_kernel, dim3(n_slots), dim3(64), 0, s, g1, ws.rmat.get(), ws.fail.get());
+      hipLaunchKernelGGL(rotate_kernel, dim3(n_items), dim3(kItemRows), 0, s, it_items, ws.rmat.get(),
+                         zb, yb);
       KERNEL_CHECK();
     };
     // start block: Z = Y, g1 = Y^T Y
     HIP_CHECK(hipMemcpyAsync(zb, yb, sizeof(float) * rows * kSub, hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1, g2,
-                       glen);
+    hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
     KERNEL_CHECK();
     orth_into_y();
     // pieces of at most kSeg entries in row order; rows longer than kSeg
@@ -935,8 +937,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     for (int it = 0; it < kIters; ++it) {
       product(yb, zb);
       if ((it + 1) % kOrthEvery == 0 || it + 1 == kIters) {
-        hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1,
-                           g2, glen);
+        hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
         KERNEL_CHECK();
         orth_into_y();
       } else {
@@ -945,8 +946,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     }
     // Ritz matrix Y^T (I + N) Y, upper triangle
     product(yb, zb);
-    hipLaunchKernelGGL(gram_kernel<true>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1,
-                       static_cast<double*>(nullptr), 0);
+    hipLaunchKernelGGL(gram_kernel<true>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
     KERNEL_CHECK();
     std::vector<double> ritz(static_cast<size_t>(n_slots) * kSub * kSub);
     std::vector<int32_t> fail(n_slots);
