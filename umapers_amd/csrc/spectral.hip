@@ -170,108 +170,105 @@ __device__ __forceinline__ void pair_of(int tid, int& pi, int& pj) {
   pj = i + t;
 }
 
-// z = y + D^-1/2 W D^-1/2 y.  Work is cut into pieces of at most kSeg
-// entries so that hub rows (hundreds of entries after the union) do not hold
-// a wave for their whole length: a row with at most kSeg entries is one
-// piece that writes its z row directly; a longer row is split into kSeg-entry
-// pieces whose partial sums go to a scratch table (slot = piece index), and a
-// second kernel adds each long row's partials in piece order, so the result
-// does not depend on scheduling.  Every piece takes 9 lanes (lane j carries
-// columns 2j, 2j+1).
+// z = y + D^-1/2 W D^-1/2 y, one launch per product.  A wave holds 7 lane
+// groups (lanes 9g .. 9g+8; lane 63 idles), lane j of a group carrying
+// columns 2j, 2j+1.  Rows of at most kSeg entries are packed 7 to a wave, one
+// per group.  A longer row (hub rows run to hundreds of entries after the
+// union) takes a whole wave: its kSeg-entry pieces are dealt to the groups
+// in turn, and the groups' partial sums are added in group order through
+// LDS, so the result does not depend on scheduling.  For every piece the
+// group first stages its column indices and weights in LDS with coalesced
+// loads, then walks them (LDS broadcast reads) with 8 neighbour-row gathers
+// in flight per lane: the vector memory pipeline is left to the neighbour
+// rows, which bound the kernel.
 //
 // The block is stored in f32 (72 B a row): the product is a gather of one
 // neighbour row per graph entry, so the row width sets the traffic.  The
 // normalised weights are formed in f64 and rounded once; the Gram matrices
 // and the Cholesky factors below stay in f64.
 constexpr int kSeg = 64;
-
-// A wave takes 7 pieces (lanes 9g .. 9g+8 for piece g; lane 63 idles).  The
-// piece's column indices and weights are first staged in LDS by coalesced
-// loads of its 9 lanes; each lane then walks the staged entries (LDS
-// broadcast reads) and gathers its two columns of every neighbour row, 8
-// gathers in flight.  Staging keeps the vector memory pipeline for the
-// neighbour rows, which bound the kernel.
-constexpr int kPiecesPerWave = 7;
+constexpr int kGroups = 7;
 constexpr int kSpmmWaves = 4;
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Waves 0 .. n_long-1 take the long rows, the rest 7 short rows each.
 __global__ __launch_bounds__(64 * kSpmmWaves) void spmm_kernel(
-    int64_t n_pieces, const int32_t* __restrict__ piece_row,
-    const int32_t* __restrict__ piece_start, const int32_t* __restrict__ prow_ptr,
+    int64_t n_long, int64_t n_short, const int32_t* __restrict__ long_list,
+    const int32_t* __restrict__ short_list, const int32_t* __restrict__ prow_ptr,
     const int32_t* __restrict__ pcol, const float* __restrict__ pnw,
-    const float* __restrict__ yp, float* __restrict__ zp, float2* __restrict__ part) {
-  __shared__ int2 staged[kSpmmWaves][kPiecesPerWave][kSeg];
+    const float* __restrict__ yp, float* __restrict__ zp) {
+  __shared__ int2 staged[kSpmmWaves][kGroups][kSeg];
+  __shared__ float2 partial[kSpmmWaves][kGroups][9];
   const int wv = threadIdx.x >> 6;
   const int wl = threadIdx.x & 63;
   const int g = wl / 9;
   const int lj = wl - 9 * g;
-  const int64_t i = (static_cast<int64_t>(blockIdx.x) * kSpmmWaves + wv) * kPiecesPerWave + g;
-  const bool valid = g < kPiecesPerWave && i < n_pieces;
-  int32_t p = 0, e0 = 0, cnt = 0;
-  bool whole = true;
-  if (valid) {
-    p = piece_row[i];
-    e0 = piece_start[i];
-    const int32_t r0 = prow_ptr[p], r1 = prow_ptr[p + 1];
-    whole = r1 - r0 <= kSeg;
-    cnt = whole ? r1 - e0 : min(kSeg, r1 - e0);
-    for (int k = lj; k < cnt; k += 9)
-      staged[wv][g][k] = make_int2(pcol[e0 + k], __float_as_int(pnw[e0 + k]));
-  }
-  // the entries are written and read by lanes of one wave
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (!valid) return;
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * kSpmmWaves + wv;
   const float2* y2 = reinterpret_cast<const float2*>(yp);
-  const int2* st = staged[wv][g];
-  float2 acc = whole ? y2[static_cast<int64_t>(p) * (kSub / 2) + lj] : make_float2(0.f, 0.f);
-  int k = 0;
-  for (; k + 8 <= cnt; k += 8) {
-    int2 c[8];
-    float2 q[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) c[j] = st[k + j];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) q[j] = y2[static_cast<int64_t>(c[j].x) * (kSub / 2) + lj];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float w = __int_as_float(c[j].y);
-      acc.x = fmaf(w, q[j].x, acc.x);
-      acc.y = fmaf(w, q[j].y, acc.y);
-    }
-  }
-  for (; k < cnt; ++k) {
-    const int2 c = st[k];
-    const float2 q = y2[static_cast<int64_t>(c.x) * (kSub / 2) + lj];
-    const float w = __int_as_float(c.y);
-    acc.x = fmaf(w, q.x, acc.x);
-    acc.y = fmaf(w, q.y, acc.y);
-  }
-  if (whole)
-    reinterpret_cast<float2*>(zp)[static_cast<int64_t>(p) * (kSub / 2) + lj] = acc;
-  else
-    part[i * (kSub / 2) + lj] = acc;
-}
+  float2* z2 = reinterpret_cast<float2*>(zp);
+  int2* st = staged[wv][g < kGroups ? g : 0];
 
-// z row of every long row: its own y row plus its pieces' partials in order.
-__global__ void spmm_combine_kernel(int64_t n_long, const int32_t* __restrict__ long_list,
-                                    const int32_t* __restrict__ prow_ptr,
-                                    const int32_t* __restrict__ piece_off,
-                                    const float2* __restrict__ part,
-                                    const float* __restrict__ yp, float* __restrict__ zp) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t i = gid / 9;
-  const int lj = static_cast<int>(gid % 9);
-  if (i >= n_long) return;
-  const int64_t p = long_list[i];
-  const int32_t np = (prow_ptr[p + 1] - prow_ptr[p] + kSeg - 1) / kSeg;
-  float2 acc = reinterpret_cast<const float2*>(yp)[p * (kSub / 2) + lj];
-  for (int32_t q = piece_off[p]; q < piece_off[p] + np; ++q) {
-    const float2 v = part[static_cast<int64_t>(q) * (kSub / 2) + lj];
-    acc.x += v.x;
-    acc.y += v.y;
+  // sum of w * y[col] over entries [e0, e0 + cnt) of one piece (cnt <= kSeg)
+  auto piece = [&](int32_t e0, int32_t cnt, float2 acc) {
+    for (int k = lj; k < cnt; k += 9) st[k] = make_int2(pcol[e0 + k], __float_as_int(pnw[e0 + k]));
+    wave_sync();
+    int k = 0;
+    for (; k + 8 <= cnt; k += 8) {
+      int2 c[8];
+      float2 q[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) c[j] = st[k + j];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) q[j] = y2[static_cast<int64_t>(c[j].x) * (kSub / 2) + lj];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float wt = __int_as_float(c[j].y);
+        acc.x = fmaf(wt, q[j].x, acc.x);
+        acc.y = fmaf(wt, q[j].y, acc.y);
+      }
+    }
+    for (; k < cnt; ++k) {
+      const int2 c = st[k];
+      const float2 q = y2[static_cast<int64_t>(c.x) * (kSub / 2) + lj];
+      const float wt = __int_as_float(c.y);
+      acc.x = fmaf(wt, q.x, acc.x);
+      acc.y = fmaf(wt, q.y, acc.y);
+    }
+    wave_sync();  // the next piece overwrites the staged entries
+    return acc;
+  };
+
+  if (w < n_long) {
+    const int32_t p = long_list[w];
+    const int32_t r0 = prow_ptr[p], r1 = prow_ptr[p + 1];
+    float2 acc = make_float2(0.f, 0.f);
+    if (g < kGroups)
+      for (int32_t e0 = r0 + g * kSeg; e0 < r1; e0 += kGroups * kSeg)
+        acc = piece(e0, min(kSeg, r1 - e0), acc);
+    if (g < kGroups) partial[wv][g][lj] = acc;
+    wave_sync();
+    if (g == 0) {
+      float2 sum = y2[static_cast<int64_t>(p) * (kSub / 2) + lj];
+#pragma unroll
+      for (int q = 0; q < kGroups; ++q) {
+        sum.x += partial[wv][q][lj].x;
+        sum.y += partial[wv][q][lj].y;
+      }
+      z2[static_cast<int64_t>(p) * (kSub / 2) + lj] = sum;
+    }
+  } else {
+    const int64_t i = (w - n_long) * kGroups + g;
+    if (g >= kGroups || i >= n_short) return;
+    const int32_t p = short_list[i];
+    const int32_t r0 = prow_ptr[p], r1 = prow_ptr[p + 1];
+    const float2 acc = piece(r0, r1 - r0, y2[static_cast<int64_t>(p) * (kSub / 2) + lj]);
+    z2[static_cast<int64_t>(p) * (kSub / 2) + lj] = acc;
   }
-  reinterpret_cast<float2*>(zp)[p * (kSub / 2) + lj] = acc;
 }
 
 __global__ void long_rows_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
@@ -281,29 +278,16 @@ __global__ void long_rows_kernel(int64_t rows, const int32_t* __restrict__ prow_
     list[atomicAdd(count, 1)] = static_cast<int32_t>(p);
 }
 
-__global__ void piece_count_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
-                                   int32_t* __restrict__ cnt) {
+__global__ void short_flag_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
+                                  int32_t* __restrict__ flag) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= rows) return;
-  const int32_t len = prow_ptr[p + 1] - prow_ptr[p];
-  cnt[p] = len <= kSeg ? 1 : (len + kSeg - 1) / kSeg;
+  if (p < rows) flag[p] = prow_ptr[p + 1] - prow_ptr[p] <= kSeg ? 1 : 0;
 }
 
-__global__ void piece_fill_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
-                                  const int32_t* __restrict__ piece_off,
-                                  int32_t* __restrict__ piece_row,
-                                  int32_t* __restrict__ piece_start) {
+__global__ void short_fill_kernel(int64_t rows, const int32_t* __restrict__ flag,
+                                  const int32_t* __restrict__ off, int32_t* __restrict__ list) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= rows) return;
-  int32_t q = piece_off[p];
-  const int32_t r0 = prow_ptr[p], r1 = prow_ptr[p + 1];
-  piece_row[q] = static_cast<int32_t>(p);
-  piece_start[q] = r0;
-  for (int32_t e = r0 + kSeg; e < r1; e += kSeg) {
-    ++q;
-    piece_row[q] = static_cast<int32_t>(p);
-    piece_start[q] = e;
-  }
+  if (p < rows && flag[p]) list[off[p]] = static_cast<int32_t>(p);
 }
 
 // The item's share of Z^T Z (or Y^T Z when CROSS) added to gram[slot], with
@@ -906,32 +890,28 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
     KERNEL_CHECK();
     orth_into_y();
-    // pieces of at most kSeg entries in row order; rows longer than kSeg
-    DevBuf<int32_t> long_list(rows), long_count(1), piece_cnt(rows), piece_off(rows);
+    // rows of more than kSeg entries (one wave each) and the others in order
+    DevBuf<int32_t> long_list(rows), long_count(1), short_flag(rows), short_off(rows),
+        short_list(rows);
     HIP_CHECK(hipMemsetAsync(long_count.get(), 0, 4, s));
     hipLaunchKernelGGL(long_rows_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
                        prow_ptr.get(), long_list.get(), long_count.get());
-    hipLaunchKernelGGL(piece_count_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
-                       prow_ptr.get(), piece_cnt.get());
+    hipLaunchKernelGGL(short_flag_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                       prow_ptr.get(), short_flag.get());
     KERNEL_CHECK();
-    const int64_t n_pieces = exclusive_scan(piece_cnt.get(), piece_off.get(), rows, scan_ws, s);
-    DevBuf<int32_t> piece_row(n_pieces), piece_start(n_pieces);
-    DevBuf<float2> part(static_cast<size_t>(n_pieces) * (kSub / 2));
-    hipLaunchKernelGGL(piece_fill_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
-                       prow_ptr.get(), piece_off.get(), piece_row.get(), piece_start.get());
+    const int64_t n_short = exclusive_scan(short_flag.get(), short_off.get(), rows, scan_ws, s);
+    hipLaunchKernelGGL(short_fill_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                       short_flag.get(), short_off.get(), short_list.get());
     KERNEL_CHECK();
     int32_t n_long = 0;
     HIP_CHECK(hipMemcpyAsync(&n_long, long_count.get(), 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t spmm_waves = n_long + (n_short + kGroups - 1) / kGroups;
     auto product = [&](const float* src, float* dst) {
-      hipLaunchKernelGGL(spmm_kernel, dim3(blocks_for(n_pieces, kSpmmWaves * kPiecesPerWave)),
-                         dim3(64 * kSpmmWaves), 0, s,
-                         n_pieces, piece_row.get(), piece_start.get(), prow_ptr.get(), pcol.get(),
-                         pnw.get(), src, dst, part.get());
-      if (n_long > 0)
-        hipLaunchKernelGGL(spmm_combine_kernel, dim3(blocks_for(static_cast<int64_t>(n_long) * 9, 256)),
-                           dim3(256), 0, s, static_cast<int64_t>(n_long), long_list.get(),
-                           prow_ptr.get(), piece_off.get(), part.get(), src, dst);
+      hipLaunchKernelGGL(spmm_kernel, dim3(blocks_for(spmm_waves, kSpmmWaves)),
+                         dim3(64 * kSpmmWaves), 0, s, static_cast<int64_t>(n_long), n_short,
+                         long_list.get(), short_list.get(), prow_ptr.get(), pcol.get(), pnw.get(),
+                         src, dst);
       KERNEL_CHECK();
     };
     for (int it = 0; it < kIters; ++it) {
