@@ -1,0 +1,86 @@
+"""Writes the golden fixtures of this directory from the oracle (the C++
+restatement of the reference's CPU path under oracle/).
+
+    python tests/golden/make_golden.py          # all fixtures (a few minutes)
+
+Fixtures (all small):
+  c1_stages.npz  config-1 (1000 x 32 i.i.d. N(0,1), seed 42) stage outputs:
+                 kNN (k=15) indices/distances, sigma/rho, the pruned fuzzy
+                 graph for 200 epochs (heads, tails, weights), and the per-edge
+                 positive / negative sample counts of 200 epochs
+  inputs.json    SHA-256 of the synthetic inputs of configs 1-3 and of the
+                 100,000-row fit set of config 5
+  quality.json   the oracle's trustworthiness@15 and kNN preservation at
+                 configs 1 and 2 over fixed seeds (the statistical parity
+                 targets of the GPU fit tests)
+
+The reference (Rust) is not run: it holds no array fixtures of its own for
+this path, and its toolchain is not available here.  Each fixture is checked
+independently by tests/test_golden.py (FP64 brute force, closed forms and
+invariants), so a fixture that the oracle wrote wrongly fails there.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+from umapers_amd import datasets  # noqa: E402
+from umapers_amd.diagnostics import knn_preservation, trustworthiness  # noqa: E402
+
+C1_SEEDS = list(range(10))
+C2_SEEDS = [42, 7, 11, 23]
+
+
+def stages() -> None:
+    x = datasets.c1()
+    idx, dist = O.knn(x, 15)
+    sig, rho = O.smooth_knn_dist(dist, 15.0)
+    h, t, w = O.fuzzy_graph(idx, dist, sig, rho, 1.0, 200)
+    pos, neg, _, _ = O.schedule_counts(w, 200, 5)
+    np.savez_compressed(os.path.join(HERE, "c1_stages.npz"), knn_idx=idx.astype(np.int32),
+                        knn_dist=dist, sigma=sig, rho=rho, heads=h.astype(np.int32),
+                        tails=t.astype(np.int32), weights=w, pos_counts=pos.astype(np.int32),
+                        neg_counts=neg.astype(np.int32))
+
+
+def inputs() -> None:
+    out = {"c1": datasets.sha256(datasets.c1()), "c2": datasets.sha256(datasets.c2()),
+           "c3": datasets.sha256(datasets.c3()),
+           "c5_fit": datasets.sha256(datasets.c5(n_fit=100_000, n_query=1)[0])}
+    with open(os.path.join(HERE, "inputs.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+def quality() -> None:
+    res = {}
+    x = datasets.c1()
+    t, k = [], []
+    for s in C1_SEEDS:
+        y = O.fit(x, O.OracleParams(n_epochs=200, random_seed=s)).embedding
+        t.append(trustworthiness(x, y))
+        k.append(knn_preservation(x, y))
+    res["c1_plain_200"] = {"seeds": C1_SEEDS, "trust": t, "knn_preservation": k}
+    x = datasets.c2()
+    t, k = [], []
+    for s in C2_SEEDS:
+        y = O.fit(x, O.OracleParams(n_epochs=500, random_seed=s, densmap=True)).embedding
+        t.append(trustworthiness(x, y))
+        k.append(knn_preservation(x, y))
+        print("c2 densmap seed", s, t[-1], flush=True)
+    res["c2_densmap_500"] = {"seeds": C2_SEEDS, "trust": t, "knn_preservation": k}
+    with open(os.path.join(HERE, "quality.json"), "w") as f:
+        json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    stages()
+    inputs()
+    quality()

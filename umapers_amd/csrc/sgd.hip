@@ -501,7 +501,7 @@ __global__ __launch_bounds__(128) void transform_sgd_kernel(
 }
 
 // Edge ends per vertex (as head plus as tail), then the hub move scale.
-constexpr int kHubEnds = 64;
+constexpr int kHubEnds = 256;
 
 __global__ void edge_ends_kernel(const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
                                  int64_t ne, int32_t* __restrict__ ends) {
