@@ -414,24 +414,30 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
 
   Frags cur;
   load(0, cur);
-  bool drain = false;
-  for (int tile = 0; tile < n_tiles; ++tile) {
+  // The tile loop runs until some wave of the block needs its buffers drained;
+  // the drain sits outside it, so the loop itself never modifies the top
+  // lists and keeps one register assignment.
+  int tile = 0;
+  while (tile < n_tiles) {
+    bool drain = false;
+    for (; tile < n_tiles && !drain; ++tile) {
+      const floatx16 acc = mma(tile, cur);
+      // the next tile's fragments reuse the registers the MFMAs just consumed
+      __builtin_amdgcn_sched_barrier(0);
+      if (tile + 1 < n_tiles) load(tile + 1, cur);
+      // slot (tile + 2) % 3 last held tile - 1, whose reads all ended before
+      // the barrier that closed the previous iteration
+      stage(tile + 2, pre);
+      fetch(tile + 3, pre);
+      filter(acc, static_cast<int32_t>(r0 + static_cast<int64_t>(tile) * kTile + 4 * half));
+      const bool need = __ballot(cnt > BUF - 16) != 0ULL;
+      if (lane == 0) s_flag[(tile & 1) * 8 + wave] = need ? 1 : 0;
+      __syncthreads();
+      // block-uniform; readfirstlane lets the compiler see it as uniform
+      drain = __builtin_amdgcn_readfirstlane(
+                  *reinterpret_cast<const uint64_t*>(s_flag + (tile & 1) * 8) != 0ULL ? 1 : 0) != 0;
+    }
     if (drain) flush(std::true_type{});
-    const floatx16 acc = mma(tile, cur);
-    // the next tile's fragments reuse the registers the MFMAs just consumed
-    __builtin_amdgcn_sched_barrier(0);
-    if (tile + 1 < n_tiles) load(tile + 1, cur);
-    // slot (tile + 2) % 3 last held tile - 1, whose reads all ended before
-    // the barrier that closed the previous iteration
-    stage(tile + 2, pre);
-    fetch(tile + 3, pre);
-    filter(acc, static_cast<int32_t>(r0 + static_cast<int64_t>(tile) * kTile + 4 * half));
-    const bool need = __ballot(cnt > BUF - 16) != 0ULL;
-    if (lane == 0) s_flag[(tile & 1) * 8 + wave] = need ? 1 : 0;
-    __syncthreads();
-    // block-uniform; readfirstlane lets the compiler see it as uniform
-    drain = __builtin_amdgcn_readfirstlane(
-                *reinterpret_cast<const uint64_t*>(s_flag + (tile & 1) * 8) != 0ULL ? 1 : 0) != 0;
   }
   flush(std::false_type{});
   if (!live) return;
