@@ -199,34 +199,41 @@ struct LaneTop {
 
 constexpr int kLdsBudget = 160 * 1024;
 constexpr int kTile = 32;   // reference rows per staged tile (one MFMA subtile)
-constexpr int kSlots = 3;   // tile ring: compute t, t+1 visible, t+2 being staged
 
-// LDS of one block: a ring of three reference tiles with their half norms,
-// two generations of per-wave flush flags, and the waves' pass buffers
-// ([slot][lane] pairs of 8 bytes).  Eight waves (two per SIMD, at most 256
-// registers each) when the buffers can stay at least 20 deep and the
-// resident query fragments plus the top list leave room, else four.  The
-// next tile's fragments are read into registers ahead of use when they are
-// at most eight 16-byte vectors per lane.
+// LDS of one block: a ring of reference tiles with their half norms and the
+// waves' pass buffers ([slot][lane] pairs of 8 bytes).  Eight waves (two per
+// SIMD, at most 256 registers each) when the buffers can stay at least 20
+// deep in the three-tile ring and the resident query fragments plus the top
+// list leave room, else four.  The next tile's fragments are read into
+// registers ahead of use when they are at most eight 16-byte vectors per
+// lane.  kSuper tiles share one block-wide barrier: the ring holds two groups
+// of kSuper tiles (one being multiplied, the next being staged), the largest
+// of 4 and 2 that fits; when neither fits, the ring holds three tiles with a
+// barrier per tile and block-synchronous drains (kSuper = 1).
 template <typename T, int DPAD, int H>
 struct CandShape {
   static constexpr int kNV = DPAD / 2 / (16 / static_cast<int>(sizeof(T)));
   static constexpr bool kPrefetch = kNV <= 8;
   static constexpr int kRowBytes = DPAD * static_cast<int>(sizeof(T)) + 16;
   static constexpr int kStageBytes = kTile * kRowBytes + kTile * 4;
-  static constexpr int kTiles = kSlots * kStageBytes;
   static constexpr int kFlagBytes = 16;
-  static constexpr int bytes(int waves, int buf) { return kTiles + kFlagBytes + waves * 64 * buf * 8; }
-  static constexpr int pick_buf(int waves) {
+  static constexpr int ring(int super) { return super == 1 ? 3 : 2 * super; }
+  static constexpr int bytes(int waves, int super, int buf) {
+    return ring(super) * kStageBytes + (super == 1 ? kFlagBytes : 0) + waves * 64 * buf * 8;
+  }
+  static constexpr int pick_buf(int waves, int super) {
     for (int b : {32, 28, 24, 20})
-      if (bytes(waves, b) <= kLdsBudget) return b;
+      if (bytes(waves, super, b) <= kLdsBudget) return b;
     return 0;
   }
   static constexpr bool kRegsFit8 = H <= 16 && (kPrefetch || 4 * kNV + 2 * H <= 120);
-  static constexpr int kWaves = (pick_buf(8) && kRegsFit8) ? 8 : 4;
-  static constexpr int kBuf = pick_buf(kWaves);
+  static constexpr int kWaves = (pick_buf(8, 1) && kRegsFit8) ? 8 : 4;
+  static constexpr int kSuper = pick_buf(kWaves, 4) ? 4 : (pick_buf(kWaves, 2) ? 2 : 1);
+  static constexpr int kRing = ring(kSuper);
+  static constexpr int kTiles = kRing * kStageBytes;
+  static constexpr int kBuf = pick_buf(kWaves, kSuper);
   static_assert(kBuf >= 20, "kNN candidate tiles do not fit in LDS");
-  static constexpr int kBytes = bytes(kWaves, kBuf);
+  static constexpr int kBytes = bytes(kWaves, kSuper, kBuf);
   static constexpr int kQueries = kWaves * 32;  // queries per block
 };
 
@@ -235,12 +242,17 @@ struct CandShape {
 // block full_groups + t takes group full_groups + t / tail_split against rows
 // [s * tail_len, (s + 1) * tail_len), s = t % tail_split, and leaves
 // (score, index) lists for knn_merge_splits_kernel.  The loop is a
-// pipeline over 32-row tiles with one barrier per tile: tile t + 2 is staged
-// while tile t is multiplied, and the fragments of tile t + 1 are read from
-// LDS into the registers tile t's MFMAs have just consumed (when they fit)
-// before tile t's scores are filtered.  A wave whose buffers could not take
-// another tile raises a flag; after the barrier every wave of the block
-// drains its buffers, so no wave idles at the barrier while another drains.
+// pipeline over 32-row tiles: tile t + kSuper is staged while tile t is
+// multiplied, one block-wide barrier closes every group of kSuper tiles, and
+// the fragments of tile t + 1 are read from LDS into the registers tile t's
+// MFMAs have just consumed (when they fit) before tile t's scores are
+// filtered.  A wave whose buffers could not take another tile drains them
+// between tiles on its own.  With kSuper = 1 the ring holds three tiles
+// (tile t + 2 staged during tile t) and a wave that needs a drain raises a
+// flag; after the per-tile barrier every wave of the block drains, so no wave
+// idles at the barrier while another drains.  Either way the drain sits
+// outside the tile loop, so the loop itself never modifies the top lists and
+// keeps one register assignment.
 template <typename T, int DPAD, int H>
 __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_candidates_kernel(
     const T* __restrict__ xq, int64_t m, const T* __restrict__ xr,
@@ -256,7 +268,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   constexpr bool kPrefetch = C::kPrefetch;
   extern __shared__ __align__(16) unsigned char lds[];
   unsigned char* s_tiles = lds;
-  uint8_t* s_flag = lds + C::kTiles;
+  uint8_t* s_flag = lds + C::kTiles;  // kSuper == 1 only
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -272,7 +284,8 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   const int64_t r0 = split_block ? static_cast<int64_t>(split) * tail_len : 0;
   const int64_t r1 = split_block ? (r0 + tail_len < n ? r0 + tail_len : n) : n;
   const int n_tiles = static_cast<int>((r1 - r0 + kTile - 1) / kTile);
-  uint2* wbuf = reinterpret_cast<uint2*>(lds + C::kTiles + C::kFlagBytes) + wave * 64 * BUF + lane;
+  uint2* wbuf = reinterpret_cast<uint2*>(lds + C::kTiles + (C::kSuper == 1 ? C::kFlagBytes : 0)) +
+                wave * 64 * BUF + lane;
 
   u32x4 qf[NV];
   {
@@ -303,7 +316,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
     if (tid < kTile && tile < n_tiles && base + tid < r1) p.nr = nr_half[base + tid];
   };
   auto stage = [&](int tile, const Pre& p) {
-    unsigned char* t = s_tiles + (tile % kSlots) * C::kStageBytes;
+    unsigned char* t = s_tiles + (tile % C::kRing) * C::kStageBytes;
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i) {
       const int c = tid + i * kThreads;
@@ -318,7 +331,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
     floatx16 seed;
   };
   auto load = [&](int tile, Frags& f) {
-    const unsigned char* st = s_tiles + (tile % kSlots) * C::kStageBytes;
+    const unsigned char* st = s_tiles + (tile % C::kRing) * C::kStageBytes;
     if constexpr (kPrefetch) {
       const unsigned char* rrow = st + l32 * C::kRowBytes + half * HALF * sizeof(T);
 #pragma unroll
@@ -340,7 +353,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
 #pragma unroll
       for (int v = 0; v < NV; ++v) acc = MfmaSteps<T, MfmaOp<T>::kSteps>::run(f.rf[v], qf[v], acc);
     } else {
-      const unsigned char* rrow = s_tiles + (tile % kSlots) * C::kStageBytes + l32 * C::kRowBytes +
+      const unsigned char* rrow = s_tiles + (tile % C::kRing) * C::kStageBytes + l32 * C::kRowBytes +
                                   half * HALF * sizeof(T);
 #pragma unroll
       for (int v = 0; v < NV; ++v)
@@ -404,40 +417,70 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   };
 
   Pre pre;
-  fetch(0, pre);
-  stage(0, pre);
-  fetch(1, pre);
-  stage(1, pre);
-  fetch(2, pre);
-  if (tid < C::kFlagBytes) s_flag[tid] = 0;
-  __syncthreads();
-
   Frags cur;
-  load(0, cur);
-  // The tile loop runs until some wave of the block needs its buffers drained;
-  // the drain sits outside it, so the loop itself never modifies the top
-  // lists and keeps one register assignment.
   int tile = 0;
-  while (tile < n_tiles) {
-    bool drain = false;
-    for (; tile < n_tiles && !drain; ++tile) {
-      const floatx16 acc = mma(tile, cur);
-      // the next tile's fragments reuse the registers the MFMAs just consumed
-      __builtin_amdgcn_sched_barrier(0);
-      if (tile + 1 < n_tiles) load(tile + 1, cur);
-      // slot (tile + 2) % 3 last held tile - 1, whose reads all ended before
-      // the barrier that closed the previous iteration
-      stage(tile + 2, pre);
-      fetch(tile + 3, pre);
-      filter(acc, static_cast<int32_t>(r0 + static_cast<int64_t>(tile) * kTile + 4 * half));
-      const bool need = __ballot(cnt > BUF - 16) != 0ULL;
-      if (lane == 0) s_flag[(tile & 1) * 8 + wave] = need ? 1 : 0;
-      __syncthreads();
-      // block-uniform; readfirstlane lets the compiler see it as uniform
-      drain = __builtin_amdgcn_readfirstlane(
-                  *reinterpret_cast<const uint64_t*>(s_flag + (tile & 1) * 8) != 0ULL ? 1 : 0) != 0;
+  if constexpr (C::kSuper == 1) {
+    fetch(0, pre);
+    stage(0, pre);
+    fetch(1, pre);
+    stage(1, pre);
+    fetch(2, pre);
+    if (tid < C::kFlagBytes) s_flag[tid] = 0;
+    __syncthreads();
+    load(0, cur);
+    while (tile < n_tiles) {
+      bool drain = false;
+      for (; tile < n_tiles && !drain; ++tile) {
+        const floatx16 acc = mma(tile, cur);
+        // the next tile's fragments reuse the registers the MFMAs just consumed
+        __builtin_amdgcn_sched_barrier(0);
+        if (tile + 1 < n_tiles) load(tile + 1, cur);
+        // slot (tile + 2) % 3 last held tile - 1, whose reads all ended before
+        // the barrier that closed the previous iteration
+        stage(tile + 2, pre);
+        fetch(tile + 3, pre);
+        filter(acc, static_cast<int32_t>(r0 + static_cast<int64_t>(tile) * kTile + 4 * half));
+        const bool need = __ballot(cnt > BUF - 16) != 0ULL;
+        if (lane == 0) s_flag[(tile & 1) * 8 + wave] = need ? 1 : 0;
+        __syncthreads();
+        // block-uniform; readfirstlane lets the compiler see it as uniform
+        drain = __builtin_amdgcn_readfirstlane(
+                    *reinterpret_cast<const uint64_t*>(s_flag + (tile & 1) * 8) != 0ULL ? 1 : 0) != 0;
+      }
+      if (drain) flush(std::true_type{});
     }
-    if (drain) flush(std::true_type{});
+  } else {
+    constexpr int S = C::kSuper;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      fetch(t, pre);
+      stage(t, pre);
+    }
+    fetch(S, pre);
+    __syncthreads();
+    load(0, cur);
+    while (tile < n_tiles) {
+      bool need = false;
+      for (; tile < n_tiles && !need; ++tile) {
+        const floatx16 acc = mma(tile, cur);
+        __builtin_amdgcn_sched_barrier(0);
+        const bool last = tile % S == S - 1;
+        // within a group the next tile is already visible; across groups it
+        // becomes visible at the barrier below
+        if (!last && tile + 1 < n_tiles) load(tile + 1, cur);
+        // slot (tile + S) % 2S last held tile - S, whose fragments were all
+        // read before the barrier that closed the previous group
+        stage(tile + S, pre);
+        fetch(tile + S + 1, pre);
+        if (last) {
+          __syncthreads();
+          if (tile + 1 < n_tiles) load(tile + 1, cur);
+        }
+        filter(acc, static_cast<int32_t>(r0 + static_cast<int64_t>(tile) * kTile + 4 * half));
+        need = __ballot(cnt > BUF - 16) != 0ULL;
+      }
+      if (need) flush(std::true_type{});
+    }
   }
   flush(std::false_type{});
   if (!live) return;
