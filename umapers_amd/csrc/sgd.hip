@@ -511,6 +511,12 @@ __global__ void edge_ends_kernel(const int32_t* __restrict__ head, const int32_t
   atomicAdd(&ends[tail[e]], 1);
 }
 
+__global__ void row_ends_kernel(const int32_t* __restrict__ row_ptr, int64_t n,
+                                int32_t* __restrict__ ends) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) ends[v] = 2 * (row_ptr[v + 1] - row_ptr[v]);
+}
+
 __global__ void hub_scale_kernel(const int32_t* __restrict__ ends, int64_t n,
                                  float* __restrict__ mscale, int32_t* __restrict__ hubs) {
   const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -675,8 +681,12 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
   ws.ends.ensure(n + 1);
   ws.mscale.ensure(n);
   HIP_CHECK(hipMemsetAsync(ws.ends.get(), 0, sizeof(int32_t) * (n + 1), s));
-  hipLaunchKernelGGL(edge_ends_kernel, dim3(blocks), dim3(256), 0, s, g.head.get(), g.tail.get(),
-                     ne, ws.ends.get());
+  if (args.symmetric)  // every edge has its reverse: ends = 2 * row length
+    hipLaunchKernelGGL(row_ends_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(),
+                       n, ws.ends.get());
+  else
+    hipLaunchKernelGGL(edge_ends_kernel, dim3(blocks), dim3(256), 0, s, g.head.get(), g.tail.get(),
+                       ne, ws.ends.get());
   hipLaunchKernelGGL(hub_scale_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, ws.ends.get(), n,
                      ws.mscale.get(), ws.ends.get() + n);
   KERNEL_CHECK();

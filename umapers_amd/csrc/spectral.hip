@@ -61,14 +61,19 @@ __global__ void init_labels_kernel(int32_t* label, int64_t n) {
 __global__ void hook_kernel(const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
                             int64_t ne, int32_t* __restrict__ label, int32_t* __restrict__ changed) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const int32_t lu = label[head[e]];
-  const int32_t lv = label[tail[e]];
-  if (lu == lv) return;
-  const int32_t hi = lu > lv ? lu : lv;
-  const int32_t lo = lu > lv ? lv : lu;
-  atomicMin(&label[hi], lo);
-  *changed = 1;
+  bool hooked = false;
+  if (e < ne) {
+    const int32_t lu = label[head[e]];
+    const int32_t lv = label[tail[e]];
+    if (lu != lv) {
+      const int32_t hi = lu > lv ? lu : lv;
+      const int32_t lo = lu > lv ? lv : lu;
+      atomicMin(&label[hi], lo);
+      hooked = true;
+    }
+  }
+  // one store per wave, not one per hooked edge
+  if (__ballot(hooked) != 0ULL && (threadIdx.x & 63) == 0) *changed = 1;
 }
 
 __global__ void jump_kernel(int32_t* __restrict__ label, int64_t n) {
@@ -114,22 +119,28 @@ __global__ void prow_count_kernel(const int32_t* __restrict__ perm, int64_t rows
 
 // Normalised weights w_ij / sqrt(d_i d_j) in f64 (the value
 // shifted_normalized_adjacency_mul forms, lib.rs:4211).
-__global__ void prow_fill_kernel(const int32_t* __restrict__ perm, int64_t rows,
+// One thread per graph entry; entries of vertices outside the GPU
+// components (pos_of < 0) are skipped.
+__global__ void prow_fill_kernel(const int32_t* __restrict__ head, int64_t ne,
                                  const int32_t* __restrict__ row_ptr,
                                  const int32_t* __restrict__ tail, const float* __restrict__ w,
                                  const double* __restrict__ inv, const int32_t* __restrict__ pos_of,
                                  const int32_t* __restrict__ prow_ptr, int32_t* __restrict__ pcol,
                                  float* __restrict__ pnw) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= rows) return;
-  const int32_t v = perm[p];
-  int32_t o = prow_ptr[p];
-  const double iv = inv[v];
-  for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e, ++o) {
-    const int32_t t = tail[e];
-    pcol[o] = pos_of[t];
-    pnw[o] = (t == v) ? 0.f : static_cast<float>(static_cast<double>(fmaxf(w[e], 0.f)) * iv * inv[t]);
-  }
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int32_t v = head[e];
+  const int32_t p = pos_of[v];
+  if (p < 0) return;
+  const int32_t o = prow_ptr[p] + static_cast<int32_t>(e - row_ptr[v]);
+  const int32_t t = tail[e];
+  pcol[o] = pos_of[t];
+  pnw[o] = (t == v) ? 0.f : static_cast<float>(static_cast<double>(fmaxf(w[e], 0.f)) * inv[v] * inv[t]);
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ a, int64_t n, int32_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 // Start block: column 0 = sqrt(degree), the rest U(-1, 1) (lib.rs:4252-4260;
@@ -849,6 +860,8 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     ws.label.ensure(n);  // reused as pos_of
     hipLaunchKernelGGL(degree_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(),
                        g.weight.get(), n, ws.deg.get(), ws.inv.get());
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, ws.label.get(), n,
+                       -1);
     hipLaunchKernelGGL(pos_of_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
                        ws.label.get());
     KERNEL_CHECK();
@@ -863,9 +876,9 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     HIP_CHECK(hipMemcpyAsync(prow_ptr.get() + rows, &pe32, 4, hipMemcpyHostToDevice, s));
     DevBuf<int32_t> pcol(std::max<int64_t>(pe, 1));
     DevBuf<float> pnw(std::max<int64_t>(pe, 1));
-    hipLaunchKernelGGL(prow_fill_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
-                       g.row_ptr.get(), g.tail.get(), g.weight.get(), ws.inv.get(), ws.label.get(),
-                       prow_ptr.get(), pcol.get(), pnw.get());
+    hipLaunchKernelGGL(prow_fill_kernel, dim3(blocks_for(g.n_edges, 256)), dim3(256), 0, s,
+                       g.head.get(), g.n_edges, g.row_ptr.get(), g.tail.get(), g.weight.get(),
+                       ws.inv.get(), ws.label.get(), prow_ptr.get(), pcol.get(), pnw.get());
     hipLaunchKernelGGL(start_block_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
                        rows, d_slot, d_local, ws.seeds.get(), ws.deg.get(), ws.y.get());
     KERNEL_CHECK();
