@@ -20,6 +20,15 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
+// Running maximum of non-negative weights (as bits, whose unsigned order is
+// the float order).  The maximum is reached early (weights run to 1), so a
+// read of the running value spares nearly every row its atomic on one
+// address; a stale read only costs an atomic.
+__device__ __forceinline__ void update_max(uint32_t* bits, float v) {
+  const uint32_t u = __float_as_uint(v);
+  if (u > __hip_atomic_load(bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(bits, u);
+}
+
 __device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* lds, int32_t* total) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -294,7 +303,7 @@ __device__ void union_row_serial(const int32_t* __restrict__ idx, const float* _
     }
   }
   ucount[i] = cnt;
-  if (cnt > 0) atomicMax(wmax_bits, __float_as_uint(local_max));
+  if (cnt > 0) update_max(wmax_bits, local_max);
 }
 
 // One wavefront per row: the outgoing (k) and incoming entries of the row are
@@ -387,7 +396,7 @@ __global__ __launch_bounds__(256) void union_rows_kernel(
   if (lane == 0) {
     const int cnt = __popcll(bal);
     ucount[i] = cnt;
-    if (cnt > 0) atomicMax(wmax_bits, __float_as_uint(mx));
+    if (cnt > 0) update_max(wmax_bits, mx);
   }
 }
 
@@ -508,7 +517,7 @@ __global__ __launch_bounds__(256) void union_long_kernel(
   if (tid == 0) {
     ucount[i] = total;
     const float m4 = fmaxf(fmaxf(wmx[0], wmx[1]), fmaxf(wmx[2], wmx[3]));
-    if (total > 0) atomicMax(wmax_bits, __float_as_uint(m4));
+    if (total > 0) update_max(wmax_bits, m4);
   }
 }
 
@@ -532,7 +541,7 @@ __global__ void bipartite_rows_kernel(const int32_t* __restrict__ idx,
     }
   }
   ucount[i] = cnt;
-  if (cnt > 0) atomicMax(wmax_bits, __float_as_uint(local_max));
+  if (cnt > 0) update_max(wmax_bits, local_max);
 }
 
 // prune_edges (lib.rs:3804-3822): keep w >= max/denom and w > 0.
