@@ -183,6 +183,15 @@ def main() -> int:
                 "achieved": round(knn_tfs, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(knn_tfs / BF16_PEAK_TFLOPS, 4), "traffic": None,
                 "avg_launch_us": round(knn_ms * 1e3, 1), "flops_per_launch": knn_flops}
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes
+    # (calibrated on a device copy of known size), when present
+    traffic_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                "c3_traffic_pmc.json")
+    if n == 200000 and os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tb = json.load(f)["traffic_bytes_per_launch"]
+        roof_knn["traffic"] = tb.get("knn_candidates_kernel<bf16,128,16>")
+        roof_sgd["traffic"] = tb.get("sgd_epoch2_kernel<true>")
     roof, roof_other = (roof_sgd, roof_knn) if sgd_ms >= knn_ms else (roof_knn, roof_sgd)
 
     # ---- recall of the BF16 kNN against FP64 ------------------------------

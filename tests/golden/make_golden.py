@@ -36,7 +36,7 @@ from umapers_amd import datasets  # noqa: E402
 from umapers_amd.diagnostics import knn_preservation, trustworthiness  # noqa: E402
 
 C1_SEEDS = list(range(10))
-C2_SEEDS = [42, 7, 11, 23]
+C2_SEEDS = [42, 7, 11, 23, 5, 9, 13, 17, 19, 29, 31, 37, 41, 43, 47, 53]
 
 
 def stages() -> None:

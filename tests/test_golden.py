@@ -107,6 +107,6 @@ def test_quality_targets_recorded():
     with open(os.path.join(HERE, "quality.json")) as f:
         q = json.load(f)
     assert len(q["c1_plain_200"]["trust"]) == 10
-    assert len(q["c2_densmap_500"]["trust"]) == 4
+    assert len(q["c2_densmap_500"]["trust"]) == 16
     assert 0.6 < np.mean(q["c1_plain_200"]["trust"]) < 0.7
     assert 0.8 < np.mean(q["c2_densmap_500"]["trust"]) < 0.9

@@ -47,11 +47,15 @@ def test_graph_and_schedule_equal_fixture(gpu, c1):
 
 def test_densmap_fit_parity_config2(gpu):
     # 20000 x 192, 10 clusters, densMAP on, 500 epochs: mean trustworthiness
-    # over the oracle's recorded seeds; Hogwild runs of one seed vary by ~0.01
+    # against the oracle's mean over 16 recorded seeds.  One seed's value moves
+    # by ~0.01 between seeds and between Hogwild runs, so the GPU side is
+    # averaged over 24 seeds (spread of the mean ~0.002; measured GPU means
+    # 0.844-0.847 against the oracle's 0.850).
     with open(os.path.join(HERE, "quality.json")) as f:
         q = json.load(f)["c2_densmap_500"]
     x = datasets.c2()
+    seeds = list(q["seeds"]) + [59, 61, 67, 71, 73, 79, 83, 89]
     t = [trustworthiness(x, Umap(random_seed=s, ann_mode="exact", n_epochs=500, densmap=True,
                                  dens_lambda=2.0, dens_frac=0.3).fit_transform(x))
-         for s in q["seeds"]]
+         for s in seeds]
     assert np.mean(t) >= np.mean(q["trust"]) - 0.01, (t, q["trust"])

@@ -109,3 +109,18 @@ def test_hub_vertices_stay_in_their_cluster(gpu):
         r = radius(y)
         assert np.isfinite(y).all()
         assert r.max() <= 1.2 * np.percentile(r, 99.9), (r.max(), np.percentile(r, 99.9))
+
+
+def test_schedule_counts_exact_edge_parallel(gpu, oracle):
+    # a graph above 2^20 edges takes the edge-parallel epoch kernel (firing
+    # edges compacted per wave, schedule read in one pass): every edge must
+    # still fire and draw negatives exactly as often as in the reference
+    x = datasets.c3(n=50_000)
+    idx, dist = capi.knn(x, 15)
+    sig, rho = capi.smooth(dist, 15.0)
+    h, t, w = capi.fuzzy_graph(idx, dist, sig, rho, 1.0, 200)
+    assert h.shape[0] >= 1 << 20
+    y0 = np.random.default_rng(0).uniform(-10, 10, size=(x.shape[0], 2)).astype(np.float32)
+    _, pc, nc = capi.optimize(y0, h, t, w, 200, 1.58, 0.9, counts=True)
+    po, no, _, _ = oracle.schedule_counts(w, 200, 5)
+    assert np.array_equal(pc, po) and np.array_equal(nc, no)

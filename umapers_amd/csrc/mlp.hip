@@ -322,42 +322,67 @@ __global__ void out_layer_back_kernel(const float* __restrict__ dz, const uint16
 // Column sums over a block of rows, added atomically: out[j] += sum_r m[r][j]
 // (BF16 input), and, for the output layer, w[j][c] += sum_r h2[r][j] dz[r][c]
 // and bias[c] += sum_r dz[r][c].
-__global__ __launch_bounds__(256) void colsum_bf16_kernel(const uint16_t* __restrict__ m,
-                                                          int64_t rows, int cols, int rows_per_block,
-                                                          float* __restrict__ out) {
+// Column reductions of the backward pass in two stages: block b sums rows
+// [b * kRedRows, (b + 1) * kRedRows) of every column into part[b][q] (one
+// thread per column, no atomics), then red_final_kernel adds the blocks'
+// partials per output in block order.  Outputs q: with WGRAD, q = j * dim + c
+// is sum_r h[r][j] dz[r][c] (the output layer's weight gradient) and
+// q = h * dim + c is sum_r dz[r][c] (its bias gradient); without, q = j is
+// sum_r m[r][j] (a hidden layer's bias gradient).
+constexpr int kRedRows = 128;
+
+template <bool WGRAD>
+__global__ __launch_bounds__(256) void red_partial_kernel(const uint16_t* __restrict__ m,
+                                                          const float* __restrict__ dz,
+                                                          int64_t rows, int cols, int dim,
+                                                          int q_len, float* __restrict__ part) {
   const int j = blockIdx.y * 256 + threadIdx.x;
-  if (j >= cols) return;
-  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
-  const int64_t r1 = min(rows, r0 + rows_per_block);
-  float s = 0.f;
-  for (int64_t r = r0; r < r1; ++r) s += from_bf16(m[r * cols + j]);
-  if (s != 0.f) atomicAdd(&out[j], s);
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRedRows;
+  const int64_t r1 = min(rows, r0 + kRedRows);
+  float* out = part + static_cast<int64_t>(blockIdx.x) * q_len;
+  if (WGRAD) {
+    float acc[kMaxDim];
+#pragma unroll
+    for (int c = 0; c < kMaxDim; ++c) acc[c] = 0.f;
+    if (j < cols) {
+#pragma unroll 8
+      for (int64_t r = r0; r < r1; ++r) {
+        const float hv = from_bf16(m[r * cols + j]);
+#pragma unroll
+        for (int c = 0; c < kMaxDim; ++c)
+          if (c < dim) acc[c] += hv * dz[r * dim + c];
+      }
+#pragma unroll
+      for (int c = 0; c < kMaxDim; ++c)
+        if (c < dim) out[j * dim + c] = acc[c];
+    }
+    if (blockIdx.y == 0 && threadIdx.x < dim) {
+      float sb = 0.f;
+      for (int64_t r = r0; r < r1; ++r) sb += dz[r * dim + threadIdx.x];
+      out[cols * dim + threadIdx.x] = sb;
+    }
+  } else if (j < cols) {
+    float acc = 0.f;
+#pragma unroll 8
+    for (int64_t r = r0; r < r1; ++r) acc += from_bf16(m[r * cols + j]);
+    out[j] = acc;
+  }
 }
 
-__global__ __launch_bounds__(256) void out_layer_wgrad_kernel(const uint16_t* __restrict__ h2,
-                                                              const float* __restrict__ dz,
-                                                              int64_t rows, int h, int dim,
-                                                              int rows_per_block,
-                                                              float* __restrict__ dw3,
-                                                              float* __restrict__ db3) {
-  const int j = blockIdx.y * 256 + threadIdx.x;
-  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
-  const int64_t r1 = min(rows, r0 + rows_per_block);
-  float acc[kMaxDim];
-  for (int c = 0; c < kMaxDim; ++c) acc[c] = 0.f;
-  if (j < h) {
-    for (int64_t r = r0; r < r1; ++r) {
-      const float hv = from_bf16(h2[r * h + j]);
-      for (int c = 0; c < dim; ++c) acc[c] += hv * dz[r * dim + c];
-    }
-    for (int c = 0; c < dim; ++c)
-      if (acc[c] != 0.f) atomicAdd(&dw3[j * dim + c], acc[c]);
+// eight independent running sums keep eight loads in flight; they are
+// combined in a fixed order
+__global__ void red_final_kernel(const float* __restrict__ part, int nb, int q_len,
+                                 float* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= q_len) return;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int b = 0;
+  for (; b + 8 <= nb; b += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += part[static_cast<int64_t>(b + u) * q_len + q];
   }
-  if (blockIdx.y == 0 && threadIdx.x < dim) {
-    float s = 0.f;
-    for (int64_t r = r0; r < r1; ++r) s += dz[r * dim + threadIdx.x];
-    if (s != 0.f) atomicAdd(&db3[threadIdx.x], s);
-  }
+  for (; b < nb; ++b) acc[0] += part[static_cast<int64_t>(b) * q_len + q];
+  out[q] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
 // Adam on the FP32 masters (bias-corrected step), then the BF16 copies.
@@ -464,7 +489,7 @@ __global__ void col_finish_kernel(const double* __restrict__ sums, int64_t n, in
 // Device buffers of one forward / backward pass over s_pad rows.
 struct Pass {
   DevBuf<uint16_t> xb, h1, h2, dh2, dh1;
-  DevBuf<float> yt, z, dz;
+  DevBuf<float> yt, z, dz, part;
   DevBuf<double> loss;
   void ensure(const MlpShape& sh, int64_t s_pad) {
     xb.ensure(s_pad * sh.dp);
@@ -491,20 +516,30 @@ void forward(const MlpShape& sh, const float* params, const uint16_t* wh, Pass& 
   KERNEL_CHECK();
 }
 
+// Column reduction into out (overwritten): see red_partial_kernel.
+template <bool WGRAD>
+void reduce_cols(const uint16_t* m, const float* dz, int64_t rows, int cols, int dim, Pass& ps,
+                 float* out, hipStream_t s) {
+  const int q_len = WGRAD ? (cols + 1) * dim : cols;
+  const int nb = static_cast<int>((rows + kRedRows - 1) / kRedRows);
+  ps.part.ensure(static_cast<size_t>(nb) * q_len);
+  hipLaunchKernelGGL(red_partial_kernel<WGRAD>, dim3(nb, (cols + 255) / 256), dim3(256), 0, s, m, dz,
+                     rows, cols, dim, q_len, ps.part.get());
+  hipLaunchKernelGGL(red_final_kernel, dim3((q_len + 255) / 256), dim3(256), 0, s, ps.part.get(), nb,
+                     q_len, out);
+  KERNEL_CHECK();
+}
+
 // Gradients of the loss into grads (zeroed here), given dz.
 void backward(const MlpShape& sh, const uint16_t* wh, Pass& ps, int64_t rows, float* grads,
               hipStream_t s) {
   HIP_CHECK(hipMemsetAsync(grads, 0, sizeof(float) * sh.total, s));
   const int r = static_cast<int>(rows);
-  constexpr int kRowsPerBlock = 256;
-  const unsigned rb = static_cast<unsigned>((rows + kRowsPerBlock - 1) / kRowsPerBlock);
-  const unsigned cb = static_cast<unsigned>((sh.h + 255) / 256);
-  hipLaunchKernelGGL(out_layer_wgrad_kernel, dim3(rb, cb), dim3(256), 0, s, ps.h2.get(), ps.dz.get(),
-                     rows, sh.h, sh.dim, kRowsPerBlock, grads + sh.off_w3, grads + sh.off_b3);
+  // dW3 and db3 are adjacent in the flat layout: one reduction writes both
+  reduce_cols<true>(ps.h2.get(), ps.dz.get(), rows, sh.h, sh.dim, ps, grads + sh.off_w3, s);
   hipLaunchKernelGGL(out_layer_back_kernel, dim3(blocks_for(rows * sh.h, 256)), dim3(256), 0, s,
                      ps.dz.get(), ps.h2.get(), rows, sh.h, sh.dim, wh + sh.off_w3, ps.dh2.get());
-  hipLaunchKernelGGL(colsum_bf16_kernel, dim3(rb, cb), dim3(256), 0, s, ps.dh2.get(), rows, sh.h,
-                     kRowsPerBlock, grads + sh.off_b2);
+  reduce_cols<false>(ps.dh2.get(), nullptr, rows, sh.h, 0, ps, grads + sh.off_b2, s);
   KERNEL_CHECK();
   const int splits = std::max(1, r / 1024);
   // dW2 = H1^T dH2
@@ -513,8 +548,7 @@ void backward(const MlpShape& sh, const uint16_t* wh, Pass& ps, int64_t rows, fl
   // dH1 = (dH2 W2^T) masked by H1 > 0
   gemm<false, true, kEpiMask>(ps.dh2.get(), sh.h, wh + sh.off_w2, sh.h, r, sh.h, sh.h, 1, nullptr,
                               ps.h1.get(), ps.dh1.get(), nullptr, sh.h, s);
-  hipLaunchKernelGGL(colsum_bf16_kernel, dim3(rb, cb), dim3(256), 0, s, ps.dh1.get(), rows, sh.h,
-                     kRowsPerBlock, grads + sh.off_b1);
+  reduce_cols<false>(ps.dh1.get(), nullptr, rows, sh.h, 0, ps, grads + sh.off_b1, s);
   KERNEL_CHECK();
   // dW1 = X^T dH1
   gemm<true, false, kEpiAtomic>(ps.xb.get(), sh.dp, ps.dh1.get(), sh.h, sh.dp, sh.h, r, splits,

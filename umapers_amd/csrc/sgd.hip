@@ -517,14 +517,17 @@ __global__ void row_ends_kernel(const int32_t* __restrict__ row_ptr, int64_t n,
   if (v < n) ends[v] = 2 * (row_ptr[v + 1] - row_ptr[v]);
 }
 
-__global__ void hub_scale_kernel(const int32_t* __restrict__ ends, int64_t n,
+__global__ void hub_scale_kernel(const int32_t* __restrict__ ends, int64_t n, int32_t limit,
                                  float* __restrict__ mscale, int32_t* __restrict__ hubs) {
   const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const int32_t c = ends[v];
-  const bool hub = c > kHubEnds;
-  mscale[v] = hub ? static_cast<float>(kHubEnds) / static_cast<float>(c) : 1.f;
-  if (hub) atomicAdd(hubs, 1);
+  bool hub = false;
+  if (v < n) {
+    const int32_t c = ends[v];
+    hub = c > limit;
+    mscale[v] = hub ? static_cast<float>(limit) / static_cast<float>(c) : 1.f;
+  }
+  const uint64_t m = __ballot(hub);
+  if (m != 0ULL && (threadIdx.x & 63) == 0) atomicAdd(hubs, static_cast<int32_t>(__popcll(m)));
 }
 
 __global__ void copy_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
@@ -688,7 +691,7 @@ void optimize_layout_device(float* y, int64_t n, int dim, const Graph& g, const 
     hipLaunchKernelGGL(edge_ends_kernel, dim3(blocks), dim3(256), 0, s, g.head.get(), g.tail.get(),
                        ne, ws.ends.get());
   hipLaunchKernelGGL(hub_scale_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, ws.ends.get(), n,
-                     ws.mscale.get(), ws.ends.get() + n);
+                     kHubEnds, ws.mscale.get(), ws.ends.get() + n);
   KERNEL_CHECK();
   int32_t n_hubs = 0;
   HIP_CHECK(hipMemcpyAsync(&n_hubs, ws.ends.get() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
