@@ -1,7 +1,8 @@
 """Writes the golden fixtures of this directory from the oracle (the C++
 restatement of the reference's CPU path under oracle/).
 
-    python tests/golden/make_golden.py          # all fixtures (a few minutes)
+    python tests/golden/make_golden.py          # all fixtures (about 15 minutes)
+    python tests/golden/make_golden.py c4_sub50k_k30_200   # one quality section
 
 Fixtures (all small):
   c1_stages.npz  config-1 (1000 x 32 i.i.d. N(0,1), seed 42) stage outputs:
@@ -11,8 +12,10 @@ Fixtures (all small):
   inputs.json    SHA-256 of the synthetic inputs of configs 1-3 and of the
                  100,000-row fit set of config 5
   quality.json   the oracle's trustworthiness@15 and kNN preservation at
-                 configs 1 and 2 over fixed seeds (the statistical parity
-                 targets of the GPU fit tests)
+                 config 1, config 2 (densMAP, 500 epochs) and a 50,000-row
+                 draw of config 4's mixture (datasets.c4(n=50000), k = 30)
+                 over fixed seeds (the statistical
+                 parity targets of the GPU fit tests)
 
 The reference (Rust) is not run: it holds no array fixtures of its own for
 this path, and its toolchain is not available here.  Each fixture is checked
@@ -37,6 +40,7 @@ from umapers_amd.diagnostics import knn_preservation, trustworthiness  # noqa: E
 
 C1_SEEDS = list(range(10))
 C2_SEEDS = [42, 7, 11, 23, 5, 9, 13, 17, 19, 29, 31, 37, 41, 43, 47, 53]
+C4_SEEDS = [42, 7, 11, 23]
 
 
 def stages() -> None:
@@ -59,28 +63,41 @@ def inputs() -> None:
         json.dump(out, f, indent=1)
 
 
-def quality() -> None:
+def _fit_quality(x, seeds, **kw) -> dict:
+    t, k = [], []
+    for s in seeds:
+        y = O.fit(x, O.OracleParams(random_seed=s, **kw)).embedding
+        t.append(trustworthiness(x, y))
+        k.append(knn_preservation(x, y))
+        print(kw, "seed", s, t[-1], flush=True)
+    return {"seeds": list(seeds), "trust": t, "knn_preservation": k}
+
+
+QUALITY = {
+    "c1_plain_200": lambda: _fit_quality(datasets.c1(), C1_SEEDS, n_epochs=200),
+    "c2_densmap_500": lambda: _fit_quality(datasets.c2(), C2_SEEDS, n_epochs=500, densmap=True),
+    "c4_sub50k_k30_200": lambda: _fit_quality(datasets.c4(n=50_000), C4_SEEDS, n_epochs=200,
+                                              n_neighbors=30),
+}
+
+
+def quality(only=None) -> None:
+    path = os.path.join(HERE, "quality.json")
     res = {}
-    x = datasets.c1()
-    t, k = [], []
-    for s in C1_SEEDS:
-        y = O.fit(x, O.OracleParams(n_epochs=200, random_seed=s)).embedding
-        t.append(trustworthiness(x, y))
-        k.append(knn_preservation(x, y))
-    res["c1_plain_200"] = {"seeds": C1_SEEDS, "trust": t, "knn_preservation": k}
-    x = datasets.c2()
-    t, k = [], []
-    for s in C2_SEEDS:
-        y = O.fit(x, O.OracleParams(n_epochs=500, random_seed=s, densmap=True)).embedding
-        t.append(trustworthiness(x, y))
-        k.append(knn_preservation(x, y))
-        print("c2 densmap seed", s, t[-1], flush=True)
-    res["c2_densmap_500"] = {"seeds": C2_SEEDS, "trust": t, "knn_preservation": k}
-    with open(os.path.join(HERE, "quality.json"), "w") as f:
+    if os.path.exists(path):
+        with open(path) as f:
+            res = json.load(f)
+    for key, make in QUALITY.items():
+        if only is None or key in only:
+            res[key] = make()
+    with open(path, "w") as f:
         json.dump(res, f, indent=1)
 
 
 if __name__ == "__main__":
-    stages()
-    inputs()
-    quality()
+    if len(sys.argv) > 1:  # named quality sections only, e.g. c4_sub50k_k30_200
+        quality(set(sys.argv[1:]))
+    else:
+        stages()
+        inputs()
+        quality()

@@ -110,3 +110,5 @@ def test_quality_targets_recorded():
     assert len(q["c2_densmap_500"]["trust"]) == 16
     assert 0.6 < np.mean(q["c1_plain_200"]["trust"]) < 0.7
     assert 0.8 < np.mean(q["c2_densmap_500"]["trust"]) < 0.9
+    assert len(q["c4_sub50k_k30_200"]["trust"]) == 4
+    assert 0.98 < np.mean(q["c4_sub50k_k30_200"]["trust"]) < 1.0
