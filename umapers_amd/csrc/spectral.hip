@@ -336,60 +336,66 @@ __global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ i
   }
 }
 
+// v held by `lane`, as a wave-uniform value
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
 // Cholesky factor of every slot's Gram matrix (upper R, G = R^T R) and its
-// inverse, one block per slot; gram[slot] is cleared for the next
+// inverse, one wave per slot; gram[slot] is cleared for the next
 // accumulation.  A pivot <= 1e-12 (or non-finite) flags the slot as failed:
 // it equals the residual norm Gram-Schmidt checks in lib.rs:4186-4190.
+//
+// Lane c holds column c of the matrix in registers (rows 0..c; the rest stay
+// zero) and the factorisation runs right-looking: step j takes the pivot from
+// lane j, scales row j and subtracts its outer product from the trailing rows,
+// with row j's entries broadcast by readlane.  The subtractions reach each
+// entry in the order k = 0, 1, .., j-1, i.e. G[j][c] - sum_k R[k][j] R[k][c]
+// in index order, and R^-1 is formed column by column by back substitution.
 __global__ __launch_bounds__(64) void chol_kernel(double* __restrict__ gram,
                                                   double* __restrict__ rinv,
                                                   int32_t* __restrict__ fail) {
-  __shared__ double G[kSub][kSub];
-  __shared__ double R[kSub][kSub + 1];
-  __shared__ double Ri[kSub][kSub + 1];
-  __shared__ int bad;
-  const int tid = threadIdx.x;
+  const int c = threadIdx.x;
   double* g = gram + static_cast<int64_t>(blockIdx.x) * kSub * kSub;
-  for (int q = tid; q < kSub * kSub; q += 64) {
-    const int i = q / kSub, j = q % kSub;
-    G[i][j] = i <= j ? g[i * kSub + j] : g[j * kSub + i];
-    R[i][j] = 0.0;
-    Ri[i][j] = 0.0;
-  }
-  if (tid == 0) bad = 0;
-  __syncthreads();
-  for (int q = tid; q < kSub * kSub; q += 64) g[q] = 0.0;
+  double a[kSub];
+#pragma unroll
+  for (int i = 0; i < kSub; ++i) a[i] = (c < kSub && i <= c) ? g[i * kSub + c] : 0.0;
+  bool bad = false;
+#pragma unroll
   for (int j = 0; j < kSub; ++j) {
-    if (tid == 0) {
-      double piv = G[j][j];
-      for (int k = 0; k < j; ++k) piv -= R[k][j] * R[k][j];
-      if (!(piv > kOrthoEps) || !isfinite(piv)) {
-        bad = 1;
-        piv = 1.0;
-      }
-      R[j][j] = sqrt(piv);
+    double piv = lane_value(a[j], j);
+    if (!(piv > kOrthoEps) || !isfinite(piv)) {
+      bad = true;
+      piv = 1.0;
     }
-    __syncthreads();
-    if (tid > j && tid < kSub) {
-      double v = G[j][tid];
-      for (int k = 0; k < j; ++k) v -= R[k][j] * R[k][tid];
-      R[j][tid] = v / R[j][j];
-    }
-    __syncthreads();
-  }
-  // column c of R^-1 by back substitution (each thread owns one column)
-  if (tid < kSub) {
-    const int c = tid;
-    Ri[c][c] = 1.0 / R[c][c];
-    for (int i = c - 1; i >= 0; --i) {
-      double v = 0.0;
-      for (int k = i + 1; k <= c; ++k) v += R[i][k] * Ri[k][c];
-      Ri[i][c] = -v / R[i][i];
+    const double rjj = sqrt(piv);
+    a[j] = c == j ? rjj : (c > j ? a[j] / rjj : a[j]);  // row j of R
+#pragma unroll
+    for (int i = j + 1; i < kSub; ++i) {
+      const double rji = lane_value(a[j], i);
+      if (c >= i) a[i] = fma(-rji, a[j], a[i]);
     }
   }
-  __syncthreads();
-  double* out = rinv + static_cast<int64_t>(blockIdx.x) * kSub * kSub;
-  for (int q = tid; q < kSub * kSub; q += 64) out[q] = Ri[q / kSub][q % kSub];
-  if (tid == 0 && bad) fail[blockIdx.x] = 1;
+  // column c of R^-1: x[i] = -(sum_{k=i+1..c} R[i][k] x[k]) / R[i][i], x[c] = 1 / R[c][c]
+  double x[kSub];
+#pragma unroll
+  for (int i = kSub - 1; i >= 0; --i) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = i + 1; k < kSub; ++k) v = fma(lane_value(a[i], k), x[k], v);
+    const double rii = lane_value(a[i], i);
+    x[i] = c == i ? 1.0 / rii : (c > i ? -v / rii : 0.0);
+  }
+  if (c < kSub) {
+    double* out = rinv + static_cast<int64_t>(blockIdx.x) * kSub * kSub;
+#pragma unroll
+    for (int i = 0; i < kSub; ++i) out[i * kSub + c] = x[i];
+  }
+  for (int q = c; q < kSub * kSub; q += 64) g[q] = 0.0;
+  if (c == 0 && bad) fail[blockIdx.x] = 1;
 }
 
 // dst = src R^-1 for the item's rows (f64 products, stored f32).  One
