@@ -148,6 +148,15 @@ int32_t umapx_transform_learned(umapx_handle* h, const float* q, int64_t m, int6
 int32_t umapx_transform_learned_device(umapx_handle* h, const float* d_q, int64_t m, int64_t d,
                                        float* d_out);
 
+/* inverse_transform_dense (lib.rs:1789-1915): embedded rows [m x n_components]
+ * back to the training feature space [m x n_features]: kNN among the fitted
+ * embedding, the local linear initialisation (lib.rs:4838-4935) and the
+ * feature-space SGD (lib.rs:5547-5645). */
+int32_t umapx_inverse_transform(umapx_handle* h, const float* q, int64_t m, int64_t dim,
+                                float* out);
+int32_t umapx_inverse_transform_device(umapx_handle* h, const float* d_q, int64_t m,
+                                       int64_t dim, float* d_out);
+
 /* Fitted state (UmapModel accessors, lib.rs:600-634). */
 int32_t umapx_n_samples(const umapx_handle* h, int64_t* n);
 int32_t umapx_n_features(const umapx_handle* h, int64_t* d);
@@ -233,6 +242,15 @@ int32_t umapx_local_radii(const int64_t* idx, const float* dist, int64_t n, int3
                           float var_shift, float* out);
 int32_t umapx_embedding_radii(const float* y, int64_t n, int32_t dim, const int64_t* idx,
                               int32_t k, float var_shift, float* out);
+
+/* The first two steps of inverse_transform_rows (lib.rs:1856-1890) and
+ * init_inverse_points (lib.rs:4838-4935) without the SGD: the exact kNN of q
+ * [m x dim] among y_train [n x dim] (k = min(k, n)), the weights
+ * 1 / (1 + a d^2b) and the local linear estimate in the space of x_train
+ * [n x d], written to out [m x d]. */
+int32_t umapx_inverse_init(const float* x_train, int64_t n, int64_t d, const float* y_train,
+                           int32_t dim, int32_t k, float a, float b, const float* q, int64_t m,
+                           float* out);
 
 #ifdef __cplusplus
 }

@@ -124,6 +124,11 @@ def lib() -> ctypes.CDLL:
                                     ctypes.c_int64, _f32p, ctypes.c_float, ctypes.c_float,
                                     _f32p, ctypes.c_int64, _f32p]
         L.orc_transform.restype = ctypes.c_int32
+        L.orc_inverse_transform.argtypes = [ctypes.POINTER(OrcParams), _f32p, ctypes.c_int64,
+                                            ctypes.c_int64, _f32p, _f32p, _f32p, ctypes.c_float,
+                                            ctypes.c_float, _f32p, ctypes.c_int64, ctypes.c_int32,
+                                            _f32p]
+        L.orc_inverse_transform.restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -382,5 +387,24 @@ def transform(x_train, y_train, q, params: OracleParams | None = None):
     rc = lib().orc_transform(ctypes.byref(cp), _p(x_train, _f32p), x_train.shape[0],
                              x_train.shape[1], _p(y_train, _f32p), a, b, _p(q, _f32p),
                              q.shape[0], _p(out, _f32p))
+    assert rc == 0
+    return out
+
+
+def inverse_transform(x_train, y_train, sigmas, rhos, q, params: OracleParams | None = None,
+                      optimize: bool = True):
+    """inverse_transform_rows (lib.rs:1800-1915): embedded rows q back to the
+    feature space of x_train, given the fitted embedding and the fit's sigma /
+    rho.  optimize=False returns the local linear initialisation only."""
+    params = params or OracleParams()
+    x_train, y_train, q = _f32(x_train), _f32(y_train), _f32(q)
+    sigmas, rhos = _f32(sigmas), _f32(rhos)
+    a, b = find_ab_params(params.spread, params.min_dist)
+    out = np.empty((q.shape[0], x_train.shape[1]), np.float32)
+    cp = params.c()
+    rc = lib().orc_inverse_transform(ctypes.byref(cp), _p(x_train, _f32p), x_train.shape[0],
+                                     x_train.shape[1], _p(y_train, _f32p), _p(sigmas, _f32p),
+                                     _p(rhos, _f32p), a, b, _p(q, _f32p), q.shape[0],
+                                     1 if optimize else 0, _p(out, _f32p))
     assert rc == 0
     return out

@@ -15,7 +15,10 @@
  *     Gram-Schmidt (lib.rs:4161-4326) and the multi-component layout
  *     (lib.rs:4463-4764);
  *   - the sequential SGD with its exact f32 per-edge schedule
- *     (lib.rs:4937-4960, 5137-5441) and the transform SGD (lib.rs:5444-5544).
+ *     (lib.rs:4937-4960, 5137-5441) and the transform SGD (lib.rs:5444-5544);
+ *   - the inverse transform: kNN in the embedding, the local linear
+ *     initialisation and the feature-space SGD (lib.rs:1800-1915, 4838-4935,
+ *     5547-5645).
  * The stages the reference runs under rayon (exact kNN rows) use OpenMP; the
  * sequential ones stay single-threaded.
  *
@@ -31,6 +34,8 @@
  *                       exists to check them against          parity unpinned
  *   Normal noise        Box-Muller, not rand_distr's ziggurat   parity unpinned
  *   symmetric eigen     cyclic Jacobi, not nalgebra 0.33.3      parity unpinned (signs)
+ *   inverse init        small LU solve vs numpy.linalg.solve, fixed rows exact,
+ *                       affine data reconstructed exactly      pinned (to f32 rounding)
  *   SGD embedding       statistical (trustworthiness, kNN preservation)
  */
 #include "umap_oracle.h"
@@ -1220,6 +1225,224 @@ void optimize_layout_transform(float* emb, int64_t m, int dim, const float* base
   }
 }
 
+/* lib.rs:3420-3422 inverse_neighbor_count */
+int32_t inverse_neighbor_count(int32_t n_neighbors, int64_t n_train) {
+  return static_cast<int32_t>(std::max<int64_t>(std::min<int64_t>(n_neighbors, n_train), 1));
+}
+
+/* Partial-pivot LU of a small f32 matrix and its solve, in the steps of
+ * nalgebra 0.33's LU::new / LU::solve (the crate is not in the reference
+ * tree; restated from its published source): pivot = first row of maximal
+ * |.| in the column, a zero pivot skips the column, coefficients are scaled
+ * by 1/pivot, rows updated as y + (-p) * x; the solve permutes b, runs the
+ * unit-lower then the upper substitution and fails on a zero diagonal. */
+struct SmallLu {
+  int n = 0;
+  vector<float> lu;            /* row-major n x n */
+  vector<std::pair<int, int>> swaps;
+  void factor(const vector<float>& m, int dim) {
+    n = dim;
+    lu = m;
+    swaps.clear();
+    for (int i = 0; i < n; ++i) {
+      int piv = i;
+      float best = std::fabs(lu[i * n + i]);
+      for (int r = i + 1; r < n; ++r)
+        if (std::fabs(lu[r * n + i]) > best) {
+          best = std::fabs(lu[r * n + i]);
+          piv = r;
+        }
+      const float diag = lu[piv * n + i];
+      if (diag == 0.0f) continue;
+      if (piv != i) {
+        swaps.push_back({i, piv});
+        for (int c = 0; c < n; ++c) std::swap(lu[i * n + c], lu[piv * n + c]);
+      }
+      const float inv = 1.0f / diag;
+      for (int r = i + 1; r < n; ++r) lu[r * n + i] *= inv;
+      for (int c = i + 1; c < n; ++c) {
+        const float a = -lu[i * n + c];
+        for (int r = i + 1; r < n; ++r) lu[r * n + c] = a * lu[r * n + i] + lu[r * n + c];
+      }
+    }
+  }
+  /* b: n x cols row-major, solved in place; false when singular */
+  bool solve(vector<float>& b, int64_t cols) const {
+    for (const auto& sw : swaps)
+      for (int64_t c = 0; c < cols; ++c) std::swap(b[sw.first * cols + c], b[sw.second * cols + c]);
+    for (int64_t c = 0; c < cols; ++c) {
+      for (int i = 0; i + 1 < n; ++i) {
+        const float coeff = b[i * cols + c];
+        for (int r = i + 1; r < n; ++r) b[r * cols + c] = -coeff * lu[r * n + i] + b[r * cols + c];
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        const float diag = lu[i * n + i];
+        if (diag == 0.0f) return false;
+        const float coeff = b[i * cols + c] / diag;
+        b[i * cols + c] = coeff;
+        for (int r = 0; r < i; ++r) b[r * cols + c] = -coeff * lu[r * n + i] + b[r * cols + c];
+      }
+    }
+    return true;
+  }
+};
+
+/* lib.rs:1856-1890: the inverse kNN weights 1 / (1 + a d^2b), normalised per
+ * row, the fixed rows (nearest embedded point at distance <= 1e-6) and the
+ * edge list (query, training row, raw weight). */
+void inverse_weights(const float* dist, int64_t m, int32_t k, float a, float b,
+                     vector<float>& raw, vector<float>& norm, vector<char>& fixed) {
+  raw.assign(static_cast<size_t>(m) * k, 0.0f);
+  norm.assign(static_cast<size_t>(m) * k, 0.0f);
+  fixed.assign(m, 0);
+  const float two_b = 2.0f * b;
+  for (int64_t i = 0; i < m; ++i) {
+    float row_sum = 0.0f;
+    for (int32_t j = 0; j < k; ++j) {
+      const float w = 1.0f / (1.0f + a * std::pow(dist[i * k + j], two_b));
+      if (w > 0.0f && std::isfinite(w)) {
+        raw[i * k + j] = w;
+        norm[i * k + j] = w;
+        row_sum += w;
+      }
+    }
+    if (row_sum > 0.0f)
+      for (int32_t j = 0; j < k; ++j) norm[i * k + j] /= row_sum;
+    fixed[i] = dist[i * k] <= 1e-6f;
+  }
+}
+
+/* lib.rs:4838-4935 init_inverse_points */
+void init_inverse_points(const float* q, int64_t m, int low_dim, const int64_t* idx,
+                         const vector<float>& norm, const vector<char>& fixed, int32_t k,
+                         const float* y_train, const float* x_train, int64_t d, float* out) {
+  const float nanv = std::numeric_limits<float>::quiet_NaN();
+  for (int64_t i = 0; i < m; ++i) {
+    float* res = out + i * d;
+    if (fixed[i]) {
+      std::copy(x_train + idx[i * k] * d, x_train + idx[i * k] * d + d, res);
+      continue;
+    }
+    vector<float> z_bar(low_dim, 0.0f), x_bar(d, 0.0f);
+    float weight_sum = 0.0f;
+    for (int32_t j = 0; j < k; ++j) {
+      const float w = norm[i * k + j];
+      if (w <= 0.0f || !std::isfinite(w)) continue;
+      weight_sum += w;
+      const int64_t r = idx[i * k + j];
+      for (int c = 0; c < low_dim; ++c) z_bar[c] += w * y_train[r * low_dim + c];
+      for (int64_t f = 0; f < d; ++f) x_bar[f] += w * x_train[r * d + f];
+    }
+    if (weight_sum <= 0.0f) {
+      std::fill(res, res + d, nanv);
+      continue;
+    }
+    vector<float> gram(static_cast<size_t>(low_dim) * low_dim, 0.0f);
+    vector<float> rhs(static_cast<size_t>(low_dim) * d, 0.0f);
+    int non_zero = 0;
+    for (int32_t j = 0; j < k; ++j) {
+      const float w = norm[i * k + j];
+      if (w <= 0.0f || !std::isfinite(w)) continue;
+      ++non_zero;
+      const int64_t r = idx[i * k + j];
+      for (int aa = 0; aa < low_dim; ++aa) {
+        const float z_a = y_train[r * low_dim + aa] - z_bar[aa];
+        for (int bb = 0; bb < low_dim; ++bb) {
+          const float z_b = y_train[r * low_dim + bb] - z_bar[bb];
+          gram[aa * low_dim + bb] += w * z_a * z_b;
+        }
+        for (int64_t f = 0; f < d; ++f) {
+          const float x_d = x_train[r * d + f] - x_bar[f];
+          rhs[aa * d + f] += w * z_a * x_d;
+        }
+      }
+    }
+    for (int c = 0; c < low_dim; ++c) gram[c * low_dim + c] += 1e-3f;
+    vector<float> pred = x_bar;
+    if (non_zero > low_dim) {
+      SmallLu lu;
+      lu.factor(gram, low_dim);
+      vector<float> beta = rhs;
+      if (lu.solve(beta, d)) {
+        bool valid = true;
+        for (int64_t f = 0; f < d && valid; ++f) {
+          float value = x_bar[f];
+          for (int aa = 0; aa < low_dim; ++aa)
+            value += (q[i * low_dim + aa] - z_bar[aa]) * beta[aa * d + f];
+          if (!std::isfinite(value)) valid = false;
+          else pred[f] = value;
+        }
+        if (!valid) pred = x_bar;
+      }
+    }
+    std::copy(pred.begin(), pred.end(), res);
+  }
+}
+
+/* lib.rs:5035-5043 euclidean_distance_with_grad_denom */
+inline float dist_sequential(const float* x, const float* y, int64_t d) {
+  float sq = 0.0f;
+  for (int64_t f = 0; f < d; ++f) {
+    const float df = x[f] - y[f];
+    sq += df * df;
+  }
+  return std::sqrt(sq);
+}
+
+/* lib.rs:5547-5645 optimize_layout_inverse (one SmallRng stream, edges in
+ * query order) */
+void optimize_layout_inverse(float* head, int64_t m, const float* tail, int64_t n, int64_t d,
+                             const vector<Edge>& edges, const vector<char>& fixed,
+                             const float* sigmas, const float* rhos, int32_t n_epochs,
+                             float alpha0, int32_t nsr, float gamma, uint64_t seed) {
+  if (edges.empty() || m == 0 || n_epochs == 0) return;
+  vector<float> w(edges.size());
+  for (size_t e = 0; e < edges.size(); ++e) w[e] = edges[e].weight;
+  const vector<float> eps = epochs_per_sample(w.data(), static_cast<int64_t>(w.size()), n_epochs);
+  vector<float> eons = eps;
+  vector<float> eps_neg(eps.size());
+  for (size_t e = 0; e < eps.size(); ++e) eps_neg[e] = eps[e] / static_cast<float>(nsr);
+  vector<float> eonns = eps_neg;
+  SmallRng rng(seed);
+  for (int32_t epoch = 0; epoch < n_epochs; ++epoch) {
+    const float alpha = alpha0 * (1.0f - static_cast<float>(epoch) / static_cast<float>(n_epochs));
+    const float ef = static_cast<float>(epoch);
+    for (size_t e = 0; e < edges.size(); ++e) {
+      if (eons[e] > ef) continue;
+      const int64_t h = edges[e].head, t = edges[e].tail;
+      float* cur = head + h * d;
+      bool finite = true;
+      for (int64_t f = 0; f < d; ++f) finite &= std::isfinite(cur[f]);
+      if (fixed[h] || !finite) continue;
+      const float* other = tail + t * d;
+      const float gd = 1e-6f + dist_sequential(cur, other, d);
+      const float gc = -(1.0f / (edges[e].weight * sigmas[t] + 1e-6f));
+      for (int64_t f = 0; f < d; ++f) {
+        const float gdist = (cur[f] - other[f]) / gd;
+        cur[f] += clip4(gc * gdist) * alpha;
+      }
+      eons[e] += eps[e];
+      const float en = eps_neg[e];
+      if (!std::isfinite(en) || en <= 0.0f) continue;
+      const float fl = std::floor((ef - eonns[e]) / en);
+      const size_t nneg = fl > 0.0f ? static_cast<size_t>(fl) : 0;
+      for (size_t s = 0; s < nneg; ++s) {
+        const int64_t kk = static_cast<int64_t>(rng.gen_index(static_cast<uint64_t>(n)));
+        const float* o2 = tail + kk * d;
+        const float dn = dist_sequential(cur, o2, d);
+        const float gd2 = 1e-6f + dn;
+        const float wh = std::exp(-std::fmax(dn - rhos[kk], 1e-6f) / (sigmas[kk] + 1e-6f));
+        const float gc2 = -gamma * ((0.0f - wh) / ((1.0f - wh) * sigmas[kk] + 1e-6f));
+        for (int64_t f = 0; f < d; ++f) {
+          const float gdist = (cur[f] - o2[f]) / gd2;
+          cur[f] += clip4(gc2 * gdist) * alpha;
+        }
+      }
+      eonns[e] += static_cast<float>(nneg) * en;
+    }
+  }
+}
+
 /* lib.rs:5054-5055 */
 int32_t density_start(int32_t n_epochs, float frac) {
   const int64_t active = static_cast<int64_t>(std::ceil(static_cast<float>(n_epochs) * frac));
@@ -1551,6 +1774,35 @@ int32_t orc_transform(const orc_params* p, const float* x_train, int64_t n, int6
   optimize_layout_transform(out, m, dim, y_train, n, edges, n_epochs, a, b,
                             p->learning_rate / 4.0f, p->negative_sample_rate,
                             p->repulsion_strength, p->random_seed ^ 0xA24BAED40B53C117ULL, nt);
+  return 0;
+}
+
+/* inverse_transform_rows (lib.rs:1800-1915); optimize = 0 stops after
+ * init_inverse_points. */
+int32_t orc_inverse_transform(const orc_params* p, const float* x_train, int64_t n, int64_t d,
+                              const float* y_train, const float* sigmas, const float* rhos,
+                              float a, float b, const float* q, int64_t m, int32_t optimize,
+                              float* out) {
+  if (m == 0) return 0;
+  const int nt = p->n_threads > 0 ? p->n_threads : omp_get_max_threads();
+  const int dim = p->n_components;
+  const int32_t k = inverse_neighbor_count(p->n_neighbors, n);
+  const int32_t n_epochs = p->n_epochs > 0 ? std::max(p->n_epochs / 3, 1) : (m <= 10000 ? 100 : 30);
+  vector<int64_t> idx(static_cast<size_t>(m) * k);
+  vector<float> dist(static_cast<size_t>(m) * k);
+  knn_impl(q, m, y_train, n, dim, k, 0, nt, idx.data(), dist.data());
+  vector<float> raw, norm;
+  vector<char> fixed;
+  inverse_weights(dist.data(), m, k, a, b, raw, norm, fixed);
+  init_inverse_points(q, m, dim, idx.data(), norm, fixed, k, y_train, x_train, d, out);
+  if (!optimize) return 0;
+  vector<Edge> edges;
+  for (int64_t i = 0; i < m; ++i)
+    for (int32_t j = 0; j < k; ++j)
+      if (raw[i * k + j] > 0.0f) edges.push_back({i, idx[i * k + j], raw[i * k + j]});
+  optimize_layout_inverse(out, m, x_train, n, d, edges, fixed, sigmas, rhos, n_epochs,
+                          p->learning_rate / 4.0f, p->negative_sample_rate,
+                          p->repulsion_strength, p->random_seed ^ 0xD1B54A32D192ED03ULL);
   return 0;
 }
 

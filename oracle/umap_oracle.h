@@ -114,6 +114,13 @@ int32_t orc_fit(const orc_params* p, const float* x, int64_t n, int64_t d,
 int32_t orc_transform(const orc_params* p, const float* x_train, int64_t n,
                       int64_t d, const float* y_train, float a, float b,
                       const float* q, int64_t m, float* out);
+/* Inverse transform (lib.rs:1800-1915): embedded rows [m x n_components]
+ * back to the feature space [m x d]; optimize = 0 stops after the local
+ * linear initialisation (init_inverse_points, lib.rs:4838-4935). */
+int32_t orc_inverse_transform(const orc_params* p, const float* x_train, int64_t n,
+                              int64_t d, const float* y_train, const float* sigmas,
+                              const float* rhos, float a, float b, const float* q,
+                              int64_t m, int32_t optimize, float* out);
 
 #ifdef __cplusplus
 }

@@ -137,3 +137,13 @@ def mlp_batch_grad(params, mean, inv_std, x_rows, y_rows, d, h, dim, n_edges, n_
                                          L.ptr(x_rows), L.ptr(y_rows), n_edges, n_neg, a, b, lam,
                                          L.ptr(grads), ctypes.pointer(loss), L.ptr(z)))
     return loss.value, grads, z
+
+
+def inverse_init(x_train, y_train, q, k, a, b):
+    x_train, y_train, q = _f32(x_train), _f32(y_train), _f32(q)
+    n, d = x_train.shape
+    m, dim = q.shape
+    out = np.empty((m, d), np.float32)
+    L.check(L.lib().umapx_inverse_init(L.ptr(x_train), n, d, L.ptr(y_train), dim, k, a, b,
+                                       L.ptr(q), m, L.ptr(out)))
+    return out

@@ -107,3 +107,18 @@ def test_injected_device_fault_propagates(tmp_path):
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                          timeout=120)
     assert "RUNTIME device error: injected fault in stage alloc" in out.stdout, out.stderr
+
+
+def test_inverse_transform_requires_fit(umapx):
+    # lib.rs:6366-6373 and test_binding.py:1073-1093: NotFitted -> RuntimeError
+    from umapers_amd import Umap
+
+    m = Umap()
+    with pytest.raises(RuntimeError, match="not fitted"):
+        m.inverse_transform(np.zeros((1, 2), np.float32))
+    with pytest.raises(RuntimeError, match=r"model must be fit before inverse_transform\(out=\.\.\.\)"):
+        m.inverse_transform(np.zeros((1, 2), np.float32), out=np.empty((1, 2), np.float32))
+    h = m._core._h
+    rc = umapx.lib().umapx_inverse_transform(h, umapx.ptr(np.zeros((1, 2), np.float32)), 1, 2,
+                                             umapx.ptr(np.zeros((1, 2), np.float32)))
+    assert rc == umapx.ERR_NOT_FITTED

@@ -209,6 +209,18 @@ class _Core:
         self.transform_into(arr, out, mode)
         return out
 
+    def inverse_transform_into(self, arr: np.ndarray, out: np.ndarray) -> None:
+        L.check(self._lib.umapx_inverse_transform(self._h, L.ptr(arr), arr.shape[0],
+                                                  arr.shape[1], L.ptr(out)))
+
+    def inverse_transform(self, arr: np.ndarray) -> np.ndarray:
+        d = self.n_features
+        if d is None:
+            raise RuntimeError("model is not fitted yet")
+        out = np.empty((arr.shape[0], d), np.float32)
+        self.inverse_transform_into(arr, out)
+        return out
+
     @property
     def embedding(self) -> np.ndarray | None:
         n = ctypes.c_int64()
@@ -487,8 +499,20 @@ class Umap:
         the number of steps (raises when it has not been trained)."""
         return self._core.mlp_loss()
 
-    def inverse_transform(self, embedded_query: Any, *, out: np.ndarray | None = None):
-        raise NotImplementedError("inverse_transform is not available on the GPU path")
+    def inverse_transform(self, embedded_query: Any, *, out: np.ndarray | None = None) -> np.ndarray:
+        """_api.py:838-873: map embedded samples back to the original feature
+        space (kNN among the fitted embedding, a local linear estimate and the
+        feature-space SGD, lib.rs:1800-1915).  out: optional writable float32
+        buffer of shape (n_samples, n_features); the model must be fit first."""
+        arr = _as_f32_matrix(embedded_query, "embedded_query")
+        if out is None:
+            return self._core.inverse_transform(arr)
+        n_features = self._core.n_features
+        if n_features is None:
+            raise RuntimeError("model must be fit before inverse_transform(out=...)")
+        out_buf = _as_out_buffer(out, (arr.shape[0], n_features))
+        self._core.inverse_transform_into(arr, out_buf)
+        return out_buf
 
     @property
     def ab_params(self) -> tuple[float, float] | None:

@@ -127,6 +127,13 @@ SIGNATURES = {
     "umapx_transform_learned": (ctypes.c_int32, [_vp, _f32p, ctypes.c_int64, ctypes.c_int64, _f32p]),
     "umapx_transform_learned_device": (ctypes.c_int32, [
         _vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp]),
+    "umapx_inverse_transform": (ctypes.c_int32, [_vp, _f32p, ctypes.c_int64, ctypes.c_int64,
+                                                 _f32p]),
+    "umapx_inverse_transform_device": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int64,
+                                                        _vp]),
+    "umapx_inverse_init": (ctypes.c_int32, [_f32p, ctypes.c_int64, ctypes.c_int64, _f32p,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                            ctypes.c_float, _f32p, ctypes.c_int64, _f32p]),
     "umapx_n_samples": (ctypes.c_int32, [_vp, _i64p]),
     "umapx_n_features": (ctypes.c_int32, [_vp, _i64p]),
     "umapx_get_embedding": (ctypes.c_int32, [_vp, _f32p]),

@@ -24,6 +24,7 @@
 #include "common.hpp"
 #include "graph.hpp"
 #include "hostmath.hpp"
+#include "inverse.hpp"
 #include "knn.hpp"
 #include "mlp.hpp"
 #include "sgd.hpp"
@@ -94,10 +95,9 @@ __global__ void first_nonfinite_kernel(const float* __restrict__ x, int64_t tota
   if (!isfinite(x[i])) atomicMin(first_row, static_cast<unsigned long long>(i / d));
 }
 
-void validate_data_device(const float* x, int64_t n, int64_t d, hipStream_t s) {
-  if (n == 0) throw Error(kEmptyData, "input data is empty");
-  if (n < 2) throw Error(kNeedTwoSamples, "need at least two samples");
-  if (d == 0) throw_invalid("input rows must have at least one feature");
+// Rejects the first row of x [n x d] (device) holding a non-finite value.
+void check_finite_device(const float* x, int64_t n, int64_t d, hipStream_t s) {
+  if (n == 0 || d == 0) return;
   DevBuf<unsigned long long> first(1);
   const unsigned long long init = std::numeric_limits<unsigned long long>::max();
   HIP_CHECK(hipMemcpyAsync(first.get(), &init, sizeof(init), hipMemcpyHostToDevice, s));
@@ -108,6 +108,13 @@ void validate_data_device(const float* x, int64_t n, int64_t d, hipStream_t s) {
   HIP_CHECK(hipMemcpyAsync(&row, first.get(), sizeof(row), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   if (row != init) throw_invalid("input contains non-finite value at row " + std::to_string(row));
+}
+
+void validate_data_device(const float* x, int64_t n, int64_t d, hipStream_t s) {
+  if (n == 0) throw Error(kEmptyData, "input data is empty");
+  if (n < 2) throw Error(kNeedTwoSamples, "need at least two samples");
+  if (d == 0) throw_invalid("input rows must have at least one feature");
+  check_finite_device(x, n, d, s);
 }
 
 // validate_params (lib.rs:2140-2228) for the dense path.
@@ -477,6 +484,49 @@ struct umapx_handle {
     HIP_CHECK(hipStreamSynchronize(stream));
   }
 
+  // inverse_transform_rows (lib.rs:1800-1915) for device rows d_q [m x n_components].
+  void inverse_device(const float* d_q, int64_t m, float* d_out) {
+    ensure_stream();
+    AllocStreamScope alloc_scope(stream);
+    InverseArgs ia;
+    ia.k = static_cast<int>(std::max<int64_t>(std::min<int64_t>(p.n_neighbors, n), 1));
+    ia.n_epochs = p.n_epochs > 0 ? std::max(p.n_epochs / 3, 1) : (m <= 10000 ? 100 : 30);
+    ia.a = a;
+    ia.b = b;
+    ia.learning_rate = p.learning_rate / 4.0f;
+    ia.negative_sample_rate = p.negative_sample_rate;
+    ia.repulsion_strength = p.repulsion_strength;
+    ia.seed = p.random_seed ^ 0xD1B54A32D192ED03ULL;
+    inverse_transform_device(d_q, m, p.n_components, y.get(), x.get(), n, d, sig.get(), rho.get(),
+                             ia, d_out, kws, stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  // lib.rs:1803-1847: fitted model, then per-row width and finiteness.
+  void check_embedded(int64_t m, int64_t dq) {
+    if (!fitted) throw Error(kNotFitted, "model is not fitted yet");
+    if (m > 0 && dq != p.n_components)
+      throw Error(kEmbeddingDimensionMismatch,
+                  "embedding dimension mismatch: expected " + std::to_string(p.n_components) +
+                      ", got " + std::to_string(dq));
+  }
+
+  void inverse_host(const float* q, int64_t m, int64_t dq, float* out) {
+    check_embedded(m, dq);
+    if (m == 0) return;
+    for (int64_t r = 0; r < m; ++r)
+      for (int64_t c = 0; c < dq; ++c)
+        if (!std::isfinite(q[r * dq + c]))
+          throw_invalid("input contains non-finite value at row " + std::to_string(r));
+    ensure_stream();
+    AllocStreamScope alloc_scope(stream);
+    DevBuf<float> dqb(static_cast<size_t>(m) * dq), dout(static_cast<size_t>(m) * d);
+    HIP_CHECK(hipMemcpyAsync(dqb.get(), q, 4 * m * dq, hipMemcpyHostToDevice, stream));
+    inverse_device(dqb.get(), m, dout.get());
+    HIP_CHECK(hipMemcpyAsync(out, dout.get(), 4 * m * d, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
   void run_transform(const float* d_q, int64_t m, float* d_out, int mode) {
     AllocStreamScope alloc_scope(stream);
     if (mode == UMAPX_TRANSFORM_LEARNED) {
@@ -676,8 +726,59 @@ int32_t umapx_transform_device(umapx_handle* h, const float* d_q, int64_t m, int
     h->check_query(m, d);
     if (m == 0) return;
     h->ensure_stream();
-    validate_data_device(d_q, std::max<int64_t>(m, 2), d, h->stream);
+    check_finite_device(d_q, m, d, h->stream);
     h->run_transform(d_q, m, d_out, h->p.transform_mode);
+  });
+}
+
+int32_t umapx_inverse_transform(umapx_handle* h, const float* q, int64_t m, int64_t dim,
+                                float* out) {
+  return guarded([&] {
+    require_handle(h);
+    if (m > 0 && (!q || !out)) throw_invalid("null argument");
+    h->inverse_host(q, m, dim, out);
+  });
+}
+
+int32_t umapx_inverse_transform_device(umapx_handle* h, const float* d_q, int64_t m, int64_t dim,
+                                       float* d_out) {
+  return guarded([&] {
+    require_handle(h);
+    h->check_embedded(m, dim);
+    if (m == 0) return;
+    h->ensure_stream();
+    check_finite_device(d_q, m, dim, h->stream);
+    h->inverse_device(d_q, m, d_out);
+  });
+}
+
+int32_t umapx_inverse_init(const float* x_train, int64_t n, int64_t d, const float* y_train,
+                           int32_t dim, int32_t k, float a, float b, const float* q, int64_t m,
+                           float* out) {
+  return guarded([&] {
+    if (m == 0) return;
+    if (!x_train || !y_train || !q || !out) throw_invalid("null argument");
+    if (n < 1 || d < 1 || dim < 1) throw_invalid("need n >= 1, d >= 1 and dim >= 1");
+    umapx_handle::StreamOwner so;
+    HIP_CHECK(hipStreamCreateWithFlags(&so.s, hipStreamNonBlocking));
+    {
+      AllocStreamScope alloc_scope(so.s);
+      DevBuf<float> dx(static_cast<size_t>(n) * d), dy(static_cast<size_t>(n) * dim),
+          dq(static_cast<size_t>(m) * dim), dout(static_cast<size_t>(m) * d);
+      HIP_CHECK(hipMemcpyAsync(dx.get(), x_train, 4 * n * d, hipMemcpyHostToDevice, so.s));
+      HIP_CHECK(hipMemcpyAsync(dy.get(), y_train, 4 * n * dim, hipMemcpyHostToDevice, so.s));
+      HIP_CHECK(hipMemcpyAsync(dq.get(), q, 4 * m * dim, hipMemcpyHostToDevice, so.s));
+      InverseArgs ia;
+      ia.k = static_cast<int>(std::max<int64_t>(std::min<int64_t>(k, n), 1));
+      ia.n_epochs = 0;
+      ia.a = a;
+      ia.b = b;
+      KnnWorkspace kws;
+      inverse_transform_device(dq.get(), m, dim, dy.get(), dx.get(), n, d, nullptr, nullptr, ia,
+                               dout.get(), kws, so.s);
+      HIP_CHECK(hipMemcpyAsync(out, dout.get(), 4 * m * d, hipMemcpyDeviceToHost, so.s));
+      HIP_CHECK(hipStreamSynchronize(so.s));
+    }
   });
 }
 
