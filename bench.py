@@ -15,6 +15,9 @@ in HBM, through the C ABI (umapx_fit_transform_device).
                    on 1000 sampled rows (self included)
   transform_rows_per_sec   plain transform (kNN into the 200k training rows +
                    per-query SGD) of 20,000 fresh rows, device resident
+  inverse_transform  those 20,000 embedded rows mapped back to 128-D (kNN in
+                   the embedding, local linear estimate, feature-space SGD):
+                   rows/s and the RMSE against the rows they came from
   learned_transform  the 100,000 x 192 learned-transform configuration: fit +
                    encoder training, encoder rows/s over 1M fresh rows, and the
                    plain transform on 10,000 of them as judge (per-row
@@ -229,6 +232,25 @@ def main() -> int:
     xform_rps = m / float(np.median(tt))
     log(f"bench: plain transform {m} rows in {np.median(tt) * 1e3:.1f} ms")
 
+    # ---- inverse transform of those embedded rows back to 128-D -------------
+    xi_d = torch.empty((m, d), dtype=torch.float32, device=dev)
+    L.check(lib.umapx_inverse_transform_device(h16, ctypes.c_void_p(yq_d.data_ptr()), m, 2,
+                                               ctypes.c_void_p(xi_d.data_ptr())))
+    torch.cuda.synchronize()
+    ti = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        L.check(lib.umapx_inverse_transform_device(h16, ctypes.c_void_p(yq_d.data_ptr()), m, 2,
+                                                   ctypes.c_void_p(xi_d.data_ptr())))
+        torch.cuda.synchronize()
+        ti.append(time.perf_counter() - t0)
+    rec = xi_d.cpu().numpy()
+    inverse = {"rows": m, "rows_per_sec": round(m / float(np.median(ti)), 1),
+               "rmse_vs_query_rows": round(float(np.sqrt(np.mean((rec - q) ** 2))), 4),
+               "query_rms_about_mean": round(float(np.sqrt(np.mean((q - q.mean(0)) ** 2))), 4)}
+    log(f"bench: inverse transform {m} rows in {np.median(ti) * 1e3:.1f} ms, "
+        f"rmse {inverse['rmse_vs_query_rows']} (rows' own rms {inverse['query_rms_about_mean']})")
+
     # ---- learned transform: 100000 x 192 fit, encoder over fresh queries ----
     mlp = None
     if args.mlp_queries > 0:
@@ -333,6 +355,7 @@ def main() -> int:
         "trustworthiness_15": round(trust, 5),
         "knn_preservation_15": round(kpres, 5),
         "transform_rows_per_sec": round(xform_rps, 1),
+        "inverse_transform": inverse,
         "learned_transform": mlp,
         "stage_sec": {key: round(prof[key], 4) for key in L.TIMING_KEYS},
         "fp32_knn_kernel_ms": round(float(np.median([t["knn_kernel_ms"] for t in tm32])), 3),

@@ -29,7 +29,6 @@ namespace {
 
 constexpr int kInvWaves = 4;     // queries (waves) per block
 constexpr int kMaxLowDim = 8;    // largest n_components
-constexpr int kInvPer = static_cast<int>(kInverseMaxFeatures / 64);  // features per lane
 
 __device__ __forceinline__ uint64_t inv_mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
@@ -295,6 +294,12 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_init_kernel(
     }
 }
 
+// PER features per lane (d <= 64 * PER).  The rows of up to kNegBatch
+// negatives are gathered before the first of them is applied, so their loads
+// overlap; the moves are then applied one after another as the reference does.
+constexpr int kNegBatch = 4;
+
+template <int PER>
 __global__ __launch_bounds__(64 * kInvWaves) void inverse_sgd_kernel(
     float* __restrict__ out, int64_t m, const float* __restrict__ x_train, int64_t d,
     uint32_t n, const int32_t* __restrict__ idx, const float* __restrict__ raw,
@@ -305,9 +310,9 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_sgd_kernel(
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kInvWaves + (threadIdx.x >> 6);
   if (i >= m || fixed[i]) return;  // the whole wave
   float* o = out + i * d;
-  float cur[kInvPer];
+  float cur[PER];
 #pragma unroll
-  for (int t = 0; t < kInvPer; ++t) {
+  for (int t = 0; t < PER; ++t) {
     const int64_t f = lane + 64 * t;
     cur[t] = f < d ? o[f] : 0.f;
   }
@@ -333,15 +338,15 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_sgd_kernel(
       if (lane_f(eons_l, j) > ef) continue;
       bool fin = true;
 #pragma unroll
-      for (int t = 0; t < kInvPer; ++t) fin = fin && isfinite(cur[t]);
+      for (int t = 0; t < PER; ++t) fin = fin && isfinite(cur[t]);
       if (__ballot(!fin) != 0ULL) continue;
       const int32_t tj = __builtin_amdgcn_readlane(t_l, j);
       const float wj = lane_f(w_l, j);
       const float* other = x_train + static_cast<int64_t>(tj) * d;
-      float ov[kInvPer];
+      float ov[PER];
       float part = 0.f;
 #pragma unroll
-      for (int t = 0; t < kInvPer; ++t) {
+      for (int t = 0; t < PER; ++t) {
         const int64_t f = lane + 64 * t;
         ov[t] = f < d ? other[f] : 0.f;
         const float df = cur[t] - ov[t];
@@ -350,7 +355,7 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_sgd_kernel(
       const float gd = 1e-6f + sqrtf(wave_sum(part));
       const float gc = -(1.f / (wj * sigmas[tj] + 1e-6f));
 #pragma unroll
-      for (int t = 0; t < kInvPer; ++t)
+      for (int t = 0; t < PER; ++t)
         if (lane + 64 * t < d) cur[t] += inv_clip4(gc * ((cur[t] - ov[t]) / gd)) * alpha;
       if (lane == j) eons_l = fadd(eons_l, eps_l);
       const float en = fdiv(lane_f(eps_l, j), nsr);
@@ -359,31 +364,50 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_sgd_kernel(
       const int nneg = fl > 0.f ? static_cast<int>(fl) : 0;
       const uint64_t key = inv_mix64(key_head + static_cast<uint64_t>(epoch) * 0xD1B54A32D192ED03ULL +
                                      static_cast<uint64_t>(j));
-      for (int s = 0; s < nneg; ++s) {
-        const uint32_t kk = inv_draw(key, static_cast<uint32_t>(s), n);
-        const float* o2 = x_train + static_cast<int64_t>(kk) * d;
-        float p2 = 0.f;
+      for (int s0 = 0; s0 < nneg; s0 += kNegBatch) {
+        float nv[kNegBatch][PER];
+        float sk[kNegBatch], rk[kNegBatch];
 #pragma unroll
-        for (int t = 0; t < kInvPer; ++t) {
-          const int64_t f = lane + 64 * t;
-          ov[t] = f < d ? o2[f] : 0.f;
-          const float df = cur[t] - ov[t];
-          p2 += df * df;
+        for (int u = 0; u < kNegBatch; ++u) {
+          sk[u] = 1.f;
+          rk[u] = 0.f;
+#pragma unroll
+          for (int t = 0; t < PER; ++t) nv[u][t] = 0.f;
+          if (s0 + u < nneg) {
+            const uint32_t kk = inv_draw(key, static_cast<uint32_t>(s0 + u), n);
+            const float* o2 = x_train + static_cast<int64_t>(kk) * d;
+#pragma unroll
+            for (int t = 0; t < PER; ++t) {
+              const int64_t f = lane + 64 * t;
+              if (f < d) nv[u][t] = o2[f];
+            }
+            sk[u] = sigmas[kk];
+            rk[u] = rhos[kk];
+          }
         }
-        const float dn = sqrtf(wave_sum(p2));
-        const float gd2 = 1e-6f + dn;
-        const float sk = sigmas[kk];
-        const float wh = expf(-fmaxf(dn - rhos[kk], 1e-6f) / (sk + 1e-6f));
-        const float gc2 = -gamma * ((0.f - wh) / ((1.f - wh) * sk + 1e-6f));
 #pragma unroll
-        for (int t = 0; t < kInvPer; ++t)
-          if (lane + 64 * t < d) cur[t] += inv_clip4(gc2 * ((cur[t] - ov[t]) / gd2)) * alpha;
+        for (int u = 0; u < kNegBatch; ++u) {
+          if (s0 + u >= nneg) break;
+          float p2 = 0.f;
+#pragma unroll
+          for (int t = 0; t < PER; ++t) {
+            const float df = cur[t] - nv[u][t];
+            p2 += df * df;
+          }
+          const float dn = sqrtf(wave_sum(p2));
+          const float gd2 = 1e-6f + dn;
+          const float wh = expf(-fmaxf(dn - rk[u], 1e-6f) / (sk[u] + 1e-6f));
+          const float gc2 = -gamma * ((0.f - wh) / ((1.f - wh) * sk[u] + 1e-6f));
+#pragma unroll
+          for (int t = 0; t < PER; ++t)
+            if (lane + 64 * t < d) cur[t] += inv_clip4(gc2 * ((cur[t] - nv[u][t]) / gd2)) * alpha;
+        }
       }
       if (lane == j) eonns_l = fadd(eonns_l, fmul(static_cast<float>(nneg), en));
     }
   }
 #pragma unroll
-  for (int t = 0; t < kInvPer; ++t) {
+  for (int t = 0; t < PER; ++t) {
     const int64_t f = lane + 64 * t;
     if (f < d) o[f] = cur[t];
   }
@@ -422,12 +446,22 @@ void inverse_transform_device(const float* q, int64_t m, int dim, const float* y
                        idx.get(), norm.get(), fixed.get(), k, y_train, x_train, d, out);
   KERNEL_CHECK();
   if (args.n_epochs > 0) {
-    hipLaunchKernelGGL(inverse_sgd_kernel, grid, dim3(64 * kInvWaves), 0, s, out, m, x_train, d,
-                       static_cast<uint32_t>(n), idx.get(), raw.get(), fixed.get(), k, wmax.get(),
-                       sigmas, rhos, args.n_epochs, args.learning_rate,
-                       static_cast<float>(args.negative_sample_rate), args.repulsion_strength,
-                       args.seed);
-    KERNEL_CHECK();
+    auto sgd = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, dim3(64 * kInvWaves), 0, s, out, m, x_train, d,
+                         static_cast<uint32_t>(n), idx.get(), raw.get(), fixed.get(), k,
+                         wmax.get(), sigmas, rhos, args.n_epochs, args.learning_rate,
+                         static_cast<float>(args.negative_sample_rate), args.repulsion_strength,
+                         args.seed);
+      KERNEL_CHECK();
+    };
+    if (d <= 64)
+      sgd(inverse_sgd_kernel<1>);
+    else if (d <= 128)
+      sgd(inverse_sgd_kernel<2>);
+    else if (d <= 256)
+      sgd(inverse_sgd_kernel<4>);
+    else
+      sgd(inverse_sgd_kernel<8>);
   }
   // the temporaries above are released in stream order after the launches
 }
