@@ -35,6 +35,12 @@ extern "C" {
 
 #define UMAPX_METRIC_EUCLIDEAN 0
 #define UMAPX_METRIC_COSINE 1
+#define UMAPX_METRIC_MANHATTAN 2
+#define UMAPX_METRIC_CHEBYSHEV 3
+#define UMAPX_METRIC_MINKOWSKI 4  /* p in umapx_params.metric_p */
+#define UMAPX_METRIC_CORRELATION 5
+#define UMAPX_METRIC_CANBERRA 6
+#define UMAPX_METRIC_BRAYCURTIS 7
 
 #define UMAPX_INIT_RANDOM 0
 #define UMAPX_INIT_SPECTRAL 1
@@ -81,6 +87,7 @@ typedef struct umapx_params {
   int32_t mlp_negatives;        /* negatives per edge (5) */
   float mlp_learning_rate;      /* Adam step size (1e-3) */
   float mlp_embedding_weight;   /* weight of the MSE to the fitted embedding (1.0) */
+  float metric_p;               /* p of UMAPX_METRIC_MINKOWSKI (2.0) */
 } umapx_params;
 
 /* FitTimingBreakdown (rust_umap/src/lib.rs:51-68) plus GPU kernel figures. */
@@ -179,6 +186,11 @@ int32_t umapx_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metri
 int32_t umapx_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,
                                int64_t d, int32_t k, int32_t metric, int32_t precision,
                                int32_t rerank_factor, int64_t* idx, float* dist);
+/* exact_nearest_neighbors_by / _to_reference_by (lib.rs:2891-2928, 3346-3399)
+ * under any UMAPX_METRIC_* code with minkowski's p (self included when q == r
+ * and m == n). */
+int32_t umapx_knn_metric(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
+                         int32_t k, int32_t metric, float metric_p, int64_t* idx, float* dist);
 
 /* knn_rows_include_self (lib.rs:3402-3417): writes 0/1. */
 int32_t umapx_knn_rows_include_self(const int64_t* idx, const float* dist, int64_t n, int32_t k,

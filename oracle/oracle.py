@@ -44,6 +44,7 @@ class OrcParams(ctypes.Structure):
         ("dens_var_shift", ctypes.c_float),
         ("output_dens", ctypes.c_int32),
         ("n_threads", ctypes.c_int32),
+        ("metric_p", ctypes.c_float),
     ]
 
 
@@ -124,6 +125,10 @@ def lib() -> ctypes.CDLL:
                                     ctypes.c_int64, _f32p, ctypes.c_float, ctypes.c_float,
                                     _f32p, ctypes.c_int64, _f32p]
         L.orc_transform.restype = ctypes.c_int32
+        L.orc_knn_metric.argtypes = [_f32p, ctypes.c_int64, _f32p, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                     ctypes.c_int32, _i64p, _f32p]
+        L.orc_knn_metric.restype = None
         L.orc_inverse_transform.argtypes = [ctypes.POINTER(OrcParams), _f32p, ctypes.c_int64,
                                             ctypes.c_int64, _f32p, _f32p, _f32p, ctypes.c_float,
                                             ctypes.c_float, _f32p, ctypes.c_int64, ctypes.c_int32,
@@ -145,7 +150,8 @@ def _i64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.int64)
 
 
-METRICS = {"euclidean": 0, "cosine": 1}
+METRICS = {"euclidean": 0, "cosine": 1, "manhattan": 2, "chebyshev": 3, "minkowski": 4,
+           "correlation": 5, "canberra": 6, "braycurtis": 7}
 
 
 def find_ab_params(spread: float, min_dist: float) -> tuple[float, float]:
@@ -162,6 +168,18 @@ def knn(x, k: int, metric: str = "euclidean", n_threads: int = 0):
     dist = np.empty((n, k), np.float32)
     lib().orc_knn(_p(x, _f32p), n, d, k, METRICS[metric], n_threads, _p(idx, _i64p),
                   _p(dist, _f32p))
+    return idx, dist
+
+
+def knn_metric(q, r, k: int, metric: str, p: float = 2.0, n_threads: int = 0):
+    """Exact kNN of q among r under any of the reference's metrics
+    (lib.rs:2891-2928; self included when q is r)."""
+    q, r = _f32(q), _f32(r)
+    m, d = q.shape
+    idx = np.empty((m, k), np.int64)
+    dist = np.empty((m, k), np.float32)
+    lib().orc_knn_metric(_p(q, _f32p), m, _p(r, _f32p), r.shape[0], d, k, METRICS[metric], p,
+                         n_threads, _p(idx, _i64p), _p(dist, _f32p))
     return idx, dist
 
 
@@ -328,6 +346,7 @@ class OracleParams:
     dens_var_shift: float = 0.1
     output_dens: bool = False
     n_threads: int = 0
+    metric_p: float = 2.0
 
     def c(self) -> OrcParams:
         return OrcParams(
@@ -336,7 +355,8 @@ class OracleParams:
             self.local_connectivity, self.set_op_mix_ratio, self.repulsion_strength,
             self.negative_sample_rate, 1 if self.init == "spectral" else 0,
             self.random_seed & 0xFFFFFFFFFFFFFFFF, int(self.densmap), self.dens_lambda,
-            self.dens_frac, self.dens_var_shift, int(self.output_dens), self.n_threads)
+            self.dens_frac, self.dens_var_shift, int(self.output_dens), self.n_threads,
+            self.metric_p)
 
 
 @dataclass

@@ -245,6 +245,98 @@ inline float cosine_with_norms(const float* x, const float* y, size_t d,
   return 1.0f - c;
 }
 
+/* lib.rs:2494-2541 manhattan_distance: scalar for d < 64; for d >= 64 eight
+ * AVX2 lane sums of |x - y|, added in lane order, then the scalar tail. */
+inline float manhattan(const float* x, const float* y, size_t d) {
+  if (d >= 64) {
+    const __m256 sign = _mm256_set1_ps(-0.0f);
+    __m256 acc = _mm256_setzero_ps();
+    size_t i = 0;
+    for (; i + 8 <= d; i += 8) {
+      const __m256 diff = _mm256_sub_ps(_mm256_loadu_ps(x + i), _mm256_loadu_ps(y + i));
+      acc = _mm256_add_ps(acc, _mm256_andnot_ps(sign, diff));
+    }
+    alignas(32) float lanes[8];
+    _mm256_store_ps(lanes, acc);
+    float sum = 0.0f;
+    for (int l = 0; l < 8; ++l) sum += lanes[l];
+    for (; i < d; ++i) sum += std::fabs(x[i] - y[i]);
+    return sum;
+  }
+  float acc = 0.0f;
+  for (size_t i = 0; i < d; ++i) acc += std::fabs(x[i] - y[i]);
+  return acc;
+}
+
+/* lib.rs:2543-2552 chebyshev_distance (f32::max ignores NaN, as fmax does) */
+inline float chebyshev(const float* x, const float* y, size_t d) {
+  float m = 0.0f;
+  for (size_t i = 0; i < d; ++i) m = std::fmax(m, std::fabs(x[i] - y[i]));
+  return m;
+}
+
+/* lib.rs:2554-2563 minkowski_distance */
+inline float minkowski(const float* x, const float* y, size_t d, float p) {
+  float acc = 0.0f;
+  for (size_t i = 0; i < d; ++i) acc += std::pow(std::fabs(x[i] - y[i]), p);
+  return std::pow(acc, 1.0f / p);
+}
+
+/* lib.rs:2565-2577 canberra_distance */
+inline float canberra(const float* x, const float* y, size_t d) {
+  float acc = 0.0f;
+  for (size_t i = 0; i < d; ++i) {
+    const float den = std::fabs(x[i]) + std::fabs(y[i]);
+    if (den > 0.0f) acc += std::fabs(x[i] - y[i]) / den;
+  }
+  return acc;
+}
+
+/* lib.rs:2579-2594 bray_curtis_distance */
+inline float braycurtis(const float* x, const float* y, size_t d) {
+  float num = 0.0f, den = 0.0f;
+  for (size_t i = 0; i < d; ++i) {
+    num += std::fabs(x[i] - y[i]);
+    den += std::fabs(x[i] + y[i]);
+  }
+  return den == 0.0f ? 0.0f : num / den;
+}
+
+/* lib.rs:2655-2698 correlation_distance and
+ * correlation_distance_from_summaries */
+inline float correlation(const float* x, const float* y, size_t d) {
+  float sx = 0.0f, sy = 0.0f, qx = 0.0f, qy = 0.0f, dt = 0.0f;
+  for (size_t i = 0; i < d; ++i) {
+    sx += x[i];
+    sy += y[i];
+    qx += x[i] * x[i];
+    qy += y[i] * y[i];
+    dt += x[i] * y[i];
+  }
+  const float nf = static_cast<float>(d);
+  const float cdot = dt - (sx * sy / nf);
+  const float xv = std::fmax(qx - (sx * sx / nf), 0.0f);
+  const float yv = std::fmax(qy - (sy * sy / nf), 0.0f);
+  const float xn = std::sqrt(xv), yn = std::sqrt(yv);
+  if (xn == 0.0f && yn == 0.0f) return 0.0f;
+  if (xn == 0.0f || yn == 0.0f) return 1.0f;
+  const float c = std::min(std::max(cdot / (xn * yn), -1.0f), 1.0f);
+  return 1.0f - c;
+}
+
+/* metric codes: 0 euclidean, 1 cosine, 2 manhattan, 3 chebyshev,
+ * 4 minkowski (p), 5 correlation, 6 canberra, 7 braycurtis */
+inline float metric_distance(int32_t metric, const float* x, const float* y, size_t d, float p) {
+  switch (metric) {
+    case 2: return manhattan(x, y, d);
+    case 3: return chebyshev(x, y, d);
+    case 4: return minkowski(x, y, d, p);
+    case 5: return correlation(x, y, d);
+    case 6: return canberra(x, y, d);
+    default: return braycurtis(x, y, d);
+  }
+}
+
 /* f32::total_cmp key */
 inline int32_t total_key(float v) {
   int32_t b;
@@ -284,7 +376,7 @@ struct TopK {
 /* lib.rs:2863-2928, 3283-3399 exact kNN (rows under rayon -> OpenMP) */
 void knn_impl(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
               int32_t k, int32_t metric, int32_t n_threads, int64_t* out_idx,
-              float* out_dist) {
+              float* out_dist, float metric_p = 2.0f) {
   vector<float> qn, rn;
   if (metric == 1) {
     qn.resize(m);
@@ -300,6 +392,9 @@ void knn_impl(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
     if (metric == 1) {
       for (int64_t j = 0; j < n; ++j)
         top.push(j, cosine_with_norms(x, r + j * d, d, qn[i], rn[j]));
+    } else if (metric >= 2) {
+      /* lib.rs:2891-2909 exact_nearest_neighbors_by: the metric value itself */
+      for (int64_t j = 0; j < n; ++j) top.push(j, metric_distance(metric, x, r + j * d, d, metric_p));
     } else {
       for (int64_t j = 0; j < n; ++j) top.push(j, sq_dist(x, r + j * d, d));
       for (auto& v : top.dist) v = std::sqrt(v); /* lib.rs:2829-2834 */
@@ -1536,6 +1631,12 @@ void orc_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
   knn_impl(x, n, x, n, d, k, metric, n_threads, idx, dist);
 }
 
+void orc_knn_metric(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int32_t k,
+                    int32_t metric, float metric_p, int32_t n_threads, int64_t* idx,
+                    float* dist) {
+  knn_impl(q, m, r, n, d, k, metric, n_threads, idx, dist, metric_p);
+}
+
 void orc_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,
                           int64_t d, int32_t k, int32_t metric, int32_t n_threads,
                           int64_t* idx, float* dist) {
@@ -1671,7 +1772,7 @@ int32_t orc_fit(const orc_params* p, const float* x, int64_t n, int64_t d,
   vector<float> dist(static_cast<size_t>(n) * k);
   double s = now_sec();
   if (knn_idx == nullptr) {
-    knn_impl(x, n, x, n, d, k, p->metric, nt, idx.data(), dist.data());
+    knn_impl(x, n, x, n, d, k, p->metric, nt, idx.data(), dist.data(), p->metric_p);
   }
   tm.knn_sec = now_sec() - s;
   const int32_t n_epochs = p->n_epochs > 0 ? p->n_epochs : (n <= 10000 ? 500 : 200);
@@ -1762,7 +1863,7 @@ int32_t orc_transform(const orc_params* p, const float* x_train, int64_t n, int6
   const int32_t n_epochs = p->n_epochs > 0 ? p->n_epochs : (m <= 10000 ? 100 : 30);
   vector<int64_t> idx(static_cast<size_t>(m) * k);
   vector<float> dist(static_cast<size_t>(m) * k);
-  knn_impl(q, m, x_train, n, d, k, p->metric, nt, idx.data(), dist.data());
+  knn_impl(q, m, x_train, n, d, k, p->metric, nt, idx.data(), dist.data(), p->metric_p);
   vector<float> sig(m), rho(m);
   const float lc = std::max(p->local_connectivity - 1.0f, 0.0f);
   smooth_knn(dist.data(), m, k, static_cast<float>(k), lc, DEFAULT_BANDWIDTH, false,

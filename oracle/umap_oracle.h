@@ -23,7 +23,9 @@ typedef struct orc_params {
   int32_t n_neighbors;
   int32_t n_components;
   int32_t n_epochs;            /* <= 0: default (500 if n <= 10000 else 200) */
-  int32_t metric;              /* 0 euclidean, 1 cosine */
+  int32_t metric;              /* 0 euclidean, 1 cosine, 2 manhattan, 3 chebyshev,
+                                  4 minkowski, 5 correlation, 6 canberra,
+                                  7 braycurtis */
   float learning_rate;
   float min_dist;
   float spread;
@@ -39,6 +41,7 @@ typedef struct orc_params {
   float dens_var_shift;
   int32_t output_dens;
   int32_t n_threads;           /* threads for the stages rayon runs in parallel */
+  float metric_p;              /* minkowski p */
 } orc_params;
 
 /* Same keys as FitTimingBreakdown (rust_umap/src/lib.rs:51-68). */
@@ -53,6 +56,11 @@ void orc_find_ab_params(float spread, float min_dist, float* a, float* b);
 
 void orc_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
              int32_t n_threads, int64_t* idx, float* dist);
+/* Exact kNN of q [m x d] among r [n x d] under any metric code (self
+ * included when q == r); metric_p is minkowski's p. */
+void orc_knn_metric(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
+                    int32_t k, int32_t metric, float metric_p, int32_t n_threads,
+                    int64_t* idx, float* dist);
 void orc_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,
                           int64_t d, int32_t k, int32_t metric,
                           int32_t n_threads, int64_t* idx, float* dist);

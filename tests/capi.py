@@ -147,3 +147,17 @@ def inverse_init(x_train, y_train, q, k, a, b):
     L.check(L.lib().umapx_inverse_init(L.ptr(x_train), n, d, L.ptr(y_train), dim, k, a, b,
                                        L.ptr(q), m, L.ptr(out)))
     return out
+
+
+METRIC_CODES = {"euclidean": 0, "cosine": 1, "manhattan": 2, "chebyshev": 3, "minkowski": 4,
+                "correlation": 5, "canberra": 6, "braycurtis": 7}
+
+
+def knn_metric(q, r, k, metric, p=2.0):
+    q, r = _f32(q), _f32(r)
+    m, d = q.shape
+    idx = np.empty((m, k), np.int64)
+    dist = np.empty((m, k), np.float32)
+    L.check(L.lib().umapx_knn_metric(L.ptr(q), m, L.ptr(r), r.shape[0], d, k,
+                                     METRIC_CODES[metric], p, L.ptr(idx, L._i64p), L.ptr(dist)))
+    return idx, dist

@@ -63,8 +63,11 @@ def test_defaults_and_get_set_params(fake):
 
 
 @pytest.mark.parametrize("kw,msg", [
-    ({"metric": "manhattan"}, "not available on the GPU path"),
     ({"metric": "bogus"}, "unsupported metric 'bogus'"),
+    ({"metric": "minkowski"}, r"requires metric_kwds=\{'p': \.\.\.\}"),
+    ({"metric": "minkowski", "metric_kwds": {"p": -1.0}}, "must be finite and > 0 for minkowski"),
+    ({"metric": "minkowski", "metric_kwds": {"p": 3, "w": 1}}, "unsupported metric_kwds for minkowski: w"),
+    ({"metric": "manhattan", "metric_kwds": {"p": 1}}, "unsupported metric_kwds for manhattan: p"),
     ({"ann_mode": "fast"}, "unsupported ann_mode 'fast'"),
     ({"dens_lambda": -1.0}, "dens_lambda must be finite and >= 0"),
     ({"init": "pca"}, "unsupported init 'pca'"),
@@ -80,6 +83,14 @@ def test_constructor_errors(fake, kw, msg):
 def test_metric_aliases(fake):
     assert Umap(metric="L2").metric == "euclidean"
     assert Umap(metric="cosine").metric == "cosine"
+    # _api.py:242-263 aliases; all eight metrics reach the C parameters
+    for alias, name, code in [("l1", "manhattan", 2), ("linf", "chebyshev", 3),
+                              ("CORRELATION", "correlation", 5), ("canberra", "canberra", 6),
+                              ("bray_curtis", "braycurtis", 7)]:
+        m = Umap(metric=alias)
+        assert m.metric == name and m._c_params().metric == code
+    m = Umap(metric="minkowski", metric_kwds={"p": 3})
+    assert m._c_params().metric == 4 and m._c_params().metric_p == 3.0
 
 
 def test_input_normalisation_and_out_buffer(fake):

@@ -38,7 +38,10 @@ _PARAM_NAMES = (
     "mlp_steps", "mlp_batch", "mlp_negatives", "mlp_learning_rate", "mlp_embedding_weight",
 )
 
-_GPU_METRICS = {"euclidean": L.METRIC_EUCLIDEAN, "cosine": L.METRIC_COSINE}
+_GPU_METRICS = {"euclidean": L.METRIC_EUCLIDEAN, "cosine": L.METRIC_COSINE,
+                "manhattan": L.METRIC_MANHATTAN, "chebyshev": L.METRIC_CHEBYSHEV,
+                "minkowski": L.METRIC_MINKOWSKI, "correlation": L.METRIC_CORRELATION,
+                "canberra": L.METRIC_CANBERRA, "braycurtis": L.METRIC_BRAYCURTIS}
 
 
 def _as_f32_matrix(x: Any, name: str) -> np.ndarray:
@@ -101,16 +104,37 @@ _METRIC_ALIASES = {
 
 
 def _normalize_metric_name(metric: Any) -> str:
-    """_api.py:242-263, then the GPU path's own restriction."""
+    """_api.py:242-263."""
     name = str(metric).lower()
     if name not in _METRIC_ALIASES:
         expected = "|".join(sorted(set(_METRIC_ALIASES.values())))
         raise ValueError(f"unsupported metric '{metric}', expected {expected}")
-    canonical = _METRIC_ALIASES[name]
-    if canonical not in _GPU_METRICS:
-        raise ValueError(
-            f"metric '{canonical}' is not available on the GPU path, expected euclidean|cosine")
-    return canonical
+    return _METRIC_ALIASES[name]
+
+
+def _parse_metric_config(metric: Any, metric_kwds: Any) -> tuple[str, Any, float | None]:
+    """_api.py:266-289: minkowski takes exactly metric_kwds={'p': p}, p finite
+    and > 0; every other metric takes no keywords."""
+    name = _normalize_metric_name(metric)
+    kwds = {} if metric_kwds is None else metric_kwds
+    if not hasattr(kwds, "keys"):
+        raise TypeError("metric_kwds must be a mapping or None")
+    keys = set(kwds.keys())
+    if name == "minkowski":
+        if "p" not in keys:
+            raise ValueError("metric='minkowski' requires metric_kwds={'p': ...}")
+        unsupported = keys - {"p"}
+        if unsupported:
+            bad = ", ".join(sorted(str(key) for key in unsupported))
+            raise ValueError(f"unsupported metric_kwds for minkowski: {bad}")
+        p = float(kwds["p"])
+        if not np.isfinite(p) or p <= 0.0:
+            raise ValueError("metric_kwds['p'] must be finite and > 0 for minkowski")
+        return name, metric_kwds, p
+    if keys:
+        bad = ", ".join(sorted(str(key) for key in keys))
+        raise ValueError(f"unsupported metric_kwds for {name}: {bad}")
+    return name, metric_kwds, None
 
 
 def _validate_dens_params(dens_lambda: float, dens_frac: float,
@@ -314,11 +338,7 @@ class Umap:
         self.n_neighbors = int(n_neighbors)
         self.n_components = int(n_components)
         self.n_epochs = None if n_epochs is None else int(n_epochs)
-        self.metric = _normalize_metric_name(metric)
-        if metric_kwds:
-            bad = ", ".join(sorted(str(k) for k in metric_kwds))
-            raise ValueError(f"unsupported metric_kwds for {self.metric}: {bad}")
-        self.metric_kwds = metric_kwds
+        self.metric, self.metric_kwds, self._metric_p = _parse_metric_config(metric, metric_kwds)
         self.learning_rate = float(learning_rate)
         self.min_dist = float(min_dist)
         self.spread = float(spread)
@@ -367,6 +387,7 @@ class Umap:
         p.n_components = self.n_components
         p.n_epochs = 0 if self.n_epochs is None else self.n_epochs
         p.metric = _GPU_METRICS[self.metric]
+        p.metric_p = 2.0 if self._metric_p is None else self._metric_p
         p.learning_rate = self.learning_rate
         p.min_dist = self.min_dist
         p.spread = self.spread

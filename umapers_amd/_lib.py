@@ -28,6 +28,12 @@ ERR_DEVICE = 8
 
 METRIC_EUCLIDEAN = 0
 METRIC_COSINE = 1
+METRIC_MANHATTAN = 2
+METRIC_CHEBYSHEV = 3
+METRIC_MINKOWSKI = 4
+METRIC_CORRELATION = 5
+METRIC_CANBERRA = 6
+METRIC_BRAYCURTIS = 7
 INIT_RANDOM = 0
 INIT_SPECTRAL = 1
 KNN_FP32 = 0
@@ -75,6 +81,7 @@ class Params(ctypes.Structure):
         ("mlp_negatives", ctypes.c_int32),
         ("mlp_learning_rate", ctypes.c_float),
         ("mlp_embedding_weight", ctypes.c_float),
+        ("metric_p", ctypes.c_float),
     ]
 
 
@@ -134,6 +141,9 @@ SIGNATURES = {
     "umapx_inverse_init": (ctypes.c_int32, [_f32p, ctypes.c_int64, ctypes.c_int64, _f32p,
                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                             ctypes.c_float, _f32p, ctypes.c_int64, _f32p]),
+    "umapx_knn_metric": (ctypes.c_int32, [_f32p, ctypes.c_int64, _f32p, ctypes.c_int64,
+                                          ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_float, _i64p, _f32p]),
     "umapx_n_samples": (ctypes.c_int32, [_vp, _i64p]),
     "umapx_n_features": (ctypes.c_int32, [_vp, _i64p]),
     "umapx_get_embedding": (ctypes.c_int32, [_vp, _f32p]),

@@ -69,12 +69,32 @@ std::string fmt_f32(float v) {
   return std::string(buf, r.ptr);
 }
 
-const char* metric_debug(int m) {
+// The reference's Debug names of Metric (lib.rs:89-101).
+std::string metric_debug(int m, float p) {
   switch (m) {
     case UMAPX_METRIC_EUCLIDEAN: return "Euclidean";
     case UMAPX_METRIC_COSINE: return "Cosine";
+    case UMAPX_METRIC_MANHATTAN: return "Manhattan";
+    case UMAPX_METRIC_CHEBYSHEV: return "Chebyshev";
+    case UMAPX_METRIC_MINKOWSKI: {
+      char buf[64];
+      std::snprintf(buf, sizeof(buf), "Minkowski { p: %g%s }", static_cast<double>(p),
+                    std::floor(p) == p ? ".0" : "");
+      return buf;
+    }
+    case UMAPX_METRIC_CORRELATION: return "Correlation";
+    case UMAPX_METRIC_CANBERRA: return "Canberra";
+    case UMAPX_METRIC_BRAYCURTIS: return "BrayCurtis";
     default: return "Unsupported";
   }
+}
+
+// validate_metric (lib.rs:2230-2239) plus the codes this library knows.
+void validate_metric(int metric, float p) {
+  if (metric < UMAPX_METRIC_EUCLIDEAN || metric > UMAPX_METRIC_BRAYCURTIS)
+    throw_invalid("unknown metric code " + std::to_string(metric));
+  if (metric == UMAPX_METRIC_MINKOWSKI && (!std::isfinite(p) || p <= 0.0f))
+    throw_invalid("minkowski metric requires finite p > 0");
 }
 
 // validate_data (lib.rs:2113-2138) on host rows.
@@ -119,8 +139,7 @@ void validate_data_device(const float* x, int64_t n, int64_t d, hipStream_t s) {
 
 // validate_params (lib.rs:2140-2228) for the dense path.
 void validate_params(const umapx_params& p, int64_t n_samples) {
-  if (p.metric != UMAPX_METRIC_EUCLIDEAN && p.metric != UMAPX_METRIC_COSINE)
-    throw_invalid("metric must be euclidean or cosine on the GPU path");
+  validate_metric(p.metric, p.metric_p);
   if (p.n_neighbors < 2) throw_invalid("n_neighbors must be >= 2");
   if (p.n_neighbors >= n_samples)
     throw_invalid("n_neighbors (" + std::to_string(p.n_neighbors) +
@@ -290,7 +309,7 @@ struct umapx_handle {
       idx.alloc(static_cast<size_t>(n_) * k);
       dist.alloc(static_cast<size_t>(n_) * k);
       knn_device(d_x, n_, d_x, n_, d_, k, p.metric, p.knn_precision, p.rerank_factor, idx.get(),
-                 dist.get(), kws, s, &kst);
+                 dist.get(), kws, s, &kst, p.metric_p);
     }
     t.knn_sec = clk.lap();
     if (kst.recorded) {
@@ -441,7 +460,7 @@ struct umapx_handle {
     DevBuf<int32_t> idx(static_cast<size_t>(m) * k);
     DevBuf<float> dist(static_cast<size_t>(m) * k);
     knn_device(d_q, m, x.get(), n, d, k, p.metric, p.knn_precision, p.rerank_factor, idx.get(),
-               dist.get(), kws, s, nullptr);
+               dist.get(), kws, s, nullptr, p.metric_p);
     DevBuf<float> sg(m), rh(m);
     const float lc = std::max(p.local_connectivity - 1.0f, 0.0f);
     smooth_knn_dist_device(dist.get(), m, k, static_cast<float>(k), lc, 1.0f, false,
@@ -603,6 +622,7 @@ void umapx_default_params(umapx_params* o) {
   o->mlp_negatives = 5;
   o->mlp_learning_rate = 1e-3f;
   o->mlp_embedding_weight = 1.0f;
+  o->metric_p = 2.0f;
 }
 
 const char* umapx_last_error(void) { return g_last_error.c_str(); }
@@ -666,8 +686,8 @@ int32_t umapx_fit_transform_with_knn(umapx_handle* h, const float* x, int64_t n,
   return guarded([&] {
     require_handle(h);
     if (knn_metric != h->p.metric)
-      throw_invalid(std::string("precomputed knn metric (") + metric_debug(knn_metric) +
-                    ") must match model metric (" + metric_debug(h->p.metric) + ")");
+      throw_invalid("precomputed knn metric (" + metric_debug(knn_metric, h->p.metric_p) +
+                    ") must match model metric (" + metric_debug(h->p.metric, h->p.metric_p) + ")");
     validate_data_host(x, n, d);
     validate_params(h->p, n);
     std::vector<int32_t> ti;
@@ -871,13 +891,16 @@ int32_t umapx_find_ab_params(float spread, float min_dist, float* a, float* b) {
   return guarded([&] { find_ab_params(spread, min_dist, a, b); });
 }
 
-int32_t umapx_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
-                               int32_t k, int32_t metric, int32_t precision, int32_t rerank_factor,
-                               int64_t* idx, float* dist) {
+}  // extern "C"
+
+namespace {
+
+int32_t knn_host(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int32_t k,
+                 int32_t metric, float metric_p, int32_t precision, int32_t rerank_factor,
+                 int64_t* idx, float* dist) {
   return guarded([&] {
     if (k < 1) throw_invalid("n_neighbors must be >= 1");
-    if (metric != UMAPX_METRIC_EUCLIDEAN && metric != UMAPX_METRIC_COSINE)
-      throw_invalid("metric must be euclidean or cosine on the GPU path");
+    validate_metric(metric, metric_p);
     if (m == 0) return;
     const bool self = (q == r) && (m == n);
     hipStream_t s;
@@ -899,13 +922,28 @@ int32_t umapx_knn_to_reference(const float* q, int64_t m, const float* r, int64_
     DevBuf<float> dd(static_cast<size_t>(m) * k);
     KnnWorkspace ws;
     knn_device(qp, m, dr.get(), n, d, k, metric, precision, rerank_factor, di.get(), dd.get(), ws, s,
-               nullptr);
+               nullptr, metric_p);
     std::vector<int32_t> hi(static_cast<size_t>(m) * k);
     HIP_CHECK(hipMemcpyAsync(hi.data(), di.get(), 4 * hi.size(), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(dist, dd.get(), 4 * hi.size(), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     idx64_from32(hi, idx);
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t umapx_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
+                               int32_t k, int32_t metric, int32_t precision, int32_t rerank_factor,
+                               int64_t* idx, float* dist) {
+  return knn_host(q, m, r, n, d, k, metric, 2.0f, precision, rerank_factor, idx, dist);
+}
+
+int32_t umapx_knn_metric(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
+                         int32_t k, int32_t metric, float metric_p, int64_t* idx, float* dist) {
+  return knn_host(q, m, r, n, d, k, metric, metric_p, UMAPX_KNN_FP32, 2, idx, dist);
 }
 
 int32_t umapx_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric, int32_t precision,

@@ -742,9 +742,14 @@ int knn_candidate_count(int k, int rerank_factor) {
 
 void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int k,
                 int metric, int precision, int rerank_factor, int32_t* out_idx,
-                float* out_dist, KnnWorkspace& ws, hipStream_t s, KnnStats* stats) {
+                float* out_dist, KnnWorkspace& ws, hipStream_t s, KnnStats* stats,
+                float metric_p) {
   maybe_inject_fault("knn");
   if (m == 0) return;
+  if (metric >= 2) {
+    knn_metric_device(q, m, r, n, d, k, metric, metric_p, out_idx, out_dist, ws, s);
+    return;
+  }
   const int dpad = knn_padded_dim(d);
   if (dpad < 0) throw_invalid("exact GPU kNN supports at most 256 features");
   const int kc = knn_candidate_count(k, rerank_factor);

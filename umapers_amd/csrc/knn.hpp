@@ -12,6 +12,7 @@ struct KnnWorkspace {
   DevBuf<float> rnorm2, qnorm, rnorm;  // rnorm2: -||y||^2/2 of the compute rows
   DevBuf<int32_t> cand;
   DevBuf<uint2> pairs;  // per-split candidate lists when the reference rows are split
+  DevBuf<float> msum;   // per-row sums / sums of squares (correlation)
 };
 
 // Events around the candidate (MFMA) kernel for live roofline figures.
@@ -28,8 +29,18 @@ int knn_candidate_count(int k, int rerank_factor);
 // q == r and m == n this is the self-kNN of the fit (self included, as the
 // reference does).  metric: 0 euclidean, 1 cosine.  precision: 0 FP32
 // candidates, 1 BF16 candidates (both re-ranked exactly in f32).
+// metric: 0 euclidean, 1 cosine (MFMA candidates + exact re-rank); 2
+// manhattan, 3 chebyshev, 4 minkowski (metric_p), 5 correlation, 6 canberra,
+// 7 braycurtis (exact distances on the vector ALUs, knn_metric_device).
 void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int k,
                 int metric, int precision, int rerank_factor, int32_t* out_idx,
-                float* out_dist, KnnWorkspace& ws, hipStream_t s, KnnStats* stats);
+                float* out_dist, KnnWorkspace& ws, hipStream_t s, KnnStats* stats,
+                float metric_p = 2.0f);
+
+// Exact top-k under metric codes 2-7, distances in the reference's f32
+// arithmetic (lib.rs:2494-2698), ties by index.
+void knn_metric_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int k,
+                       int metric, float p, int32_t* out_idx, float* out_dist, KnnWorkspace& ws,
+                       hipStream_t s);
 
 }  // namespace umapx

@@ -1,0 +1,274 @@
+// Exact kNN under the reference's non-inner-product metrics (manhattan,
+// chebyshev, minkowski, correlation, canberra, braycurtis):
+// exact_nearest_neighbors_by / exact_nearest_neighbors_to_reference_by
+// (lib.rs:2891-2928, 3346-3399) with the distances of lib.rs:2494-2698.
+//
+// These distances are not inner products, so they run on the vector ALUs,
+// not on MFMA.  One wave per block; lane l owns query l of the block and keeps
+// its best H (distance, index) pairs sorted in registers.  The block's 64
+// query rows sit in LDS transposed ([feature][lane], so the 64 lanes read 64
+// consecutive words), and the reference rows stream through LDS 16 at a time
+// (row-major, so every lane reads the same word: a broadcast).  Each lane
+// evaluates its query against the 16 rows with 16 independent accumulators,
+// feature by feature in the reference's order and without contraction, so
+// every distance except minkowski's (device powf) equals the reference's
+// bit for bit: manhattan keeps the eight AVX2 lane sums of lib.rs:2518-2541
+// for d >= 64, correlation takes its per-row sums from a separate pass and
+// the pair's dot product in feature order.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "knn.hpp"
+
+namespace umapx {
+namespace {
+
+constexpr int kMetricRows = 16;  // reference rows per LDS tile
+
+// f32::total_cmp key and its inverse (the map is an involution).
+__device__ __forceinline__ int total_key_f(float v) {
+  const int b = __float_as_int(v);
+  return b ^ static_cast<int>(static_cast<unsigned>(b >> 31) >> 1);
+}
+__device__ __forceinline__ float key_to_float(int key) {
+  return __int_as_float(key ^ static_cast<int>(static_cast<unsigned>(key >> 31) >> 1));
+}
+
+// Rust's f32::clamp: NaN passes through.
+__device__ __forceinline__ float clamp_pm1(float c) { return c < -1.f ? -1.f : (c > 1.f ? 1.f : c); }
+
+// Per-row sum and sum of squares in feature order (the row halves of
+// correlation_distance, lib.rs:2655-2676).
+__global__ void row_summaries_kernel(const float* __restrict__ x, int64_t rows, int d,
+                                     float* __restrict__ sum, float* __restrict__ sq) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const float* r = x + i * d;
+  float s = 0.f, q = 0.f;
+  for (int f = 0; f < d; ++f) {
+    s = fadd(s, r[f]);
+    q = fadd(q, fmul(r[f], r[f]));
+  }
+  sum[i] = s;
+  sq[i] = q;
+}
+
+// METRIC: 2 manhattan, 3 chebyshev, 4 minkowski, 5 correlation, 6 canberra,
+// 7 braycurtis.  H: kept candidates per query (>= k).
+template <int METRIC, int H>
+__global__ __launch_bounds__(64) void knn_metric_kernel(
+    const float* __restrict__ q, int64_t m, const float* __restrict__ r, int64_t n, int d, int k,
+    float p, const float* __restrict__ qsum, const float* __restrict__ qsq,
+    const float* __restrict__ rsum, const float* __restrict__ rsq, int32_t* __restrict__ out_idx,
+    float* __restrict__ out_dist) {
+  extern __shared__ float lds[];
+  float* qt = lds;           // [d][64]
+  float* rt = lds + 64 * d;  // [kMetricRows][d]
+  const int lane = threadIdx.x;
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * 64;
+  const int64_t i = q0 + lane;
+  const bool act = i < m;
+  const int qrows = static_cast<int>(m - q0 < 64 ? m - q0 : 64);
+  for (int e = lane; e < 64 * d; e += 64) {
+    const int row = e / d, f = e - row * d;
+    qt[f * 64 + row] = row < qrows ? q[(q0 + row) * d + f] : 0.f;
+  }
+  float xs = 0.f, xq = 0.f;
+  if (METRIC == 5 && act) {
+    xs = qsum[i];
+    xq = qsq[i];
+  }
+  int keys[H];
+  int32_t ids[H];
+#pragma unroll
+  for (int s = 0; s < H; ++s) {
+    keys[s] = INT_MAX;
+    ids[s] = INT_MAX;
+  }
+  const int nb = d / 8;
+  for (int64_t r0 = 0; r0 < n; r0 += kMetricRows) {
+    const int rows = static_cast<int>(n - r0 < kMetricRows ? n - r0 : kMetricRows);
+    __syncthreads();
+    for (int e = lane; e < kMetricRows * d; e += 64) {
+      const int row = e / d;
+      rt[e] = row < rows ? r[r0 * d + e] : 0.f;
+    }
+    __syncthreads();
+    float dist[kMetricRows];
+    if (METRIC == 2 && d >= 64) {
+      // eight lane sums over the full 8-feature blocks, added in lane order,
+      // then the tail one feature at a time
+      float tot[kMetricRows];
+#pragma unroll
+      for (int t = 0; t < kMetricRows; ++t) tot[t] = 0.f;
+      for (int l = 0; l < 8; ++l) {
+        float acc[kMetricRows];
+#pragma unroll
+        for (int t = 0; t < kMetricRows; ++t) acc[t] = 0.f;
+        for (int b = 0; b < nb; ++b) {
+          const int f = 8 * b + l;
+          const float xv = qt[f * 64 + lane];
+#pragma unroll
+          for (int t = 0; t < kMetricRows; ++t) acc[t] = fadd(acc[t], fabsf(fsub(xv, rt[t * d + f])));
+        }
+#pragma unroll
+        for (int t = 0; t < kMetricRows; ++t) tot[t] = fadd(tot[t], acc[t]);
+      }
+      for (int f = 8 * nb; f < d; ++f) {
+        const float xv = qt[f * 64 + lane];
+#pragma unroll
+        for (int t = 0; t < kMetricRows; ++t) tot[t] = fadd(tot[t], fabsf(fsub(xv, rt[t * d + f])));
+      }
+#pragma unroll
+      for (int t = 0; t < kMetricRows; ++t) dist[t] = tot[t];
+    } else {
+      float acc[kMetricRows], acc2[kMetricRows];
+#pragma unroll
+      for (int t = 0; t < kMetricRows; ++t) {
+        acc[t] = 0.f;
+        acc2[t] = 0.f;
+      }
+      for (int f = 0; f < d; ++f) {
+        const float xv = qt[f * 64 + lane];
+#pragma unroll
+        for (int t = 0; t < kMetricRows; ++t) {
+          const float yv = rt[t * d + f];
+          if (METRIC == 2) {
+            acc[t] = fadd(acc[t], fabsf(fsub(xv, yv)));
+          } else if (METRIC == 3) {
+            acc[t] = fmaxf(acc[t], fabsf(fsub(xv, yv)));
+          } else if (METRIC == 4) {
+            acc[t] = fadd(acc[t], powf(fabsf(fsub(xv, yv)), p));
+          } else if (METRIC == 5) {
+            acc[t] = fadd(acc[t], fmul(xv, yv));
+          } else if (METRIC == 6) {
+            const float den = fadd(fabsf(xv), fabsf(yv));
+            if (den > 0.f) acc[t] = fadd(acc[t], fdiv(fabsf(fsub(xv, yv)), den));
+          } else {
+            acc[t] = fadd(acc[t], fabsf(fsub(xv, yv)));
+            acc2[t] = fadd(acc2[t], fabsf(fadd(xv, yv)));
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kMetricRows; ++t) {
+        if (METRIC == 4) {
+          dist[t] = powf(acc[t], fdiv(1.f, p));
+        } else if (METRIC == 5) {
+          // correlation_distance_from_summaries (lib.rs:2678-2698)
+          const float nf = static_cast<float>(d);
+          const int64_t rj = r0 + (t < rows ? t : 0);
+          const float ys = rsum[rj], yq = rsq[rj];
+          const float cdot = fsub(acc[t], fdiv(fmul(xs, ys), nf));
+          const float xvar = fmaxf(fsub(xq, fdiv(fmul(xs, xs), nf)), 0.f);
+          const float yvar = fmaxf(fsub(yq, fdiv(fmul(ys, ys), nf)), 0.f);
+          const float xn = sqrtf(xvar), yn = sqrtf(yvar);
+          if (xn == 0.f && yn == 0.f)
+            dist[t] = 0.f;
+          else if (xn == 0.f || yn == 0.f)
+            dist[t] = 1.f;
+          else
+            dist[t] = fsub(1.f, clamp_pm1(fdiv(cdot, fmul(xn, yn))));
+        } else if (METRIC == 7) {
+          dist[t] = acc2[t] == 0.f ? 0.f : fdiv(acc[t], acc2[t]);
+        } else {
+          dist[t] = acc[t];
+        }
+      }
+    }
+    // top-H by (distance total order, index), rows in increasing index order
+#pragma unroll
+    for (int t = 0; t < kMetricRows; ++t) {
+      if (t >= rows) break;
+      const int key = total_key_f(dist[t]);
+      const int32_t id = static_cast<int32_t>(r0 + t);
+      if (key < keys[H - 1] || (key == keys[H - 1] && id < ids[H - 1])) {
+        keys[H - 1] = key;
+        ids[H - 1] = id;
+#pragma unroll
+        for (int s = H - 1; s > 0; --s) {
+          const bool sw = keys[s] < keys[s - 1] || (keys[s] == keys[s - 1] && ids[s] < ids[s - 1]);
+          if (sw) {
+            const int tk = keys[s];
+            keys[s] = keys[s - 1];
+            keys[s - 1] = tk;
+            const int32_t ti = ids[s];
+            ids[s] = ids[s - 1];
+            ids[s - 1] = ti;
+          }
+        }
+      }
+    }
+  }
+  if (!act) return;
+#pragma unroll
+  for (int s = 0; s < H; ++s)
+    if (s < k) {
+      out_idx[i * k + s] = ids[s];
+      out_dist[i * k + s] = key_to_float(keys[s]);
+    }
+}
+
+template <int METRIC, int H>
+void launch_metric(const float* q, int64_t m, const float* r, int64_t n, int d, int k, float p,
+                   const float* qs, const float* qq, const float* rs, const float* rq,
+                   int32_t* out_idx, float* out_dist, hipStream_t s) {
+  const size_t bytes = static_cast<size_t>(64 + kMetricRows) * d * sizeof(float);
+  auto kern = knn_metric_kernel<METRIC, H>;
+  HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+  hipLaunchKernelGGL(kern, dim3(blocks_for(m, 64)), dim3(64), bytes, s, q, m, r, n, d, k, p, qs,
+                     qq, rs, rq, out_idx, out_dist);
+  KERNEL_CHECK();
+}
+
+template <int H>
+void dispatch_metric(int metric, const float* q, int64_t m, const float* r, int64_t n, int d,
+                     int k, float p, const float* qs, const float* qq, const float* rs,
+                     const float* rq, int32_t* out_idx, float* out_dist, hipStream_t s) {
+  switch (metric) {
+    case 2: launch_metric<2, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
+    case 3: launch_metric<3, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
+    case 4: launch_metric<4, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
+    case 5: launch_metric<5, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
+    case 6: launch_metric<6, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
+    default: launch_metric<7, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
+  }
+}
+
+}  // namespace
+
+void knn_metric_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int k,
+                       int metric, float p, int32_t* out_idx, float* out_dist, KnnWorkspace& ws,
+                       hipStream_t s) {
+  if (metric < 2 || metric > 7) throw_invalid("unknown metric code");
+  if (d < 1 || d > 256) throw_invalid("exact GPU kNN supports at most 256 features");
+  if (k < 1 || k > 32 || k > n) throw_invalid("n_neighbors must be <= 32 on the GPU path");
+  if (n >= (int64_t{1} << 31)) throw_invalid("exact GPU kNN supports fewer than 2^31 rows");
+  if (m == 0) return;
+  const float* qs = nullptr;
+  const float* qq = nullptr;
+  const float* rs = nullptr;
+  const float* rq = nullptr;
+  if (metric == 5) {
+    ws.msum.ensure(static_cast<size_t>(2 * (n + m)));
+    float* base = ws.msum.get();
+    hipLaunchKernelGGL(row_summaries_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, r, n,
+                       static_cast<int>(d), base, base + n);
+    hipLaunchKernelGGL(row_summaries_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, s, q, m,
+                       static_cast<int>(d), base + 2 * n, base + 2 * n + m);
+    KERNEL_CHECK();
+    rs = base;
+    rq = base + n;
+    qs = base + 2 * n;
+    qq = base + 2 * n + m;
+  }
+  const int dd = static_cast<int>(d);
+  if (k <= 16)
+    dispatch_metric<16>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, s);
+  else
+    dispatch_metric<32>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, s);
+}
+
+}  // namespace umapx
