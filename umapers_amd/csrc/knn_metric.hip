@@ -6,15 +6,17 @@
 // These distances are not inner products, so they run on the vector ALUs,
 // not on MFMA.  One wave per block; lane l owns query l of the block and keeps
 // its best H (distance, index) pairs sorted in registers.  The block's 64
-// query rows sit in LDS transposed ([feature][lane], so the 64 lanes read 64
-// consecutive words), and the reference rows stream through LDS 16 at a time
-// (row-major, so every lane reads the same word: a broadcast).  Each lane
-// evaluates its query against the 16 rows with 16 independent accumulators,
-// feature by feature in the reference's order and without contraction, so
-// every distance except minkowski's (device powf) equals the reference's
+// query rows sit in LDS transposed ([feature / 4][lane][4] when d is a
+// multiple of 4, so each lane reads its 4 features as one 16-byte word and
+// the wave a contiguous 1 KB), and the reference rows stream through LDS 16
+// at a time (row-major, so every lane reads the same 16 bytes: a broadcast).
+// Each lane evaluates its query against the 16 rows with 16 independent
+// accumulators, feature by feature in the reference's order and without
+// contraction, so every distance except minkowski's equals the reference's
 // bit for bit: manhattan keeps the eight AVX2 lane sums of lib.rs:2518-2541
 // for d >= 64, correlation takes its per-row sums from a separate pass and
-// the pair's dot product in feature order.
+// the pair's dot product in feature order.  Minkowski raises each |x - y| to
+// p with the hardware log2 / exp2, so its distances agree to a tolerance.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -37,6 +39,33 @@ __device__ __forceinline__ float key_to_float(int key) {
 
 // Rust's f32::clamp: NaN passes through.
 __device__ __forceinline__ float clamp_pm1(float c) { return c < -1.f ? -1.f : (c > 1.f ? 1.f : c); }
+
+// |x|^p for the minkowski sum from the hardware log2 / exp2 (about 1 ulp
+// each); minkowski distances are compared to a tolerance, not bit for bit.
+__device__ __forceinline__ float abs_pow(float v, float p) {
+  const float a = fabsf(v);
+  return a == 0.f ? 0.f : __builtin_amdgcn_exp2f(p * __builtin_amdgcn_logf(a));
+}
+
+// One feature of one (query, row) pair, in the reference's f32 operations.
+template <int METRIC>
+__device__ __forceinline__ void pair_step(float& acc, float& acc2, float xv, float yv, float p) {
+  if (METRIC == 2) {
+    acc = fadd(acc, fabsf(fsub(xv, yv)));
+  } else if (METRIC == 3) {
+    acc = fmaxf(acc, fabsf(fsub(xv, yv)));
+  } else if (METRIC == 4) {
+    acc = fadd(acc, abs_pow(fsub(xv, yv), p));
+  } else if (METRIC == 5) {
+    acc = fadd(acc, fmul(xv, yv));
+  } else if (METRIC == 6) {
+    const float den = fadd(fabsf(xv), fabsf(yv));
+    if (den > 0.f) acc = fadd(acc, fdiv(fabsf(fsub(xv, yv)), den));
+  } else {
+    acc = fadd(acc, fabsf(fsub(xv, yv)));
+    acc2 = fadd(acc2, fabsf(fadd(xv, yv)));
+  }
+}
 
 // Per-row sum and sum of squares in feature order (the row halves of
 // correlation_distance, lib.rs:2655-2676).
@@ -63,17 +92,25 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
     const float* __restrict__ rsum, const float* __restrict__ rsq, int32_t* __restrict__ out_idx,
     float* __restrict__ out_dist) {
   extern __shared__ float lds[];
-  float* qt = lds;           // [d][64]
+  float* qt = lds;           // [d / 4][64][4], or [d][64] when d % 4 != 0
   float* rt = lds + 64 * d;  // [kMetricRows][d]
   const int lane = threadIdx.x;
   const int64_t q0 = static_cast<int64_t>(blockIdx.x) * 64;
   const int64_t i = q0 + lane;
   const bool act = i < m;
   const int qrows = static_cast<int>(m - q0 < 64 ? m - q0 : 64);
+  const bool v4 = (d & 3) == 0;
   for (int e = lane; e < 64 * d; e += 64) {
     const int row = e / d, f = e - row * d;
-    qt[f * 64 + row] = row < qrows ? q[(q0 + row) * d + f] : 0.f;
+    const float v = row < qrows ? q[(q0 + row) * d + f] : 0.f;
+    if (v4)
+      qt[((f >> 2) * 64 + row) * 4 + (f & 3)] = v;
+    else
+      qt[f * 64 + row] = v;
   }
+  const float4* qt4 = reinterpret_cast<const float4*>(qt);
+  const float4* rt4 = reinterpret_cast<const float4*>(rt);
+  const int d4 = d >> 2;
   float xs = 0.f, xq = 0.f;
   if (METRIC == 5 && act) {
     xs = qsum[i];
@@ -96,7 +133,57 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
     }
     __syncthreads();
     float dist[kMetricRows];
-    if (METRIC == 2 && d >= 64) {
+    if (METRIC == 2 && d >= 64 && v4) {
+      // the eight lane sums as two float4 halves of each 8-feature block,
+      // four rows at a time; the lanes added in order, then the tail
+      constexpr int kPass = 4;
+      const int nb8 = d >> 3;
+#pragma unroll
+      for (int t0 = 0; t0 < kMetricRows; t0 += kPass) {
+        float4 lo[kPass], hi[kPass];
+#pragma unroll
+        for (int t = 0; t < kPass; ++t) {
+          lo[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+          hi[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        for (int b = 0; b < nb8; ++b) {
+          const float4 xa = qt4[(2 * b) * 64 + lane], xb = qt4[(2 * b + 1) * 64 + lane];
+#pragma unroll
+          for (int t = 0; t < kPass; ++t) {
+            const float4 ya = rt4[(t0 + t) * d4 + 2 * b], yb = rt4[(t0 + t) * d4 + 2 * b + 1];
+            lo[t].x = fadd(lo[t].x, fabsf(fsub(xa.x, ya.x)));
+            lo[t].y = fadd(lo[t].y, fabsf(fsub(xa.y, ya.y)));
+            lo[t].z = fadd(lo[t].z, fabsf(fsub(xa.z, ya.z)));
+            lo[t].w = fadd(lo[t].w, fabsf(fsub(xa.w, ya.w)));
+            hi[t].x = fadd(hi[t].x, fabsf(fsub(xb.x, yb.x)));
+            hi[t].y = fadd(hi[t].y, fabsf(fsub(xb.y, yb.y)));
+            hi[t].z = fadd(hi[t].z, fabsf(fsub(xb.z, yb.z)));
+            hi[t].w = fadd(hi[t].w, fabsf(fsub(xb.w, yb.w)));
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < kPass; ++t) {
+          float sum = 0.f;
+          sum = fadd(sum, lo[t].x);
+          sum = fadd(sum, lo[t].y);
+          sum = fadd(sum, lo[t].z);
+          sum = fadd(sum, lo[t].w);
+          sum = fadd(sum, hi[t].x);
+          sum = fadd(sum, hi[t].y);
+          sum = fadd(sum, hi[t].z);
+          sum = fadd(sum, hi[t].w);
+          if (nb8 * 2 < d4) {  // a 4-feature tail
+            const float4 xa = qt4[(2 * nb8) * 64 + lane];
+            const float4 ya = rt4[(t0 + t) * d4 + 2 * nb8];
+            sum = fadd(sum, fabsf(fsub(xa.x, ya.x)));
+            sum = fadd(sum, fabsf(fsub(xa.y, ya.y)));
+            sum = fadd(sum, fabsf(fsub(xa.z, ya.z)));
+            sum = fadd(sum, fabsf(fsub(xa.w, ya.w)));
+          }
+          dist[t0 + t] = sum;
+        }
+      }
+    } else if (METRIC == 2 && d >= 64) {
       // eight lane sums over the full 8-feature blocks, added in lane order,
       // then the tail one feature at a time
       float tot[kMetricRows];
@@ -129,26 +216,23 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
         acc[t] = 0.f;
         acc2[t] = 0.f;
       }
-      for (int f = 0; f < d; ++f) {
-        const float xv = qt[f * 64 + lane];
+      if (v4) {
+        for (int f4 = 0; f4 < d4; ++f4) {
+          const float4 xv = qt4[f4 * 64 + lane];
 #pragma unroll
-        for (int t = 0; t < kMetricRows; ++t) {
-          const float yv = rt[t * d + f];
-          if (METRIC == 2) {
-            acc[t] = fadd(acc[t], fabsf(fsub(xv, yv)));
-          } else if (METRIC == 3) {
-            acc[t] = fmaxf(acc[t], fabsf(fsub(xv, yv)));
-          } else if (METRIC == 4) {
-            acc[t] = fadd(acc[t], powf(fabsf(fsub(xv, yv)), p));
-          } else if (METRIC == 5) {
-            acc[t] = fadd(acc[t], fmul(xv, yv));
-          } else if (METRIC == 6) {
-            const float den = fadd(fabsf(xv), fabsf(yv));
-            if (den > 0.f) acc[t] = fadd(acc[t], fdiv(fabsf(fsub(xv, yv)), den));
-          } else {
-            acc[t] = fadd(acc[t], fabsf(fsub(xv, yv)));
-            acc2[t] = fadd(acc2[t], fabsf(fadd(xv, yv)));
+          for (int t = 0; t < kMetricRows; ++t) {
+            const float4 yv = rt4[t * d4 + f4];
+            pair_step<METRIC>(acc[t], acc2[t], xv.x, yv.x, p);
+            pair_step<METRIC>(acc[t], acc2[t], xv.y, yv.y, p);
+            pair_step<METRIC>(acc[t], acc2[t], xv.z, yv.z, p);
+            pair_step<METRIC>(acc[t], acc2[t], xv.w, yv.w, p);
           }
+        }
+      } else {
+        for (int f = 0; f < d; ++f) {
+          const float xv = qt[f * 64 + lane];
+#pragma unroll
+          for (int t = 0; t < kMetricRows; ++t) pair_step<METRIC>(acc[t], acc2[t], xv, rt[t * d + f], p);
         }
       }
 #pragma unroll
