@@ -545,44 +545,59 @@ __global__ void bipartite_rows_kernel(const int32_t* __restrict__ idx,
 }
 
 // prune_edges (lib.rs:3804-3822): keep w >= max/denom and w > 0.
-__global__ void prune_count_kernel(const int32_t* __restrict__ tmp_off,
-                                   const float* __restrict__ tmp_w,
-                                   const int32_t* __restrict__ ucount, int64_t n,
-                                   const uint32_t* __restrict__ wmax_bits, float denom,
-                                   int do_prune, int32_t* __restrict__ kcount) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float thr = do_prune ? fdiv(__uint_as_float(*wmax_bits), denom) : 0.f;
-  const int32_t base = tmp_off[i];
-  int cnt = 0;
-  for (int c = 0; c < ucount[i]; ++c) {
-    const float w = tmp_w[base + c];
-    if ((!do_prune || w >= thr) && w > 0.f) ++cnt;
-  }
-  kcount[i] = cnt;
+// prune_edges (lib.rs:3804-3822) over the per-row union lists: one wave per
+// row, lanes over the row's entries, so reads and writes are coalesced and a
+// hub row does not hold one lane for a thousand iterations.  kcount[i] = the
+// entries kept; the write pass keeps their order (ballot prefix counts).
+constexpr int kPruneWaves = 4;
+
+__device__ __forceinline__ bool prune_keep(float w, float thr, int do_prune) {
+  return (!do_prune || w >= thr) && w > 0.f;
 }
 
-__global__ void prune_write_kernel(const int32_t* __restrict__ tmp_off,
-                                   const int32_t* __restrict__ tmp_col,
-                                   const float* __restrict__ tmp_w,
-                                   const int32_t* __restrict__ ucount, int64_t n,
-                                   const uint32_t* __restrict__ wmax_bits, float denom,
-                                   int do_prune, const int32_t* __restrict__ row_ptr,
-                                   int32_t* __restrict__ head, int32_t* __restrict__ tail,
-                                   float* __restrict__ weight) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(64 * kPruneWaves) void prune_count_kernel(
+    const int32_t* __restrict__ tmp_off, const float* __restrict__ tmp_w,
+    const int32_t* __restrict__ ucount, int64_t n, const uint32_t* __restrict__ wmax_bits,
+    float denom, int do_prune, int32_t* __restrict__ kcount) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPruneWaves + (threadIdx.x >> 6);
+  if (i >= n) return;  // the whole wave
+  const int lane = threadIdx.x & 63;
   const float thr = do_prune ? fdiv(__uint_as_float(*wmax_bits), denom) : 0.f;
-  const int32_t base = tmp_off[i];
+  const int32_t base = tmp_off[i], cnt = ucount[i];
+  int kept = 0;
+  for (int c0 = 0; c0 < cnt; c0 += 64) {
+    const int c = c0 + lane;
+    const bool keep = c < cnt && prune_keep(tmp_w[base + c], thr, do_prune);
+    kept += __popcll(__ballot(keep));
+  }
+  if (lane == 0) kcount[i] = kept;
+}
+
+__global__ __launch_bounds__(64 * kPruneWaves) void prune_write_kernel(
+    const int32_t* __restrict__ tmp_off, const int32_t* __restrict__ tmp_col,
+    const float* __restrict__ tmp_w, const int32_t* __restrict__ ucount, int64_t n,
+    const uint32_t* __restrict__ wmax_bits, float denom, int do_prune,
+    const int32_t* __restrict__ row_ptr, int32_t* __restrict__ head, int32_t* __restrict__ tail,
+    float* __restrict__ weight) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPruneWaves + (threadIdx.x >> 6);
+  if (i >= n) return;  // the whole wave
+  const int lane = threadIdx.x & 63;
+  const float thr = do_prune ? fdiv(__uint_as_float(*wmax_bits), denom) : 0.f;
+  const int32_t base = tmp_off[i], cnt = ucount[i];
   int32_t o = row_ptr[i];
-  for (int c = 0; c < ucount[i]; ++c) {
-    const float w = tmp_w[base + c];
-    if ((!do_prune || w >= thr) && w > 0.f) {
-      head[o] = static_cast<int32_t>(i);
-      tail[o] = tmp_col[base + c];
-      weight[o] = w;
-      ++o;
+  const uint64_t below = (uint64_t{1} << lane) - 1;
+  for (int c0 = 0; c0 < cnt; c0 += 64) {
+    const int c = c0 + lane;
+    const float w = c < cnt ? tmp_w[base + c] : 0.f;
+    const bool keep = c < cnt && prune_keep(w, thr, do_prune);
+    const uint64_t mask = __ballot(keep);
+    if (keep) {
+      const int32_t at = o + __popcll(mask & below);
+      head[at] = static_cast<int32_t>(i);
+      tail[at] = tmp_col[base + c];
+      weight[at] = w;
     }
+    o += __popcll(mask);
   }
 }
 
@@ -709,10 +724,11 @@ void smooth_knn_dist_device(const float* dist, int64_t n, int k, float k_value,
 
 static void finish_graph(int64_t n, int k, int prune_epochs, const int32_t* tmp_off, Graph& g,
                          GraphWorkspace& ws, hipStream_t s) {
+  g.symmetric = false;
   const float denom = prune_epochs > 10 ? static_cast<float>(prune_epochs) : 500.0f;
   const int do_prune = prune_epochs > 0 ? 1 : 0;
   ws.kcount.ensure(static_cast<size_t>(n));
-  hipLaunchKernelGGL(prune_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, tmp_off,
+  hipLaunchKernelGGL(prune_count_kernel, dim3(blocks_for(n, kPruneWaves)), dim3(64 * kPruneWaves), 0, s, tmp_off,
                      ws.tmp_w.get(), ws.ucount.get(), n, ws.umax.get(), denom, do_prune,
                      ws.kcount.get());
   KERNEL_CHECK();
@@ -725,7 +741,7 @@ static void finish_graph(int64_t n, int k, int prune_epochs, const int32_t* tmp_
   g.head.ensure(std::max<int64_t>(ne, 1));
   g.tail.ensure(std::max<int64_t>(ne, 1));
   g.weight.ensure(std::max<int64_t>(ne, 1));
-  hipLaunchKernelGGL(prune_write_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, tmp_off,
+  hipLaunchKernelGGL(prune_write_kernel, dim3(blocks_for(n, kPruneWaves)), dim3(64 * kPruneWaves), 0, s, tmp_off,
                      ws.tmp_col.get(), ws.tmp_w.get(), ws.ucount.get(), n, ws.umax.get(), denom,
                      do_prune, g.row_ptr.get(), g.head.get(), g.tail.get(), g.weight.get());
   KERNEL_CHECK();
@@ -803,6 +819,7 @@ void fuzzy_graph_device(const int32_t* idx, const float* dist, const float* sigm
                      ws.in_off.get(), n, k, ws.tmp_off.get());
   KERNEL_CHECK();
   finish_graph(n, k, prune_epochs, ws.tmp_off.get(), g, ws, s);
+  g.symmetric = true;  // the union holds every edge in both directions
 }
 
 void bipartite_graph_device(const int32_t* idx, const float* dist, const float* sigmas,

@@ -25,6 +25,7 @@ struct Graph {
   DevBuf<int32_t> tail;     // E
   DevBuf<float> weight;     // E
   float max_weight = 0.f;
+  bool symmetric = false;  // every edge (h, t) has its reverse (t, h): the fit's union
 };
 
 struct GraphWorkspace {

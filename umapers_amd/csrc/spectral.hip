@@ -21,9 +21,11 @@
 #include "spectral.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "hostmath.hpp"
 
@@ -74,6 +76,38 @@ __global__ void hook_kernel(const int32_t* __restrict__ head, const int32_t* __r
   }
   // one store per wave, not one per hooked edge
   if (__ballot(hooked) != 0ULL && (threadIdx.x & 63) == 0) *changed = 1;
+}
+
+// The same hooking over a symmetric edge list: each undirected edge is
+// stored both ways, so only the direction whose head carries the larger label
+// hooks.  Edges sorted by head put a head's run in consecutive lanes; the
+// wave takes one atomic per distinct target (the head's label) with the
+// smallest label offered for it, instead of one atomic per edge.
+__global__ void hook_sym_kernel(const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
+                                int64_t ne, int32_t* __restrict__ label,
+                                int32_t* __restrict__ changed) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int32_t hi = 0, lo = 0;
+  bool act = false;
+  if (e < ne) {
+    hi = label[head[e]];
+    lo = label[tail[e]];
+    act = hi > lo;
+  }
+  uint64_t m = __ballot(act);
+  if (m != 0ULL && lane == 0) *changed = 1;
+  while (m != 0ULL) {
+    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    const int32_t target = __shfl(hi, leader);
+    const bool mine = act && hi == target;
+    int32_t v = mine ? lo : INT32_MAX;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    if (lane == leader) atomicMin(&label[target], v);
+    act = act && !mine;
+    m = __ballot(act);
+  }
 }
 
 __global__ void jump_kernel(int32_t* __restrict__ label, int64_t n) {
@@ -469,6 +503,16 @@ __global__ __launch_bounds__(256) void project_kernel(const int32_t* __restrict_
     atomicMax(reinterpret_cast<unsigned int*>(&maxabs[slot]), __float_as_uint(mx));
 }
 
+// scale[slot] = target[slot] / max|coord| of the slot, 1 when that is 0
+// (noisy_scale_coords, lib.rs:4766-4779).
+__global__ void scale_kernel(const float* __restrict__ maxabs, const float* __restrict__ target,
+                             int n_slots, float* __restrict__ scale) {
+  const int sl = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (sl >= n_slots) return;
+  const float mx = maxabs[sl];
+  scale[sl] = mx > 0.f ? fdiv(target[sl], mx) : 1.f;
+}
+
 // y = coords * scale[slot] + N(0, 1e-4) + anchor[slot]  (noisy_scale_coords /
 // add_noise + anchor, lib.rs:4745-4791; counter-based normals).
 __global__ void finish_kernel(const int32_t* __restrict__ perm, int64_t rows,
@@ -643,7 +687,10 @@ int64_t connected_components_device(const Graph& g, std::vector<int32_t>& labels
   KERNEL_CHECK();
   for (int it = 0; it < 10000; ++it) {
     HIP_CHECK(hipMemsetAsync(ws.flag.get(), 0, sizeof(int32_t), s));
-    if (g.n_edges > 0)
+    if (g.n_edges > 0 && g.symmetric)
+      hipLaunchKernelGGL(hook_sym_kernel, dim3(blocks_for(g.n_edges, 256)), dim3(256), 0, s,
+                         g.head.get(), g.tail.get(), g.n_edges, ws.label.get(), ws.flag.get());
+    else if (g.n_edges > 0)
       hipLaunchKernelGGL(hook_kernel, dim3(blocks_for(g.n_edges, 256)), dim3(256), 0, s,
                          g.head.get(), g.tail.get(), g.n_edges, ws.label.get(), ws.flag.get());
     hipLaunchKernelGGL(jump_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, ws.label.get(), n);
@@ -674,16 +721,22 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
   // components, ordered by smallest member, members ascending
   std::vector<int32_t> labels;
   connected_components_device(g, labels, ws, s);
-  std::vector<int64_t> comp_index(n, -1);
-  std::vector<Comp> comps;
+  // two passes: component of each row (labels[v] is overwritten by it) and
+  // sizes, then members filled into reserved lists
+  std::vector<int32_t> comp_index(n, -1);
+  std::vector<int32_t> sizes;
   for (int64_t v = 0; v < n; ++v) {
-    const int32_t root = labels[v];
-    if (comp_index[root] < 0) {
-      comp_index[root] = static_cast<int64_t>(comps.size());
-      comps.emplace_back();
+    int32_t& c = comp_index[labels[v]];
+    if (c < 0) {
+      c = static_cast<int32_t>(sizes.size());
+      sizes.push_back(0);
     }
-    comps[comp_index[root]].members.push_back(static_cast<int32_t>(v));
+    labels[v] = c;
+    ++sizes[c];
   }
+  std::vector<Comp> comps(sizes.size());
+  for (size_t c = 0; c < sizes.size(); ++c) comps[c].members.reserve(sizes[c]);
+  for (int64_t v = 0; v < n; ++v) comps[labels[v]].members.push_back(static_cast<int32_t>(v));
   const int64_t nc = static_cast<int64_t>(comps.size());
   rep.n_components = nc;
   const bool multi = nc > 1;
@@ -696,6 +749,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
   if (multi) {
     // centroids on the GPU (f64), meta layout on the host
     std::vector<int32_t> perm_all;
+    perm_all.reserve(n);
     std::vector<int32_t> items;
     for (int64_t c = 0; c < nc; ++c) {
       const int32_t start = static_cast<int32_t>(perm_all.size());
@@ -821,17 +875,23 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
   if (n_slots > 0) {
     std::vector<int32_t> perm, slot_of_pos, local_of_pos, items;
     std::vector<uint64_t> seeds(n_slots), noise_seeds(n_slots);
+    size_t total_rows = 0;
+    for (int sl = 0; sl < n_slots; ++sl) total_rows += comps[gpu_comps[sl]].members.size();
+    perm.resize(total_rows);
+    slot_of_pos.resize(total_rows);
+    local_of_pos.resize(total_rows);
+    items.reserve(3 * (total_rows / kItemRows + n_slots));
+    size_t at = 0;
     for (int sl = 0; sl < n_slots; ++sl) {
       const int64_t c = gpu_comps[sl];
-      const int32_t start = static_cast<int32_t>(perm.size());
+      const int32_t start = static_cast<int32_t>(at);
       const auto& mem = comps[c].members;
       if (static_cast<int64_t>(mem.size()) <= kSub) return false;
-      for (size_t i = 0; i < mem.size(); ++i) {
-        perm.push_back(mem[i]);
-        slot_of_pos.push_back(sl);
-        local_of_pos.push_back(static_cast<int32_t>(i));
-      }
-      const int32_t end = static_cast<int32_t>(perm.size());
+      std::copy(mem.begin(), mem.end(), perm.begin() + at);
+      std::fill(slot_of_pos.begin() + at, slot_of_pos.begin() + at + mem.size(), sl);
+      std::iota(local_of_pos.begin() + at, local_of_pos.begin() + at + mem.size(), 0);
+      at += mem.size();
+      const int32_t end = static_cast<int32_t>(at);
       for (int32_t p = start; p < end; p += kItemRows) {
         items.push_back(sl);
         items.push_back(p);
@@ -953,7 +1013,8 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     HIP_CHECK(hipMemcpyAsync(fail.data(), ws.fail.get(), 4 * n_slots, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     std::vector<double> vmat(static_cast<size_t>(n_slots) * kSub * dim);
-    std::vector<float> scale(n_slots), anchor(static_cast<size_t>(n_slots) * dim);
+    std::vector<float> anchor(static_cast<size_t>(n_slots) * dim);
+    std::vector<int> ok_slots;
     for (int sl = 0; sl < n_slots; ++sl) {
       const int64_t c = gpu_comps[sl];
       if (fail[sl]) {
@@ -983,6 +1044,13 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
           for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * kSub + b) * dim + k] = 0.0;
         continue;
       }
+      ok_slots.push_back(sl);
+    }
+    // The Ritz problems of the converged slots are independent: with many
+    // components they are spread over a few host threads (each slot's result
+    // is the same whichever thread solves it).
+    auto ritz_slot = [&](int sl) {
+      const int64_t c = gpu_comps[sl];
       std::vector<double> a(ritz.begin() + static_cast<size_t>(sl) * kSub * kSub,
                             ritz.begin() + static_cast<size_t>(sl + 1) * kSub * kSub);
       for (int i = 0; i < kSub; ++i)
@@ -995,27 +1063,38 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
       for (int b = 0; b < kSub; ++b)
         for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * kSub + b) * dim + k] = vec[b * kSub + order[k + 1]];
       for (int k = 0; k < dim; ++k) anchor[sl * dim + k] = layout[c * dim + k];
-      ++rep.n_iterative;
+    };
+    const int n_ok = static_cast<int>(ok_slots.size());
+    const int n_threads = std::min(8, n_ok / 4);
+    if (n_threads <= 1) {
+      for (int sl : ok_slots) ritz_slot(sl);
+    } else {
+      std::atomic<int> next{0};
+      std::vector<std::thread> pool;
+      for (int t = 0; t < n_threads; ++t)
+        pool.emplace_back([&] {
+          for (int i = next++; i < n_ok; i = next++) ritz_slot(ok_slots[i]);
+        });
+      for (auto& th : pool) th.join();
     }
+    rep.n_iterative += n_ok;
     ws.vmat.ensure(vmat.size());
     ws.maxabs.ensure(n_slots);
     ws.scale.ensure(n_slots);
+    // anchors then the per-slot target extent in one upload; the scale
+    // target / max|coord| is formed on the device (no round trip)
+    for (int sl = 0; sl < n_slots; ++sl) anchor.push_back(multi ? range[gpu_comps[sl]] : 10.0f);
     ws.anchor.ensure(anchor.size());
     DevBuf<float> coords(static_cast<size_t>(n) * dim);
     HIP_CHECK(hipMemcpyAsync(ws.vmat.get(), vmat.data(), 8 * vmat.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(ws.anchor.get(), anchor.data(), 4 * anchor.size(), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemsetAsync(ws.maxabs.get(), 0, 4 * n_slots, s));
     hipLaunchKernelGGL(project_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
                        d_slot, yb, ws.vmat.get(), dim, coords.get(), ws.maxabs.get());
+    hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(n_slots, 64)), dim3(64), 0, s,
+                       ws.maxabs.get(), ws.anchor.get() + static_cast<size_t>(n_slots) * dim, n_slots,
+                       ws.scale.get());
     KERNEL_CHECK();
-    std::vector<float> mx(n_slots);
-    HIP_CHECK(hipMemcpyAsync(mx.data(), ws.maxabs.get(), 4 * n_slots, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    for (int sl = 0; sl < n_slots; ++sl) {
-      const float target = multi ? range[gpu_comps[sl]] : 10.0f;
-      scale[sl] = mx[sl] > 0.f ? target / mx[sl] : 1.f;
-    }
-    HIP_CHECK(hipMemcpyAsync(ws.scale.get(), scale.data(), 4 * n_slots, hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(ws.anchor.get(), anchor.data(), 4 * anchor.size(), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(finish_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows, d_slot,
                        d_local, ws.seeds.get() + n_slots, ws.scale.get(), ws.anchor.get(), dim,
                        coords.get(), y);
