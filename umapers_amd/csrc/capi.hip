@@ -476,6 +476,7 @@ struct umapx_handle {
     sa.negative_sample_rate = p.negative_sample_rate;
     sa.repulsion_strength = p.repulsion_strength;
     sa.seed = p.random_seed ^ 0xA24BAED40B53C117ULL;
+    sa.max_row_edges = k;  // the bipartite graph keeps at most k edges per query
     SgdWorkspace ws;
     optimize_transform_device(d_out, m, p.n_components, y.get(), n, g, sa, ws, s);
     HIP_CHECK(hipStreamSynchronize(s));

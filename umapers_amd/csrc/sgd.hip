@@ -500,6 +500,95 @@ __global__ __launch_bounds__(128) void transform_sgd_kernel(
   for (int c = 0; c < dim; ++c) yq[hq * dim + c] = cur[c];
 }
 
+// The same walk for 2-D embeddings with each query's edges (at most 32, the
+// kNN width) held in LDS: tail rows, eps and both schedule words are read
+// once, and the rows of an edge's negatives are gathered together before the
+// moves are applied one after another.  The arithmetic, the draws and their
+// order are those of transform_sgd_kernel<2>, so the results are identical;
+// only the memory round trips per edge fall from 2 + nneg to about 1.
+constexpr int kXfEdges = 32;
+constexpr int kXfNegBatch = 8;
+
+__global__ __launch_bounds__(64) void transform_sgd2_kernel(
+    float2* __restrict__ yq, int64_t m, const float2* __restrict__ yb, uint32_t n_base,
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ tail,
+    const float* __restrict__ eps, int n_epochs, float alpha0, float a, float b, float rep,
+    float nsr, uint64_t seed) {
+  __shared__ float2 trow[kXfEdges][64];
+  __shared__ float s_eps[kXfEdges][64], s_eons[kXfEdges][64], s_eonns[kXfEdges][64];
+  const int lane = threadIdx.x;
+  const int64_t hq = static_cast<int64_t>(blockIdx.x) * 64 + lane;
+  if (hq >= m) return;  // no block barriers below
+  const int32_t s0 = row_ptr[hq];
+  const int cnt = row_ptr[hq + 1] - s0;  // <= kXfEdges, checked by the host
+  for (int j = 0; j < cnt; ++j) {
+    trow[j][lane] = yb[tail[s0 + j]];
+    const float e = eps[s0 + j];
+    s_eps[j][lane] = e;
+    s_eons[j][lane] = e;
+    s_eonns[j][lane] = fdiv(e, nsr);
+  }
+  float2 cur = yq[hq];
+  const uint64_t key_head = mix64(seed ^ (static_cast<uint64_t>(hq) * 0x9E3779B97F4A7C15ULL));
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    const float alpha = fmul(alpha0, fsub(1.f, fdiv(static_cast<float>(epoch), static_cast<float>(n_epochs))));
+    const float ef = static_cast<float>(epoch);
+    for (int j = 0; j < cnt; ++j) {
+      const bool finite = isfinite(cur.x) && isfinite(cur.y);
+      const float eo = s_eons[j][lane];
+      if (eo > ef || !finite) continue;
+      const float2 other = trow[j][lane];
+      const float dx = cur.x - other.x, dy = cur.y - other.y;
+      float d2 = 0.f;  // accumulated as the generic kernel's loop does
+      d2 += dx * dx;
+      d2 += dy * dy;
+      float gc = 0.f;
+      if (d2 > 0.f) {
+        const float pb = pow_pos(d2, b);
+        gc = (-2.f * a * b) * (pb / d2) / (a * pb + 1.f);
+      }
+      cur.x += clip4(gc * (cur.x - other.x)) * alpha;
+      cur.y += clip4(gc * (cur.y - other.y)) * alpha;
+      const float ej = s_eps[j][lane];
+      s_eons[j][lane] = fadd(eo, ej);
+      const float en = fdiv(ej, nsr);
+      if (!isfinite(en) || en <= 0.f) continue;
+      const float eonn = s_eonns[j][lane];
+      const float fl = floorf(fdiv(fsub(ef, eonn), en));
+      const int nneg = fl > 0.f ? static_cast<int>(fl) : 0;
+      s_eonns[j][lane] = fadd(eonn, fmul(static_cast<float>(nneg), en));
+      if (nneg <= 0) continue;
+      const uint64_t key = mix64(key_head + static_cast<uint64_t>(epoch) * 0xD1B54A32D192ED03ULL +
+                                 static_cast<uint64_t>(j));
+      for (int sb = 0; sb < nneg; sb += kXfNegBatch) {
+        float2 nv[kXfNegBatch];
+#pragma unroll
+        for (int u = 0; u < kXfNegBatch; ++u)
+          nv[u] = sb + u < nneg ? yb[draw_index(key, static_cast<uint32_t>(sb + u), n_base)]
+                                : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < kXfNegBatch; ++u) {
+          if (sb + u >= nneg) break;
+          const float ex = cur.x - nv[u].x, ey = cur.y - nv[u].y;
+          float dd = 0.f;
+          dd += ex * ex;
+          dd += ey * ey;
+          float g2 = 0.f;
+          if (dd > 0.f) {
+            const float pb = pow_pos(dd, b);
+            g2 = rep / ((0.001f + dd) * (a * pb + 1.f));
+          }
+          if (g2 > 0.f) {
+            cur.x += clip4(g2 * (cur.x - nv[u].x)) * alpha;
+            cur.y += clip4(g2 * (cur.y - nv[u].y)) * alpha;
+          }
+        }
+      }
+    }
+  }
+  yq[hq] = cur;
+}
+
 // Edge ends per vertex (as head plus as tail), then the hub move scale.
 constexpr int kHubEnds = 256;
 
@@ -772,7 +861,12 @@ void optimize_transform_device(float* yq, int64_t m, int dim, const float* y_bas
                      ws.eonns.get(), ne, nsr);
   KERNEL_CHECK();
   const float rep = 2.f * args.repulsion_strength * args.b;
-  if (dim == 2)
+  if (dim == 2 && args.max_row_edges > 0 && args.max_row_edges <= kXfEdges)
+    hipLaunchKernelGGL(transform_sgd2_kernel, dim3(blocks_for(m, 64)), dim3(64), 0, s,
+                       reinterpret_cast<float2*>(yq), m, reinterpret_cast<const float2*>(y_base),
+                       static_cast<uint32_t>(n_base), g.row_ptr.get(), g.tail.get(), ws.eps.get(),
+                       args.n_epochs, args.learning_rate, args.a, args.b, rep, nsr, args.seed);
+  else if (dim == 2)
     hipLaunchKernelGGL(transform_sgd_kernel<2>, dim3(blocks_for(m, 128)), dim3(128), 0, s, yq, m,
                        dim, y_base, static_cast<uint32_t>(n_base), g.row_ptr.get(), g.tail.get(),
                        ws.eps.get(), ws.eons.get(), ws.eonns.get(), args.n_epochs,

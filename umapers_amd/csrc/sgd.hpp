@@ -23,6 +23,9 @@ struct SgdArgs {
   // reverse (t, h) of equal weight.  A 2-D tail move is then made by the
   // reverse edge's own thread instead of a float atomic.
   bool symmetric = false;
+  // transform walk: the largest number of edges of one query row (its kNN
+  // width), 0 when unknown; rows of at most 32 edges take the LDS-staged walk
+  int max_row_edges = 0;
   // optional per-edge sample counters (tests of the exact schedule)
   int32_t* pos_counts = nullptr;
   int32_t* neg_counts = nullptr;
