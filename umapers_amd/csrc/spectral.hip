@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <exception>
 #include <cstring>
 #include <numeric>
 #include <thread>
@@ -1070,12 +1071,20 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
       for (int sl : ok_slots) ritz_slot(sl);
     } else {
       std::atomic<int> next{0};
+      std::vector<std::exception_ptr> errors(n_threads);
       std::vector<std::thread> pool;
       for (int t = 0; t < n_threads; ++t)
-        pool.emplace_back([&] {
-          for (int i = next++; i < n_ok; i = next++) ritz_slot(ok_slots[i]);
+        pool.emplace_back([&, t] {
+          try {
+            for (int i = next++; i < n_ok; i = next++) ritz_slot(ok_slots[i]);
+          } catch (...) {
+            errors[t] = std::current_exception();
+            next = n_ok;  // the other threads stop at their next slot
+          }
         });
       for (auto& th : pool) th.join();
+      for (auto& err : errors)
+        if (err) std::rethrow_exception(err);
     }
     rep.n_iterative += n_ok;
     ws.vmat.ensure(vmat.size());
