@@ -12,9 +12,10 @@
 //    wave-uniform and the right-hand side is solved lane by lane.  The f32
 //    operations follow the reference's order without contraction, so the
 //    result equals the oracle's up to the last bit of powf.
-// 4. inverse_sgd_kernel: optimize_layout_inverse (lib.rs:5547-5645), one wave
-//    per query moving that query's row in feature space (Gauss-Seidel over
-//    its own edges, as the reference's edge order does for one head).  The
+// 4. inverse_sgd_kernel: optimize_layout_inverse (lib.rs:5547-5645), a group
+//    of 16-64 lanes per query moving that query's row in feature space
+//    (Gauss-Seidel over its own edges, as the reference's edge order does for
+//    one head).  The
 //    per-edge schedule is the reference's exact f32 one; negatives come from
 //    a counter-based stream per (query, epoch, edge) instead of the
 //    reference's single SmallRng stream.
@@ -294,74 +295,100 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_init_kernel(
     }
 }
 
-// PER features per lane (d <= 64 * PER).  The rows of up to kNegBatch
-// negatives are gathered before the first of them is applied, so their loads
-// overlap; the moves are then applied one after another as the reference does.
+// G lanes per query (64 / G queries per wave), PER features per lane
+// (d <= G * PER): small groups keep the per-step reductions short and let one
+// wave serve several queries.  The rows of up to kNegBatch negatives are
+// gathered before the first of them is applied, so their loads overlap; the
+// moves are then applied one after another as the reference does.  Each
+// query's edge state (weight, tail, eps and both schedule words) sits in LDS.
 constexpr int kNegBatch = 4;
+constexpr int kInvMaxK = 32;  // the kNN width bound
 
-template <int PER>
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <int PER, int G>
 __global__ __launch_bounds__(64 * kInvWaves) void inverse_sgd_kernel(
     float* __restrict__ out, int64_t m, const float* __restrict__ x_train, int64_t d,
     uint32_t n, const int32_t* __restrict__ idx, const float* __restrict__ raw,
     const int32_t* __restrict__ fixed, int k, const uint32_t* __restrict__ wmax_bits,
     const float* __restrict__ sigmas, const float* __restrict__ rhos, int n_epochs, float alpha0,
     float nsr, float gamma, uint64_t seed) {
+  constexpr int kQ = 64 / G;  // queries per wave
+  __shared__ float s_w[kInvWaves * kQ][kInvMaxK], s_eps[kInvWaves * kQ][kInvMaxK];
+  __shared__ float s_eons[kInvWaves * kQ][kInvMaxK], s_eonns[kInvWaves * kQ][kInvMaxK];
+  __shared__ int32_t s_t[kInvWaves * kQ][kInvMaxK];
   const int lane = threadIdx.x & 63;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kInvWaves + (threadIdx.x >> 6);
-  if (i >= m || fixed[i]) return;  // the whole wave
+  const int wv = threadIdx.x >> 6;
+  const int grp = lane / G, sl = lane % G;
+  const int slot = wv * kQ + grp;
+  const int64_t i = (static_cast<int64_t>(blockIdx.x) * kInvWaves + wv) * kQ + grp;
+  if (i >= m || fixed[i]) return;  // the whole group; no block barriers follow
+  const uint64_t gmask = (G == 64 ? ~uint64_t{0} : ((uint64_t{1} << G) - 1)) << (grp * G);
   float* o = out + i * d;
   float cur[PER];
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
-    const int64_t f = lane + 64 * t;
+    const int64_t f = sl + G * t;
     cur[t] = f < d ? o[f] : 0.f;
   }
-  // make_epochs_per_sample (lib.rs:4937-4960) over all inverse edges; lane j
-  // holds edge j's weight, tail and schedule state
+  // make_epochs_per_sample (lib.rs:4937-4960) over all inverse edges
   const float wmax = fmaxf(__uint_as_float(*wmax_bits), 1e-12f);
   const float ne = static_cast<float>(n_epochs);
-  float w_l = 0.f, eps_l = INFINITY, eons_l = INFINITY, eonns_l = INFINITY;
-  int32_t t_l = 0;
-  if (lane < k) {
-    w_l = raw[i * k + lane];
-    t_l = idx[i * k + lane];
-    const float ns = fmul(ne, fdiv(w_l, wmax));
-    eps_l = ns > 0.f ? fdiv(ne, ns) : INFINITY;
-    eons_l = eps_l;
-    eonns_l = fdiv(eps_l, nsr);
+  for (int j = sl; j < k; j += G) {
+    const float w = raw[i * k + j];
+    const float ns = fmul(ne, fdiv(w, wmax));
+    const float e = ns > 0.f ? fdiv(ne, ns) : INFINITY;
+    s_w[slot][j] = w;
+    s_t[slot][j] = idx[i * k + j];
+    s_eps[slot][j] = e;
+    s_eons[slot][j] = e;
+    s_eonns[slot][j] = fdiv(e, nsr);
   }
+  // the edge state is written and read by lanes of one wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const uint64_t key_head = inv_mix64(seed ^ (static_cast<uint64_t>(i) * 0x9E3779B97F4A7C15ULL));
   for (int epoch = 0; epoch < n_epochs; ++epoch) {
     const float alpha = fmul(alpha0, fsub(1.f, fdiv(static_cast<float>(epoch), ne)));
     const float ef = static_cast<float>(epoch);
     for (int j = 0; j < k; ++j) {
-      if (lane_f(eons_l, j) > ef) continue;
+      const float eo = s_eons[slot][j];
+      if (eo > ef) continue;
       bool fin = true;
 #pragma unroll
       for (int t = 0; t < PER; ++t) fin = fin && isfinite(cur[t]);
-      if (__ballot(!fin) != 0ULL) continue;
-      const int32_t tj = __builtin_amdgcn_readlane(t_l, j);
-      const float wj = lane_f(w_l, j);
+      if ((__ballot(!fin) & gmask) != 0ULL) continue;
+      const int32_t tj = s_t[slot][j];
+      const float wj = s_w[slot][j];
       const float* other = x_train + static_cast<int64_t>(tj) * d;
       float ov[PER];
       float part = 0.f;
 #pragma unroll
       for (int t = 0; t < PER; ++t) {
-        const int64_t f = lane + 64 * t;
+        const int64_t f = sl + G * t;
         ov[t] = f < d ? other[f] : 0.f;
         const float df = cur[t] - ov[t];
         part += df * df;
       }
-      const float gd = 1e-6f + sqrtf(wave_sum(part));
+      const float gd = 1e-6f + sqrtf(group_sum<G>(part));
       const float gc = -(1.f / (wj * sigmas[tj] + 1e-6f));
 #pragma unroll
       for (int t = 0; t < PER; ++t)
-        if (lane + 64 * t < d) cur[t] += inv_clip4(gc * ((cur[t] - ov[t]) / gd)) * alpha;
-      if (lane == j) eons_l = fadd(eons_l, eps_l);
-      const float en = fdiv(lane_f(eps_l, j), nsr);
+        if (sl + G * t < d) cur[t] += inv_clip4(gc * ((cur[t] - ov[t]) / gd)) * alpha;
+      const float ej = s_eps[slot][j];
+      if (sl == 0) s_eons[slot][j] = fadd(eo, ej);
+      const float en = fdiv(ej, nsr);
       if (!isfinite(en) || en <= 0.f) continue;
-      const float fl = floorf(fdiv(fsub(ef, lane_f(eonns_l, j)), en));
+      const float eonn = s_eonns[slot][j];
+      const float fl = floorf(fdiv(fsub(ef, eonn), en));
       const int nneg = fl > 0.f ? static_cast<int>(fl) : 0;
+      if (sl == 0) s_eonns[slot][j] = fadd(eonn, fmul(static_cast<float>(nneg), en));
       const uint64_t key = inv_mix64(key_head + static_cast<uint64_t>(epoch) * 0xD1B54A32D192ED03ULL +
                                      static_cast<uint64_t>(j));
       for (int s0 = 0; s0 < nneg; s0 += kNegBatch) {
@@ -378,7 +405,7 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_sgd_kernel(
             const float* o2 = x_train + static_cast<int64_t>(kk) * d;
 #pragma unroll
             for (int t = 0; t < PER; ++t) {
-              const int64_t f = lane + 64 * t;
+              const int64_t f = sl + G * t;
               if (f < d) nv[u][t] = o2[f];
             }
             sk[u] = sigmas[kk];
@@ -394,21 +421,20 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_sgd_kernel(
             const float df = cur[t] - nv[u][t];
             p2 += df * df;
           }
-          const float dn = sqrtf(wave_sum(p2));
+          const float dn = sqrtf(group_sum<G>(p2));
           const float gd2 = 1e-6f + dn;
           const float wh = expf(-fmaxf(dn - rk[u], 1e-6f) / (sk[u] + 1e-6f));
           const float gc2 = -gamma * ((0.f - wh) / ((1.f - wh) * sk[u] + 1e-6f));
 #pragma unroll
           for (int t = 0; t < PER; ++t)
-            if (lane + 64 * t < d) cur[t] += inv_clip4(gc2 * ((cur[t] - nv[u][t]) / gd2)) * alpha;
+            if (sl + G * t < d) cur[t] += inv_clip4(gc2 * ((cur[t] - nv[u][t]) / gd2)) * alpha;
         }
       }
-      if (lane == j) eonns_l = fadd(eonns_l, fmul(static_cast<float>(nneg), en));
     }
   }
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
-    const int64_t f = lane + 64 * t;
+    const int64_t f = sl + G * t;
     if (f < d) o[f] = cur[t];
   }
 }
@@ -426,7 +452,7 @@ void inverse_transform_device(const float* q, int64_t m, int dim, const float* y
     throw_invalid("inverse_transform on the GPU supports at most 512 features");
   if (n >= (int64_t{1} << 31)) throw_invalid("inverse_transform supports fewer than 2^31 training rows");
   const int k = args.k;
-  if (k < 1 || k > 64 || k > n) throw_invalid("inverse_transform: invalid neighbour count");
+  if (k < 1 || k > kInvMaxK || k > n) throw_invalid("inverse_transform: invalid neighbour count");
   DevBuf<int32_t> idx(static_cast<size_t>(m) * k);
   DevBuf<float> dist(static_cast<size_t>(m) * k);
   knn_device(q, m, y_train, n, dim, k, 0, 0, 2, idx.get(), dist.get(), kws, s, nullptr);
@@ -446,8 +472,9 @@ void inverse_transform_device(const float* q, int64_t m, int dim, const float* y
                        idx.get(), norm.get(), fixed.get(), k, y_train, x_train, d, out);
   KERNEL_CHECK();
   if (args.n_epochs > 0) {
-    auto sgd = [&](auto kern) {
-      hipLaunchKernelGGL(kern, grid, dim3(64 * kInvWaves), 0, s, out, m, x_train, d,
+    auto sgd = [&](auto kern, int group) {
+      const dim3 sgrid(blocks_for(m, kInvWaves * (64 / group)));
+      hipLaunchKernelGGL(kern, sgrid, dim3(64 * kInvWaves), 0, s, out, m, x_train, d,
                          static_cast<uint32_t>(n), idx.get(), raw.get(), fixed.get(), k,
                          wmax.get(), sigmas, rhos, args.n_epochs, args.learning_rate,
                          static_cast<float>(args.negative_sample_rate), args.repulsion_strength,
@@ -455,13 +482,13 @@ void inverse_transform_device(const float* q, int64_t m, int dim, const float* y
       KERNEL_CHECK();
     };
     if (d <= 64)
-      sgd(inverse_sgd_kernel<1>);
+      sgd(inverse_sgd_kernel<4, 16>, 16);
     else if (d <= 128)
-      sgd(inverse_sgd_kernel<2>);
+      sgd(inverse_sgd_kernel<8, 16>, 16);
     else if (d <= 256)
-      sgd(inverse_sgd_kernel<4>);
+      sgd(inverse_sgd_kernel<8, 32>, 32);
     else
-      sgd(inverse_sgd_kernel<8>);
+      sgd(inverse_sgd_kernel<8, 64>, 64);
   }
   // the temporaries above are released in stream order after the launches
 }
