@@ -109,10 +109,33 @@ def test_input_normalisation_and_out_buffer(fake):
     assert m.fit_transform(x, out=out) is out
 
 
-def test_sparse_input_rejected(fake):
+def test_sparse_input_rules(fake):
+    # the reference's sparse-path rules (test_binding.py:1314-1530): CSR / CSC /
+    # COO fit like their dense rows; densMAP and canberra / braycurtis are
+    # dense-only; sparse queries need a sparse fit; no inverse for sparse fits
     sp = pytest.importorskip("scipy.sparse")
-    with pytest.raises(ValueError, match="sparse input"):
-        Umap().fit(sp.csr_matrix(np.eye(20)))
+    x = np.random.default_rng(3).normal(size=(60, 6)).astype(np.float32)
+    x[x < 0.2] = 0.0
+    for conv in (sp.csr_matrix, sp.csc_matrix, sp.coo_matrix):
+        m = Umap(n_neighbors=5, n_epochs=10)
+        assert m.fit_transform(conv(x)).shape == (60, 2)
+        assert m._core.calls[-1] == "fit_transform" and m._fitted_sparse
+    with pytest.raises(ValueError, match="dense input only"):
+        Umap(densmap=True).fit(sp.csr_matrix(x))
+    with pytest.raises(ValueError, match="dense input only"):
+        Umap(output_dens=True).fit(sp.csr_matrix(x))
+    with pytest.raises(ValueError, match="Canberra metric is currently supported for dense input only"):
+        Umap(metric="canberra").fit(sp.csr_matrix(x))
+    with pytest.raises(ValueError, match="at most 256 features"):
+        Umap().fit(sp.csr_matrix((10, 300), dtype=np.float32))
+    dense = Umap(n_neighbors=5, n_epochs=10)
+    dense.fit(x)
+    with pytest.raises(ValueError, match="supported only for sparse-fitted models"):
+        dense.transform(sp.csr_matrix(x[:3]))
+    m = Umap(n_neighbors=5, n_epochs=10)
+    m.fit(sp.csr_matrix(x))
+    with pytest.raises(ValueError, match="not supported for sparse-trained models"):
+        m.inverse_transform(np.zeros((2, 2), np.float32))
 
 
 def test_missing_library_fails_loudly(monkeypatch):

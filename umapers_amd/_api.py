@@ -64,6 +64,35 @@ def _is_sparse(x: Any) -> bool:
     return getattr(x, "format", None) in {"csr", "csc", "coo"}
 
 
+# Sparse rows are densified on the host and run the dense path: the
+# reference's own sparse kNN equals its dense kNN on the same values
+# (lib.rs sparse_csr_knn_matches_dense_euclidean), and the GPU kernels take
+# at most this many features.
+_SPARSE_MAX_FEATURES = 256
+
+
+def _sparse_to_dense(x: Any, name: str) -> np.ndarray:
+    """_api.py:143-190 (CSR/CSC/COO normalised to canonical CSR), then dense
+    float32 rows."""
+    fmt = getattr(x, "format", None)
+    matrix = x
+    if fmt in {"csc", "coo"}:
+        tocsr = getattr(x, "tocsr", None)
+        if not callable(tocsr):
+            raise ValueError(f"{name} sparse input with format={fmt!r} cannot be converted to CSR")
+        matrix = tocsr()
+        if getattr(matrix, "format", None) != "csr":
+            raise ValueError(f"{name}.tocsr() must return a CSR matrix")
+    if getattr(matrix, "has_canonical_format", True) is False:
+        matrix = matrix.copy()
+        matrix.sum_duplicates()
+    n_rows, n_cols = matrix.shape
+    if n_cols > _SPARSE_MAX_FEATURES:
+        raise ValueError(f"sparse input on the GPU path is densified and supports at most "
+                         f"{_SPARSE_MAX_FEATURES} features, got {n_cols}")
+    return np.ascontiguousarray(matrix.toarray(), dtype=np.float32)
+
+
 def _as_out_buffer(out: Any, shape: tuple[int, int]) -> np.ndarray:
     """_api.py:193-206."""
     if not isinstance(out, np.ndarray):
@@ -446,21 +475,38 @@ class Umap:
         self.radii_original_, self.radii_embedding_ = radii
         self.rad_orig_, self.rad_emb_ = radii
 
-    def _reject_sparse(self, data: Any) -> None:
-        if _is_sparse(data):
-            raise ValueError("sparse input is not available on the GPU path; pass a dense array")
+    def _fit_rows(self, data: Any) -> np.ndarray:
+        """Dense rows for fit; sparse input follows the reference's sparse-fit
+        rules (_api.py:562-570, lib.rs:2241-2249) and is densified."""
+        if not _is_sparse(data):
+            self._fitted_sparse = False
+            return _as_f32_matrix(data, "data")
+        if self.densmap or self.output_dens:
+            raise ValueError("densmap and output_dens currently support dense input only")
+        if self.metric in ("canberra", "braycurtis"):
+            name = {"canberra": "Canberra", "braycurtis": "BrayCurtis"}[self.metric]
+            raise ValueError(f"invalid parameter: {name} metric is currently supported for dense "
+                             f"input only")
+        arr = _sparse_to_dense(data, "data")
+        self._fitted_sparse = True
+        return arr
+
+    def _query_rows(self, query: Any) -> np.ndarray:
+        if not _is_sparse(query):
+            return _as_f32_matrix(query, "query")
+        if not getattr(self, "_fitted_sparse", False):
+            raise ValueError("sparse query transform is supported only for sparse-fitted models")
+        return _sparse_to_dense(query, "query")
 
     def fit(self, data: Any, y: Any | None = None) -> "Umap":
-        self._reject_sparse(data)
-        arr = _as_f32_matrix(data, "data")
+        arr = self._fit_rows(data)
         self._core.fit(arr)
         self._sync_fitted(self._core.embedding, arr.shape[1], arr.shape[0])
         return self
 
     def fit_transform(self, data: Any, y: Any | None = None, *,
                       out: np.ndarray | None = None):
-        self._reject_sparse(data)
-        arr = _as_f32_matrix(data, "data")
+        arr = self._fit_rows(data)
         if out is not None and self.output_dens:
             raise ValueError("out cannot be used when output_dens=True")
         if out is None:
@@ -480,6 +526,7 @@ class Umap:
         if self.densmap or self.output_dens:
             raise ValueError("profile_fit_transform currently supports standard UMAP only")
         arr = _as_f32_matrix(data, "data")
+        self._fitted_sparse = False
         result = self._core.profile_fit_transform(arr)
         self._sync_fitted(result["embedding"], arr.shape[1], arr.shape[0])
         return result
@@ -494,12 +541,12 @@ class Umap:
         out_buf = np.empty(shape, np.float32) if out is None else _as_out_buffer(out, shape)
         self._core.fit_transform_with_knn_into(arr, idx, dist, out_buf, knn_metric,
                                                validate_precomputed)
+        self._fitted_sparse = False
         self._sync_fitted(out_buf if out is None else out_buf.copy(), arr.shape[1], arr.shape[0])
         return out_buf
 
     def transform(self, query: Any, *, out: np.ndarray | None = None) -> np.ndarray:
-        self._reject_sparse(query)
-        arr = _as_f32_matrix(query, "query")
+        arr = self._query_rows(query)
         shape = (arr.shape[0], self.n_components)
         if out is None:
             return self._core.transform(arr)
@@ -526,12 +573,19 @@ class Umap:
         feature-space SGD, lib.rs:1800-1915).  out: optional writable float32
         buffer of shape (n_samples, n_features); the model must be fit first."""
         arr = _as_f32_matrix(embedded_query, "embedded_query")
+        # lib.rs:1806-1810; the out= buffer is checked first, as the reference's
+        # facade does before its core call
+        sparse_msg = "invalid parameter: inverse_transform is not supported for sparse-trained models yet"
         if out is None:
+            if getattr(self, "_fitted_sparse", False):
+                raise ValueError(sparse_msg)
             return self._core.inverse_transform(arr)
         n_features = self._core.n_features
         if n_features is None:
             raise RuntimeError("model must be fit before inverse_transform(out=...)")
         out_buf = _as_out_buffer(out, (arr.shape[0], n_features))
+        if getattr(self, "_fitted_sparse", False):
+            raise ValueError(sparse_msg)
         self._core.inverse_transform_into(arr, out_buf)
         return out_buf
 
