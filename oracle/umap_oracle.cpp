@@ -1320,6 +1320,109 @@ void optimize_layout_transform(float* emb, int64_t m, int dim, const float* base
   }
 }
 
+/* ------------------------------------------------ approximate kNN ----
+ * lib.rs:2932-3168 approximate_nearest_neighbors_with_distance: random
+ * initial pools, then rounds over neighbours-of-neighbours candidates with a
+ * few random explorations, each round built from the previous table, until
+ * no row changes.  Euclidean and cosine distances (lib.rs:3154-3168,
+ * 3237-3239). */
+uint64_t mix_approx_seed(uint64_t seed, uint64_t row, uint64_t iteration) {
+  uint64_t v = seed ^ (row * 0x9E3779B97F4A7C15ULL) ^ (iteration * 0xBF58476D1CE4E5B9ULL);
+  v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+  return v ^ (v >> 31);
+}
+
+inline bool push_unique(vector<int64_t>& out, int64_t idx) {
+  for (int64_t v : out)
+    if (v == idx) return false;
+  out.push_back(idx);
+  return true;
+}
+
+/* lib.rs:2932-2938 sorted_unique_neighbors (inputs are unique already) */
+void sorted_top(vector<std::pair<int64_t, float>>& pairs, int32_t k) {
+  std::sort(pairs.begin(), pairs.end(), [](const std::pair<int64_t, float>& a,
+                                           const std::pair<int64_t, float>& b) {
+    return nb_less(a.first, a.second, b.first, b.second);
+  });
+  if (static_cast<int32_t>(pairs.size()) > k) pairs.resize(k);
+}
+
+void knn_approx_impl(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
+                     int32_t candidate_pool, int32_t n_iters, uint64_t seed, int nt,
+                     int64_t* out_idx, float* out_dist) {
+  vector<float> norms;
+  if (metric == 1) {
+    norms.resize(n);
+    for (int64_t i = 0; i < n; ++i) norms[i] = l2_norm(x + i * d, d);
+  }
+  auto dist = [&](int64_t i, int64_t j) -> float {
+    if (metric == 1) return cosine_with_norms(x + i * d, x + j * d, d, norms[i], norms[j]);
+    return std::sqrt(sq_dist(x + i * d, x + j * d, d));
+  };
+  const int64_t pool = std::min<int64_t>(std::max<int64_t>(candidate_pool, k), n - 1);
+  const int64_t max_cand = static_cast<int64_t>(std::max<int32_t>(candidate_pool, k)) * 4 + 1;
+  vector<int64_t> tab_i(static_cast<size_t>(n) * k), nxt_i(tab_i.size());
+  vector<float> tab_d(tab_i.size()), nxt_d(tab_i.size());
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+  for (int64_t i = 0; i < n; ++i) {
+    SmallRng rng(mix_approx_seed(seed, static_cast<uint64_t>(i), 0));
+    vector<int64_t> cand{i};
+    while (static_cast<int64_t>(cand.size()) < pool + 1)
+      push_unique(cand, static_cast<int64_t>(rng.gen_index(static_cast<uint64_t>(n))));
+    vector<std::pair<int64_t, float>> pairs;
+    for (int64_t j : cand) pairs.push_back({j, dist(i, j)});
+    sorted_top(pairs, k);
+    for (int32_t c = 0; c < k; ++c) {
+      tab_i[i * k + c] = pairs[c].first;
+      tab_d[i * k + c] = pairs[c].second;
+    }
+  }
+  const int64_t exploration = std::min<int64_t>(std::max<int64_t>(k / 2, 4), max_cand);
+  for (int32_t it = 0; it < n_iters; ++it) {
+    int changed = 0;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt) reduction(| : changed)
+    for (int64_t i = 0; i < n; ++i) {
+      SmallRng rng(mix_approx_seed(seed, static_cast<uint64_t>(i), static_cast<uint64_t>(it) + 1));
+      /* lib.rs:3053-3081 collect_approx_candidate_indices */
+      vector<int64_t> cand;
+      for (int32_t a = 0; a < k; ++a) {
+        const int64_t nb = tab_i[i * k + a];
+        push_unique(cand, nb);
+        for (int32_t b = 0; b < k; ++b) push_unique(cand, tab_i[nb * k + b]);
+      }
+      push_unique(cand, i);
+      for (int64_t e = 0; e < exploration; ++e)
+        push_unique(cand, static_cast<int64_t>(rng.gen_index(static_cast<uint64_t>(n))));
+      std::sort(cand.begin(), cand.end());
+      if (static_cast<int64_t>(cand.size()) > max_cand) {
+        for (int64_t s2 = 0; s2 < max_cand; ++s2) {
+          const int64_t r = s2 + static_cast<int64_t>(
+                                     rng.gen_index(static_cast<uint64_t>(cand.size() - s2)));
+          std::swap(cand[s2], cand[r]);
+        }
+        cand.resize(max_cand);
+      }
+      vector<std::pair<int64_t, float>> pairs;
+      for (int64_t j : cand) pairs.push_back({j, dist(i, j)});
+      sorted_top(pairs, k);
+      bool same = true;
+      for (int32_t c = 0; c < k; ++c) {
+        same = same && tab_i[i * k + c] == pairs[c].first && tab_d[i * k + c] == pairs[c].second;
+        nxt_i[i * k + c] = pairs[c].first;
+        nxt_d[i * k + c] = pairs[c].second;
+      }
+      if (!same) changed = 1;
+    }
+    tab_i.swap(nxt_i);
+    tab_d.swap(nxt_d);
+    if (!changed) break;
+  }
+  std::copy(tab_i.begin(), tab_i.end(), out_idx);
+  std::copy(tab_d.begin(), tab_d.end(), out_dist);
+}
+
 /* lib.rs:3420-3422 inverse_neighbor_count */
 int32_t inverse_neighbor_count(int32_t n_neighbors, int64_t n_train) {
   return static_cast<int32_t>(std::max<int64_t>(std::min<int64_t>(n_neighbors, n_train), 1));
@@ -1635,6 +1738,13 @@ void orc_knn_metric(const float* q, int64_t m, const float* r, int64_t n, int64_
                     int32_t metric, float metric_p, int32_t n_threads, int64_t* idx,
                     float* dist) {
   knn_impl(q, m, r, n, d, k, metric, n_threads, idx, dist, metric_p);
+}
+
+void orc_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
+                    int32_t candidate_pool, int32_t n_iters, uint64_t seed, int32_t n_threads,
+                    int64_t* idx, float* dist) {
+  const int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
+  knn_approx_impl(x, n, d, k, metric, candidate_pool, n_iters, seed, nt, idx, dist);
 }
 
 void orc_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,

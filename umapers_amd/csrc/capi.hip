@@ -308,8 +308,18 @@ struct umapx_handle {
     if (pre_idx == nullptr) {
       idx.alloc(static_cast<size_t>(n_) * k);
       dist.alloc(static_cast<size_t>(n_) * k);
-      knn_device(d_x, n_, d_x, n_, d_, k, p.metric, p.knn_precision, p.rerank_factor, idx.get(),
-                 dist.get(), kws, s, &kst, p.metric_p);
+      // ann_mode "approximate" (use_approximate_knn with threshold 0) runs the
+      // reference's approximate rounds; auto mode keeps the exact kNN, which
+      // on this GPU is faster than the rounds and exact
+      const bool approx = p.use_approximate_knn && p.approx_knn_threshold == 0 &&
+                          (p.metric == UMAPX_METRIC_EUCLIDEAN || p.metric == UMAPX_METRIC_COSINE);
+      if (approx)
+        knn_approx_device(d_x, n_, d_, k, p.metric, p.approx_knn_candidates, p.approx_knn_iters,
+                          p.random_seed ^ 0xC0FEFEED1234ABCDULL, idx.get(), dist.get(), kws, s);
+      else
+        knn_device(d_x, n_, d_x, n_, d_, k, p.metric, p.knn_precision, p.rerank_factor, idx.get(),
+                   dist.get(), kws, s, &kst, p.metric_p);
+      t.used_approximate_knn = approx ? 1 : 0;
     }
     t.knn_sec = clk.lap();
     if (kst.recorded) {
@@ -413,7 +423,6 @@ struct umapx_handle {
     t.store_state_sec = clk.lap();
     t.n_edges = graph.n_edges;
     t.n_epochs = n_epochs;
-    t.used_approximate_knn = 0;
     t.init_used_spectral = spectral_ok ? 1 : 0;
     t.n_graph_components = static_cast<int32_t>(rep.n_components);
     if (p.transform_mode == UMAPX_TRANSFORM_LEARNED) {
@@ -945,6 +954,33 @@ int32_t umapx_knn_to_reference(const float* q, int64_t m, const float* r, int64_
 int32_t umapx_knn_metric(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
                          int32_t k, int32_t metric, float metric_p, int64_t* idx, float* dist) {
   return knn_host(q, m, r, n, d, k, metric, metric_p, UMAPX_KNN_FP32, 2, idx, dist);
+}
+
+int32_t umapx_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
+                         int32_t candidate_pool, int32_t n_iters, uint64_t seed, int64_t* idx,
+                         float* dist) {
+  return guarded([&] {
+    if (!x || !idx || !dist) throw_invalid("null argument");
+    if (n < 2 || d < 1) throw_invalid("need at least two samples and one feature");
+    if (d > 256) throw_invalid("approximate kNN on the GPU supports at most 256 features");
+    umapx_handle::StreamOwner so;
+    HIP_CHECK(hipStreamCreateWithFlags(&so.s, hipStreamNonBlocking));
+    std::vector<int32_t> hi(static_cast<size_t>(n) * k);
+    {
+      AllocStreamScope alloc_scope(so.s);
+      DevBuf<float> dx(static_cast<size_t>(n) * d);
+      DevBuf<int32_t> di(hi.size());
+      DevBuf<float> dd(hi.size());
+      HIP_CHECK(hipMemcpyAsync(dx.get(), x, 4 * n * d, hipMemcpyHostToDevice, so.s));
+      KnnWorkspace ws;
+      knn_approx_device(dx.get(), n, d, k, metric, candidate_pool, n_iters, seed, di.get(),
+                        dd.get(), ws, so.s);
+      HIP_CHECK(hipMemcpyAsync(hi.data(), di.get(), 4 * hi.size(), hipMemcpyDeviceToHost, so.s));
+      HIP_CHECK(hipMemcpyAsync(dist, dd.get(), 4 * hi.size(), hipMemcpyDeviceToHost, so.s));
+      HIP_CHECK(hipStreamSynchronize(so.s));
+    }
+    idx64_from32(hi, idx);
+  });
 }
 
 int32_t umapx_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric, int32_t precision,
