@@ -67,8 +67,7 @@ typedef struct umapx_params {
   int32_t init;                 /* UMAPX_INIT_SPECTRAL */
   uint64_t random_seed;         /* 42 */
   int32_t use_approximate_knn;  /* with approx_knn_threshold 0 ("approximate" mode): the
-                                   reference's approximate rounds (euclidean / cosine);
-                                   otherwise the exact kNN */
+                                   reference's approximate rounds; otherwise the exact kNN */
   int32_t approx_knn_candidates;
   int32_t approx_knn_iters;
   int32_t approx_knn_threshold;
@@ -188,13 +187,13 @@ int32_t umapx_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metri
 int32_t umapx_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,
                                int64_t d, int32_t k, int32_t metric, int32_t precision,
                                int32_t rerank_factor, int64_t* idx, float* dist);
-/* approximate_nearest_neighbors (lib.rs:2932-3168, 3213-3239): the
- * reference's neighbours-of-neighbours rounds from random pools, euclidean or
- * cosine, on the reference's RNG stream (seed = random_seed ^
+/* approximate_nearest_neighbors (lib.rs:2932-3168, 3213-3281): the
+ * reference's neighbours-of-neighbours rounds from random pools under any
+ * UMAPX_METRIC_* code, on the reference's RNG stream (seed = random_seed ^
  * 0xC0FEFEED1234ABCD in a fit, lib.rs:2326). */
 int32_t umapx_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
-                         int32_t candidate_pool, int32_t n_iters, uint64_t seed, int64_t* idx,
-                         float* dist);
+                         float metric_p, int32_t candidate_pool, int32_t n_iters, uint64_t seed,
+                         int64_t* idx, float* dist);
 /* exact_nearest_neighbors_by / _to_reference_by (lib.rs:2891-2928, 3346-3399)
  * under any UMAPX_METRIC_* code with minkowski's p (self included when q == r
  * and m == n). */

@@ -130,8 +130,9 @@ def lib() -> ctypes.CDLL:
                                      ctypes.c_int32, _i64p, _f32p]
         L.orc_knn_metric.restype = None
         L.orc_knn_approx.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
-                                     ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                     ctypes.c_uint64, ctypes.c_int32, _i64p, _f32p]
+                                     ctypes.c_int32, ctypes.c_float, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, _i64p,
+                                     _f32p]
         L.orc_knn_approx.restype = None
         L.orc_inverse_transform.argtypes = [ctypes.POINTER(OrcParams), _f32p, ctypes.c_int64,
                                             ctypes.c_int64, _f32p, _f32p, _f32p, ctypes.c_float,
@@ -188,7 +189,8 @@ def knn_metric(q, r, k: int, metric: str, p: float = 2.0, n_threads: int = 0):
 
 
 def knn_approx(x, k: int, metric: str = "euclidean", candidate_pool: int = 50,
-               n_iters: int = 14, seed: int = 42 ^ 0xC0FEFEED1234ABCD, n_threads: int = 0):
+               n_iters: int = 14, seed: int = 42 ^ 0xC0FEFEED1234ABCD, n_threads: int = 0,
+               p: float = 2.0):
     """approximate_nearest_neighbors (lib.rs:2932-3168): the reference's
     neighbours-of-neighbours refinement from random pools; the seed is the
     model's random_seed ^ 0xC0FE_FEED_1234_ABCD (lib.rs:2326)."""
@@ -196,7 +198,7 @@ def knn_approx(x, k: int, metric: str = "euclidean", candidate_pool: int = 50,
     n, d = x.shape
     idx = np.empty((n, k), np.int64)
     dist = np.empty((n, k), np.float32)
-    lib().orc_knn_approx(_p(x, _f32p), n, d, k, METRICS[metric], candidate_pool, n_iters,
+    lib().orc_knn_approx(_p(x, _f32p), n, d, k, METRICS[metric], p, candidate_pool, n_iters,
                          seed & 0xFFFFFFFFFFFFFFFF, n_threads, _p(idx, _i64p), _p(dist, _f32p))
     return idx, dist
 

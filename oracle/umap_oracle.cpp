@@ -1324,8 +1324,7 @@ void optimize_layout_transform(float* emb, int64_t m, int dim, const float* base
  * lib.rs:2932-3168 approximate_nearest_neighbors_with_distance: random
  * initial pools, then rounds over neighbours-of-neighbours candidates with a
  * few random explorations, each round built from the previous table, until
- * no row changes.  Euclidean and cosine distances (lib.rs:3154-3168,
- * 3237-3239). */
+ * no row changes, under any of the metrics (lib.rs:3213-3281). */
 uint64_t mix_approx_seed(uint64_t seed, uint64_t row, uint64_t iteration) {
   uint64_t v = seed ^ (row * 0x9E3779B97F4A7C15ULL) ^ (iteration * 0xBF58476D1CE4E5B9ULL);
   v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -1351,7 +1350,7 @@ void sorted_top(vector<std::pair<int64_t, float>>& pairs, int32_t k) {
 
 void knn_approx_impl(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
                      int32_t candidate_pool, int32_t n_iters, uint64_t seed, int nt,
-                     int64_t* out_idx, float* out_dist) {
+                     int64_t* out_idx, float* out_dist, float metric_p = 2.0f) {
   vector<float> norms;
   if (metric == 1) {
     norms.resize(n);
@@ -1359,6 +1358,7 @@ void knn_approx_impl(const float* x, int64_t n, int64_t d, int32_t k, int32_t me
   }
   auto dist = [&](int64_t i, int64_t j) -> float {
     if (metric == 1) return cosine_with_norms(x + i * d, x + j * d, d, norms[i], norms[j]);
+    if (metric >= 2) return metric_distance(metric, x + i * d, x + j * d, d, metric_p);
     return std::sqrt(sq_dist(x + i * d, x + j * d, d));
   };
   const int64_t pool = std::min<int64_t>(std::max<int64_t>(candidate_pool, k), n - 1);
@@ -1741,10 +1741,10 @@ void orc_knn_metric(const float* q, int64_t m, const float* r, int64_t n, int64_
 }
 
 void orc_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
-                    int32_t candidate_pool, int32_t n_iters, uint64_t seed, int32_t n_threads,
-                    int64_t* idx, float* dist) {
+                    float metric_p, int32_t candidate_pool, int32_t n_iters, uint64_t seed,
+                    int32_t n_threads, int64_t* idx, float* dist) {
   const int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
-  knn_approx_impl(x, n, d, k, metric, candidate_pool, n_iters, seed, nt, idx, dist);
+  knn_approx_impl(x, n, d, k, metric, candidate_pool, n_iters, seed, nt, idx, dist, metric_p);
 }
 
 void orc_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,

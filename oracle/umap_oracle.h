@@ -61,11 +61,11 @@ void orc_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
 void orc_knn_metric(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
                     int32_t k, int32_t metric, float metric_p, int32_t n_threads,
                     int64_t* idx, float* dist);
-/* approximate_nearest_neighbors (lib.rs:2932-3168, 3213-3239), euclidean (0)
- * or cosine (1): the k rows of each sample's table, self included. */
+/* approximate_nearest_neighbors (lib.rs:2932-3168, 3213-3281) under any
+ * metric code: the k entries of each sample's table. */
 void orc_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
-                    int32_t candidate_pool, int32_t n_iters, uint64_t seed, int32_t n_threads,
-                    int64_t* idx, float* dist);
+                    float metric_p, int32_t candidate_pool, int32_t n_iters, uint64_t seed,
+                    int32_t n_threads, int64_t* idx, float* dist);
 void orc_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,
                           int64_t d, int32_t k, int32_t metric,
                           int32_t n_threads, int64_t* idx, float* dist);

@@ -164,12 +164,12 @@ def knn_metric(q, r, k, metric, p=2.0):
 
 
 def knn_approx(x, k, metric="euclidean", candidate_pool=50, n_iters=14,
-               seed=42 ^ 0xC0FEFEED1234ABCD):
+               seed=42 ^ 0xC0FEFEED1234ABCD, p=2.0):
     x = _f32(x)
     n, d = x.shape
     idx = np.empty((n, k), np.int64)
     dist = np.empty((n, k), np.float32)
-    L.check(L.lib().umapx_knn_approx(L.ptr(x), n, d, k, METRIC_CODES[metric], candidate_pool,
+    L.check(L.lib().umapx_knn_approx(L.ptr(x), n, d, k, METRIC_CODES[metric], p, candidate_pool,
                                      n_iters, seed & 0xFFFFFFFFFFFFFFFF, L.ptr(idx, L._i64p),
                                      L.ptr(dist)))
     return idx, dist

@@ -19,7 +19,8 @@ def _x(n, d, seed):
     return (c[rng.integers(0, 8, n)] + rng.normal(size=(n, d))).astype(np.float32)
 
 
-@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+@pytest.mark.parametrize("metric", ["euclidean", "cosine", "manhattan", "chebyshev", "minkowski",
+                                    "correlation", "canberra", "braycurtis"])
 def test_full_pool_start_is_exact(metric):
     x = _x(60, 12, 1)
     ia, da = O.knn_approx(x, 8, metric, candidate_pool=59, n_iters=0)

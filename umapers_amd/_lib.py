@@ -142,8 +142,8 @@ SIGNATURES = {
                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                             ctypes.c_float, _f32p, ctypes.c_int64, _f32p]),
     "umapx_knn_approx": (ctypes.c_int32, [_f32p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
-                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                          ctypes.c_uint64, _i64p, _f32p]),
+                                          ctypes.c_int32, ctypes.c_float, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_uint64, _i64p, _f32p]),
     "umapx_knn_metric": (ctypes.c_int32, [_f32p, ctypes.c_int64, _f32p, ctypes.c_int64,
                                           ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_float, _i64p, _f32p]),

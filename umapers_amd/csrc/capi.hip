@@ -311,11 +311,11 @@ struct umapx_handle {
       // ann_mode "approximate" (use_approximate_knn with threshold 0) runs the
       // reference's approximate rounds; auto mode keeps the exact kNN, which
       // on this GPU is faster than the rounds and exact
-      const bool approx = p.use_approximate_knn && p.approx_knn_threshold == 0 &&
-                          (p.metric == UMAPX_METRIC_EUCLIDEAN || p.metric == UMAPX_METRIC_COSINE);
+      const bool approx = p.use_approximate_knn && p.approx_knn_threshold == 0;
       if (approx)
-        knn_approx_device(d_x, n_, d_, k, p.metric, p.approx_knn_candidates, p.approx_knn_iters,
-                          p.random_seed ^ 0xC0FEFEED1234ABCDULL, idx.get(), dist.get(), kws, s);
+        knn_approx_device(d_x, n_, d_, k, p.metric, p.metric_p, p.approx_knn_candidates,
+                          p.approx_knn_iters, p.random_seed ^ 0xC0FEFEED1234ABCDULL, idx.get(),
+                          dist.get(), kws, s);
       else
         knn_device(d_x, n_, d_x, n_, d_, k, p.metric, p.knn_precision, p.rerank_factor, idx.get(),
                    dist.get(), kws, s, &kst, p.metric_p);
@@ -957,8 +957,8 @@ int32_t umapx_knn_metric(const float* q, int64_t m, const float* r, int64_t n, i
 }
 
 int32_t umapx_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
-                         int32_t candidate_pool, int32_t n_iters, uint64_t seed, int64_t* idx,
-                         float* dist) {
+                         float metric_p, int32_t candidate_pool, int32_t n_iters, uint64_t seed,
+                         int64_t* idx, float* dist) {
   return guarded([&] {
     if (!x || !idx || !dist) throw_invalid("null argument");
     if (n < 2 || d < 1) throw_invalid("need at least two samples and one feature");
@@ -973,8 +973,9 @@ int32_t umapx_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_
       DevBuf<float> dd(hi.size());
       HIP_CHECK(hipMemcpyAsync(dx.get(), x, 4 * n * d, hipMemcpyHostToDevice, so.s));
       KnnWorkspace ws;
-      knn_approx_device(dx.get(), n, d, k, metric, candidate_pool, n_iters, seed, di.get(),
-                        dd.get(), ws, so.s);
+      validate_metric(metric, metric_p);
+      knn_approx_device(dx.get(), n, d, k, metric, metric_p, candidate_pool, n_iters, seed,
+                        di.get(), dd.get(), ws, so.s);
       HIP_CHECK(hipMemcpyAsync(hi.data(), di.get(), 4 * hi.size(), hipMemcpyDeviceToHost, so.s));
       HIP_CHECK(hipMemcpyAsync(dist, dd.get(), 4 * hi.size(), hipMemcpyDeviceToHost, so.s));
       HIP_CHECK(hipStreamSynchronize(so.s));

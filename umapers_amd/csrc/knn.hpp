@@ -38,10 +38,10 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
                 float metric_p = 2.0f);
 
 // approximate_nearest_neighbors (lib.rs:2932-3168, 3213-3239): the
-// reference's neighbours-of-neighbours rounds from random pools, euclidean
-// (0) or cosine (1), drawn from the reference's RNG stream; out_idx /
-// out_dist [n x k], self included.
-void knn_approx_device(const float* x, int64_t n, int64_t d, int k, int metric,
+// reference's neighbours-of-neighbours rounds from random pools under any
+// metric code, drawn from the reference's RNG stream; out_idx / out_dist
+// [n x k].
+void knn_approx_device(const float* x, int64_t n, int64_t d, int k, int metric, float metric_p,
                        int candidate_pool, int n_iters, uint64_t seed, int32_t* out_idx,
                        float* out_dist, KnnWorkspace& ws, hipStream_t s);
 

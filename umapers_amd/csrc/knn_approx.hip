@@ -1,6 +1,6 @@
 // Approximate kNN: the reference's neighbours-of-neighbours refinement
-// (approximate_nearest_neighbors_with_distance, lib.rs:2932-3168) for the
-// euclidean and cosine metrics (lib.rs:3154-3168, 3213-3239).
+// (approximate_nearest_neighbors_with_distance, lib.rs:2932-3168) under any
+// of the reference's metrics (lib.rs:3213-3281).
 //
 // Each sample starts from a random pool (itself plus pool draws), and each
 // round rebuilds every row from the previous table: the row's neighbours and
@@ -14,7 +14,8 @@
 // rejection zone), drawn by one lane in the reference's order.  Candidate
 // lists are sorted in LDS (bitonic), distances are computed one candidate per
 // lane with the reference's f32 operations (eight lane sums for d >= 64), so
-// a table equals the oracle's restatement entry for entry.
+// a table equals the oracle's restatement entry for entry (minkowski aside:
+// its powf is the device's).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -123,13 +124,84 @@ __device__ float pair_sum(const float* __restrict__ x, const float* __restrict__
   return sum;
 }
 
-// euclidean (sqrt of the squared sum) or cosine_distance_with_norms
-// (lib.rs:2740-2765)
+// The other metrics one pair at a time in the reference's f32 operations
+// (lib.rs:2494-2698): manhattan's eight lane sums for d >= 64, chebyshev's
+// running max, minkowski (device powf), correlation's five sums, canberra,
+// braycurtis.
+__device__ float metric_pair(const float* __restrict__ a, const float* __restrict__ b, int d,
+                             int metric, float p) {
+  if (metric == 2) {
+    if (d < 64) {
+      float acc = 0.f;
+      for (int f = 0; f < d; ++f) acc = fadd(acc, fabsf(fsub(a[f], b[f])));
+      return acc;
+    }
+    float lanes[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int f = 0;
+    for (; f + 8 <= d; f += 8) {
+#pragma unroll
+      for (int l = 0; l < 8; ++l) lanes[l] = fadd(lanes[l], fabsf(fsub(a[f + l], b[f + l])));
+    }
+    float sum = 0.f;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) sum = fadd(sum, lanes[l]);
+    for (; f < d; ++f) sum = fadd(sum, fabsf(fsub(a[f], b[f])));
+    return sum;
+  }
+  if (metric == 3) {
+    float m = 0.f;
+    for (int f = 0; f < d; ++f) m = fmaxf(m, fabsf(fsub(a[f], b[f])));
+    return m;
+  }
+  if (metric == 4) {
+    float acc = 0.f;
+    for (int f = 0; f < d; ++f) acc = fadd(acc, powf(fabsf(fsub(a[f], b[f])), p));
+    return powf(acc, fdiv(1.f, p));
+  }
+  if (metric == 5) {
+    float sx = 0.f, sy = 0.f, qx = 0.f, qy = 0.f, dt = 0.f;
+    for (int f = 0; f < d; ++f) {
+      sx = fadd(sx, a[f]);
+      sy = fadd(sy, b[f]);
+      qx = fadd(qx, fmul(a[f], a[f]));
+      qy = fadd(qy, fmul(b[f], b[f]));
+      dt = fadd(dt, fmul(a[f], b[f]));
+    }
+    const float nf = static_cast<float>(d);
+    const float cdot = fsub(dt, fdiv(fmul(sx, sy), nf));
+    const float xv = fmaxf(fsub(qx, fdiv(fmul(sx, sx), nf)), 0.f);
+    const float yv = fmaxf(fsub(qy, fdiv(fmul(sy, sy), nf)), 0.f);
+    const float xn = sqrtf(xv), yn = sqrtf(yv);
+    if (xn == 0.f && yn == 0.f) return 0.f;
+    if (xn == 0.f || yn == 0.f) return 1.f;
+    float c = fdiv(cdot, fmul(xn, yn));
+    c = c < -1.f ? -1.f : (c > 1.f ? 1.f : c);
+    return fsub(1.f, c);
+  }
+  if (metric == 6) {
+    float acc = 0.f;
+    for (int f = 0; f < d; ++f) {
+      const float den = fadd(fabsf(a[f]), fabsf(b[f]));
+      if (den > 0.f) acc = fadd(acc, fdiv(fabsf(fsub(a[f], b[f])), den));
+    }
+    return acc;
+  }
+  float num = 0.f, den = 0.f;
+  for (int f = 0; f < d; ++f) {
+    num = fadd(num, fabsf(fsub(a[f], b[f])));
+    den = fadd(den, fabsf(fadd(a[f], b[f])));
+  }
+  return den == 0.f ? 0.f : fdiv(num, den);
+}
+
+// euclidean (sqrt of the squared sum), cosine_distance_with_norms
+// (lib.rs:2740-2765) or one of the other metrics
 __device__ float approx_distance(const float* __restrict__ x, int64_t d, int metric,
-                                 const float* __restrict__ norms, int64_t i, int64_t j) {
+                                 const float* __restrict__ norms, int64_t i, int64_t j, float p) {
   const float* a = x + i * d;
   const float* b = x + j * d;
   if (metric == 0) return sqrtf(pair_sum<false>(a, b, static_cast<int>(d)));
+  if (metric >= 2) return metric_pair(a, b, static_cast<int>(d), metric, p);
   const float nx = norms[i], ny = norms[j];
   const float dt = pair_sum<true>(a, b, static_cast<int>(d));
   if (nx == 0.f && ny == 0.f) return 0.f;
@@ -162,11 +234,11 @@ __device__ bool finish_row(const float* __restrict__ x, int64_t d, int metric,
                            const float* __restrict__ norms, int64_t i, const int32_t* cand, int C,
                            uint64_t* keys, int k, const int32_t* __restrict__ old_idx,
                            const float* __restrict__ old_dist, int32_t* __restrict__ new_idx,
-                           float* __restrict__ new_dist, int lane) {
+                           float* __restrict__ new_dist, int lane, float p) {
   int P = 64;
   while (P < C) P <<= 1;
   for (int c = lane; c < P; c += 64)
-    keys[c] = c < C ? pair_key(approx_distance(x, d, metric, norms, i, cand[c]), cand[c])
+    keys[c] = c < C ? pair_key(approx_distance(x, d, metric, norms, i, cand[c], p), cand[c])
                     : ~uint64_t{0};
   wave_sync_lds();
   wave_sort(keys, P, lane);
@@ -186,7 +258,8 @@ __device__ bool finish_row(const float* __restrict__ x, int64_t d, int metric,
 // until pool + 1 distinct candidates; their k nearest start the table.
 __global__ __launch_bounds__(64 * kApproxWaves) void approx_init_kernel(
     const float* __restrict__ x, int64_t n, int64_t d, int metric, const float* __restrict__ norms,
-    int k, int pool, uint64_t seed, int32_t* __restrict__ tab_idx, float* __restrict__ tab_dist) {
+    int k, int pool, uint64_t seed, int32_t* __restrict__ tab_idx, float* __restrict__ tab_dist,
+    float p) {
   __shared__ int32_t s_cand[kApproxWaves][kKeyCap];
   __shared__ uint64_t s_keys[kApproxWaves][kKeyCap];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -206,7 +279,7 @@ __global__ __launch_bounds__(64 * kApproxWaves) void approx_init_kernel(
   }
   wave_sync_lds();
   finish_row(x, d, metric, norms, i, cand, pool + 1, s_keys[wv], k, nullptr, nullptr, tab_idx,
-             tab_dist, lane);
+             tab_dist, lane, p);
 }
 
 // One round (lib.rs:3046-3081 collect_approx_candidate_indices, 3120-3140).
@@ -214,7 +287,8 @@ __global__ __launch_bounds__(64 * kApproxWaves) void approx_round_kernel(
     const float* __restrict__ x, int64_t n, int64_t d, int metric, const float* __restrict__ norms,
     int k, int max_cand, int exploration, uint64_t seed, int round,
     const int32_t* __restrict__ old_idx, const float* __restrict__ old_dist,
-    int32_t* __restrict__ new_idx, float* __restrict__ new_dist, int32_t* __restrict__ changed) {
+    int32_t* __restrict__ new_idx, float* __restrict__ new_dist, int32_t* __restrict__ changed,
+    float p) {
   __shared__ int32_t s_cand[kApproxWaves][kCandCap];
   __shared__ uint64_t s_keys[kApproxWaves][kKeyCap];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -285,17 +359,17 @@ __global__ __launch_bounds__(64 * kApproxWaves) void approx_round_kernel(
   wave_sync_lds();
   const int C = total < max_cand ? total : max_cand;
   if (finish_row(x, d, metric, norms, i, cand, C, s_keys[wv], k, old_idx, old_dist, new_idx,
-                 new_dist, lane) &&
+                 new_dist, lane, p) &&
       lane == 0)
     atomicOr(changed, 1);
 }
 
 }  // namespace
 
-void knn_approx_device(const float* x, int64_t n, int64_t d, int k, int metric,
+void knn_approx_device(const float* x, int64_t n, int64_t d, int k, int metric, float metric_p,
                        int candidate_pool, int n_iters, uint64_t seed, int32_t* out_idx,
                        float* out_dist, KnnWorkspace& ws, hipStream_t s) {
-  if (metric != 0 && metric != 1) throw_invalid("approximate kNN supports euclidean and cosine");
+  if (metric < 0 || metric > 7) throw_invalid("unknown metric code");
   if (k < 1 || k > 32 || k >= n) throw_invalid("n_neighbors must be <= 32 and below the sample count");
   if (candidate_pool < 1 || candidate_pool > 255)
     throw_invalid("approx_knn_candidates must be in [1, 255] on the GPU path");
@@ -319,12 +393,12 @@ void knn_approx_device(const float* x, int64_t n, int64_t d, int k, int metric,
   float* nxt_d = dist2.get();
   const dim3 grid(blocks_for(n, kApproxWaves)), block(64 * kApproxWaves);
   hipLaunchKernelGGL(approx_init_kernel, grid, block, 0, s, x, n, d, metric, norms, k, pool, seed,
-                     cur_i, cur_d);
+                     cur_i, cur_d, metric_p);
   KERNEL_CHECK();
   for (int round = 0; round < n_iters; ++round) {
     HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int32_t), s));
     hipLaunchKernelGGL(approx_round_kernel, grid, block, 0, s, x, n, d, metric, norms, k, max_cand,
-                       exploration, seed, round, cur_i, cur_d, nxt_i, nxt_d, flag.get());
+                       exploration, seed, round, cur_i, cur_d, nxt_i, nxt_d, flag.get(), metric_p);
     KERNEL_CHECK();
     int32_t changed = 0;
     HIP_CHECK(hipMemcpyAsync(&changed, flag.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
