@@ -126,8 +126,8 @@ def test_sparse_input_rules(fake):
         Umap(output_dens=True).fit(sp.csr_matrix(x))
     with pytest.raises(ValueError, match="Canberra metric is currently supported for dense input only"):
         Umap(metric="canberra").fit(sp.csr_matrix(x))
-    with pytest.raises(ValueError, match="at most 256 features"):
-        Umap().fit(sp.csr_matrix((10, 300), dtype=np.float32))
+    with pytest.raises(ValueError, match="at most 2560 features"):
+        Umap().fit(sp.csr_matrix((10, 3000), dtype=np.float32))
     dense = Umap(n_neighbors=5, n_epochs=10)
     dense.fit(x)
     with pytest.raises(ValueError, match="supported only for sparse-fitted models"):

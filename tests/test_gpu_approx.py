@@ -24,7 +24,7 @@ def _x(n, d, seed):
 
 @pytest.mark.parametrize("metric", ["euclidean", "cosine"])
 @pytest.mark.parametrize("n,d,k,iters", [(1500, 16, 10, 0), (1500, 16, 10, 2), (2500, 70, 15, 14),
-                                         (900, 128, 30, 3)])
+                                         (900, 128, 30, 3), (700, 420, 15, 4)])
 def test_table_equals_oracle(gpu, oracle, metric, n, d, k, iters):
     x = _x(n, d, n + d)
     seed = 1234 ^ 0xC0FEFEED1234ABCD

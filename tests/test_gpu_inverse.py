@@ -96,3 +96,23 @@ def test_inverse_api_out_buffer_empty_and_errors(gpu):
         m.inverse_transform(bad)
     with pytest.raises(TypeError):
         m.inverse_transform(t, out=np.empty((18, 10), np.float64))
+
+
+@pytest.mark.parametrize("d", [784, 1500])
+def test_wide_rows_fit_and_inverse(gpu, oracle, d):
+    # rows wider than 256 features: the GEMM-scored kNN in the fit, then the
+    # inverse SGD with 16 (d <= 1024) or 40 features per lane
+    x_all = _mixture(2300, d, 6, d)
+    x, hold = x_all[:2000], x_all[2000:]
+    m = Umap(n_neighbors=15, n_epochs=100, random_seed=4, ann_mode="exact")
+    emb = m.fit_transform(x)
+    assert np.array_equal(m.inverse_transform(emb[:16]), x[:16])
+    q = m.transform(hold)
+    gpu_rec = m.inverse_transform(q)
+    idx, dist = oracle.knn(x, 15)
+    sig, rho = oracle.smooth_knn_dist(dist, 15.0)
+    p = oracle.OracleParams(n_neighbors=15, n_epochs=100, random_seed=4)
+    orc_rec = oracle.inverse_transform(x, emb, sig, rho, q, p, optimize=True)
+    assert np.all(np.isfinite(gpu_rec))
+    rmse = lambda r: float(np.sqrt(np.mean((r - hold) ** 2)))  # noqa: E731
+    assert abs(rmse(gpu_rec) - rmse(orc_rec)) <= 0.03 * rmse(orc_rec)

@@ -101,3 +101,59 @@ def test_knn_to_reference_many_queries(gpu, oracle, rng):
         oi, od = oracle.knn_to_reference(q, r, 15, "cosine" if metric else "euclidean")
         assert np.array_equal(idx, oi)
         assert np.array_equal(dist, od)
+
+
+# Rows wider than 256 features take the GEMM score-block path with a
+# selection kernel; the re-rank makes FP32 results equal to the oracle's.
+WIDE = [(700, 300, 15), (1100, 784, 15), (400, 520, 30), (257, 257, 5)]
+
+
+@pytest.mark.parametrize("n,d,k", WIDE)
+@pytest.mark.parametrize("metric", [0, 1])
+def test_wide_rows_fp32_bit_exact_vs_oracle(gpu, oracle, rng, n, d, k, metric):
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    idx, dist = capi.knn(x, k, metric, 0)
+    oi, od = oracle.knn(x, k, "cosine" if metric else "euclidean")
+    assert np.array_equal(idx, oi)
+    assert np.array_equal(dist.view(np.int32), od.view(np.int32))
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_wide_rows_bf16_recall(gpu, oracle, metric):
+    x = datasets.gaussian_mixture(2500, 400, 8, 3.0, 7, (0.5, 1.5))
+    idx, dist = capi.knn(x, 15, metric, 1, 2)
+    oi, od = oracle.knn(x, 15, "cosine" if metric else "euclidean")
+    assert recall_at_k(idx, oi) >= 0.99
+    same = idx == oi
+    assert np.array_equal(dist[same].view(np.int32), od[same].view(np.int32))
+
+
+def test_wide_rows_many_duplicates(gpu, oracle, rng):
+    # 1500 copies of one row: their score rows hold more than a thousand
+    # equal values at the top, which the selection resolves by bisection
+    base = rng.normal(size=(1, 320)).astype(np.float32)
+    x = np.concatenate([np.repeat(base, 1500, axis=0),
+                        rng.normal(size=(600, 320)).astype(np.float32)])
+    idx, dist = capi.knn(x, 10, 0, 0)
+    oi, od = oracle.knn(x, 10)
+    assert np.array_equal(idx, oi)
+    assert np.array_equal(dist, od)
+
+
+def test_wide_rows_to_reference(gpu, oracle, rng):
+    r = rng.normal(size=(900, 600)).astype(np.float32)
+    q = rng.normal(size=(130, 600)).astype(np.float32)
+    for metric in (0, 1):
+        idx, dist = capi.knn_to_reference(q, r, 15, metric)
+        oi, od = oracle.knn_to_reference(q, r, 15, "cosine" if metric else "euclidean")
+        assert np.array_equal(idx, oi)
+        assert np.array_equal(dist, od)
+
+
+def test_wide_rows_fit_parity(gpu, oracle):
+    from umapers_amd import Umap
+    from umapers_amd.diagnostics import trustworthiness
+    x = datasets.gaussian_mixture(3000, 784, 8, 3.0, 13, (0.5, 1.5))
+    g = Umap(n_epochs=200, random_seed=5, ann_mode="exact").fit_transform(x)
+    fit = oracle.fit(x, oracle.OracleParams(n_epochs=200, random_seed=5))
+    assert abs(trustworthiness(x, g) - trustworthiness(x, fit.embedding)) <= 0.01

@@ -25,7 +25,8 @@ def _data(n, d, seed):
 
 
 @pytest.mark.parametrize("metric", EXACT)
-@pytest.mark.parametrize("n,d,k", [(700, 40, 10), (1000, 70, 15), (613, 128, 20), (300, 7, 5)])
+@pytest.mark.parametrize("n,d,k", [(700, 40, 10), (1000, 70, 15), (613, 128, 20), (300, 7, 5),
+                                   (400, 300, 10), (330, 778, 15)])
 def test_self_knn_bit_exact(gpu, oracle, metric, n, d, k):
     x = _data(n, d, n + d)
     gi, gd = capi.knn_metric(x, x, k, metric)
@@ -71,3 +72,9 @@ def test_minkowski_p_reaches_the_kernel(gpu):
     _, d1 = capi.knn_metric(x, x, 8, "minkowski", 1.0)
     _, dm = capi.knn_metric(x, x, 8, "manhattan")
     assert np.allclose(d1, dm, rtol=1e-5, atol=1e-5)
+
+
+def test_too_wide_for_the_metric_kernel_is_an_error(gpu):
+    x = np.zeros((40, 2561), np.float32)
+    with pytest.raises(Exception, match="at most 2560 features"):
+        capi.knn_metric(x, x, 5, "manhattan")

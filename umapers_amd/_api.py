@@ -66,9 +66,9 @@ def _is_sparse(x: Any) -> bool:
 
 # Sparse rows are densified on the host and run the dense path: the
 # reference's own sparse kNN equals its dense kNN on the same values
-# (lib.rs sparse_csr_knn_matches_dense_euclidean), and the GPU kernels take
-# at most this many features.
-_SPARSE_MAX_FEATURES = 256
+# (lib.rs sparse_csr_knn_matches_dense_euclidean), and the GPU kNN under
+# every sparse-capable metric takes at most this many features.
+_SPARSE_MAX_FEATURES = 2560
 
 
 def _sparse_to_dense(x: Any, name: str) -> np.ndarray:

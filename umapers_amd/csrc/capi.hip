@@ -962,7 +962,6 @@ int32_t umapx_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_
   return guarded([&] {
     if (!x || !idx || !dist) throw_invalid("null argument");
     if (n < 2 || d < 1) throw_invalid("need at least two samples and one feature");
-    if (d > 256) throw_invalid("approximate kNN on the GPU supports at most 256 features");
     umapx_handle::StreamOwner so;
     HIP_CHECK(hipStreamCreateWithFlags(&so.s, hipStreamNonBlocking));
     std::vector<int32_t> hi(static_cast<size_t>(n) * k);

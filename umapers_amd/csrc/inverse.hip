@@ -449,7 +449,7 @@ void inverse_transform_device(const float* q, int64_t m, int dim, const float* y
   if (m == 0) return;
   if (dim < 1 || dim > kMaxLowDim) throw_invalid("n_components must be <= 8 on the GPU path");
   if (args.n_epochs > 0 && d > kInverseMaxFeatures)
-    throw_invalid("inverse_transform on the GPU supports at most 512 features");
+    throw_invalid("inverse_transform on the GPU supports at most 2560 features");
   if (n >= (int64_t{1} << 31)) throw_invalid("inverse_transform supports fewer than 2^31 training rows");
   const int k = args.k;
   if (k < 1 || k > kInvMaxK || k > n) throw_invalid("inverse_transform: invalid neighbour count");
@@ -487,8 +487,12 @@ void inverse_transform_device(const float* q, int64_t m, int dim, const float* y
       sgd(inverse_sgd_kernel<8, 16>, 16);
     else if (d <= 256)
       sgd(inverse_sgd_kernel<8, 32>, 32);
-    else
+    else if (d <= 512)
       sgd(inverse_sgd_kernel<8, 64>, 64);
+    else if (d <= 1024)
+      sgd(inverse_sgd_kernel<16, 64>, 64);
+    else
+      sgd(inverse_sgd_kernel<40, 64>, 64);
   }
   // the temporaries above are released in stream order after the launches
 }

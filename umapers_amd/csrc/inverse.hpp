@@ -17,8 +17,9 @@ struct InverseArgs {
   uint64_t seed = 0;
 };
 
-// Largest feature count the inverse SGD holds in registers (8 per lane).
-constexpr int64_t kInverseMaxFeatures = 512;
+// Largest feature count the inverse SGD holds in registers (up to 40 per
+// lane of a 64-lane group).
+constexpr int64_t kInverseMaxFeatures = 2560;
 
 // q [m x dim] embedded rows (device), y_train [n x dim] the fitted embedding,
 // x_train [n x d] the training rows, sigmas / rhos [n] the fit's; out [m x d].

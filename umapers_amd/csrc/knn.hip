@@ -41,16 +41,13 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
 // Column sums in f64 (for mean-centring).  grid-stride over rows.
 __global__ void column_sums_kernel(const float* __restrict__ x, int64_t n, int d,
                                    int normalize, double* __restrict__ colsum) {
-  // one thread per feature column within a block-row chunk
-  const int f = threadIdx.x;
-  if (f >= d) return;
-  double s = 0.0;
-  for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
-    float v = x[r * d + f];
-    s += static_cast<double>(v);
-  }
+  // one thread per feature column (or several, past 1024 columns)
   (void)normalize;
-  atomicAdd(&colsum[f], s);
+  for (int f = threadIdx.x; f < d; f += blockDim.x) {
+    double s = 0.0;
+    for (int64_t r = blockIdx.x; r < n; r += gridDim.x) s += static_cast<double>(x[r * d + f]);
+    atomicAdd(&colsum[f], s);
+  }
 }
 
 // Column sums of the L2-normalised rows (cosine).  One wave per row.
@@ -750,8 +747,13 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
     knn_metric_device(q, m, r, n, d, k, metric, metric_p, out_idx, out_dist, ws, s);
     return;
   }
-  const int dpad = knn_padded_dim(d);
-  if (dpad < 0) throw_invalid("exact GPU kNN supports at most 256 features");
+  int dpad = knn_padded_dim(d);
+  const bool wide = dpad < 0;  // past 256 features: GEMM score blocks + selection
+  if (wide) {
+    // the cosine column sums keep one f64 per feature in LDS
+    if (d > 16384) throw_invalid("exact GPU kNN supports at most 16384 features");
+    dpad = static_cast<int>((d + 63) / 64 * 64);
+  }
   const int kc = knn_candidate_count(k, rerank_factor);
   if (kc < 0) throw_invalid("n_neighbors must be <= 32 and n_neighbors * rerank_factor <= 64");
   if (k > n) throw_invalid("n_neighbors must not exceed the number of reference rows");
@@ -764,10 +766,14 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
   HIP_CHECK(hipMemsetAsync(ws.colsum.get(), 0, sizeof(double) * d, s));
   const unsigned cs_blocks = static_cast<unsigned>(std::min<int64_t>(n, 1024));
   if (metric == 1) {
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(column_sums_normalized_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(sizeof(double) * d)));
     hipLaunchKernelGGL(column_sums_normalized_kernel, dim3(cs_blocks), dim3(256),
                        sizeof(double) * d, s, r, n, static_cast<int>(d), ws.colsum.get());
   } else {
-    hipLaunchKernelGGL(column_sums_kernel, dim3(cs_blocks), dim3(std::max<int64_t>(64, ((d + 63) / 64) * 64)),
+    hipLaunchKernelGGL(column_sums_kernel, dim3(cs_blocks),
+                       dim3(std::min<int64_t>(1024, std::max<int64_t>(64, ((d + 63) / 64) * 64))),
                        0, s, r, n, static_cast<int>(d), 0, ws.colsum.get());
   }
   KERNEL_CHECK();
@@ -797,7 +803,10 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
 
   ws.cand.ensure(static_cast<size_t>(m) * kc);
   if (stats) HIP_CHECK(hipEventRecord(stats->cand_start, s));
-  if (bf16) {
+  if (wide) {
+    knn_wide_candidates(qp, m, ws.rprep.get(), ws.rnorm2.get(), r, n, d, dpad, bf16, kc,
+                        ws.cand.get(), ws, s);
+  } else if (bf16) {
     if (kc == 32)
       dispatch_dpad<uint16_t, 16>(dpad, static_cast<const uint16_t*>(qp), m,
                                   reinterpret_cast<const uint16_t*>(ws.rprep.get()),

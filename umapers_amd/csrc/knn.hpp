@@ -13,6 +13,8 @@ struct KnnWorkspace {
   DevBuf<int32_t> cand;
   DevBuf<uint2> pairs;  // per-split candidate lists when the reference rows are split
   DevBuf<float> msum;   // per-row sums / sums of squares (correlation)
+  DevBuf<float> scores; // score block of the wide-row path, or the metric
+                        // kernel's query tiles of rows wider than 512
 };
 
 // Events around the candidate (MFMA) kernel for live roofline figures.
@@ -23,6 +25,14 @@ struct KnnStats {
 };
 
 int knn_padded_dim(int64_t d);
+
+// Candidates for rows wider than 256 features: score blocks Q R^T by GEMM,
+// then the kc largest q.r + nr_half[r] per query (kc 32 or 64).  qp / rp are
+// the prepared [m x dpad] / [n x dpad] rows (BF16 when bf16, else FP32); r
+// the original [n x d] reference rows (to find identical rows).
+void knn_wide_candidates(const void* qp, int64_t m, const void* rp, const float* nr_half,
+                         const float* r, int64_t n, int64_t d, int dpad, bool bf16, int kc,
+                         int32_t* cand, KnnWorkspace& ws, hipStream_t s);
 int knn_candidate_count(int k, int rerank_factor);
 
 // Top-k of every query row q[m x d] against r[n x d] (device pointers).  When

@@ -84,16 +84,19 @@ __global__ void row_summaries_kernel(const float* __restrict__ x, int64_t rows, 
 }
 
 // METRIC: 2 manhattan, 3 chebyshev, 4 minkowski, 5 correlation, 6 canberra,
-// 7 braycurtis.  H: kept candidates per query (>= k).
-template <int METRIC, int H>
+// 7 braycurtis.  H: kept candidates per query (>= k).  QG: the transposed
+// query tile lives in the block's slice of qglob (HBM, read through the
+// caches) instead of LDS, for rows too wide for both tiles in LDS.
+template <int METRIC, int H, bool QG>
 __global__ __launch_bounds__(64) void knn_metric_kernel(
     const float* __restrict__ q, int64_t m, const float* __restrict__ r, int64_t n, int d, int k,
     float p, const float* __restrict__ qsum, const float* __restrict__ qsq,
     const float* __restrict__ rsum, const float* __restrict__ rsq, int32_t* __restrict__ out_idx,
-    float* __restrict__ out_dist) {
+    float* __restrict__ out_dist, float* __restrict__ qglob) {
   extern __shared__ float lds[];
-  float* qt = lds;           // [d / 4][64][4], or [d][64] when d % 4 != 0
-  float* rt = lds + 64 * d;  // [kMetricRows][d]
+  // [d / 4][64][4], or [d][64] when d % 4 != 0
+  float* qt = QG ? qglob + static_cast<int64_t>(blockIdx.x) * 64 * d : lds;
+  float* rt = QG ? lds : lds + 64 * d;  // [kMetricRows][d]
   const int lane = threadIdx.x;
   const int64_t q0 = static_cast<int64_t>(blockIdx.x) * 64;
   const int64_t i = q0 + lane;
@@ -294,30 +297,38 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
     }
 }
 
+// Rows up to kMetricLdsFeatures keep both tiles in LDS; wider rows (up to
+// kMetricMaxFeatures, the reference tile alone in LDS) keep the query tile
+// in HBM.
+constexpr int kMetricLdsFeatures = 160 * 1024 / ((64 + kMetricRows) * 4);  // 512
+constexpr int kMetricMaxFeatures = 160 * 1024 / (kMetricRows * 4);         // 2560
+
 template <int METRIC, int H>
 void launch_metric(const float* q, int64_t m, const float* r, int64_t n, int d, int k, float p,
                    const float* qs, const float* qq, const float* rs, const float* rq,
-                   int32_t* out_idx, float* out_dist, hipStream_t s) {
-  const size_t bytes = static_cast<size_t>(64 + kMetricRows) * d * sizeof(float);
-  auto kern = knn_metric_kernel<METRIC, H>;
+                   int32_t* out_idx, float* out_dist, float* qglob, hipStream_t s) {
+  const bool qg = d > kMetricLdsFeatures;
+  const size_t bytes = static_cast<size_t>(qg ? kMetricRows : 64 + kMetricRows) * d * sizeof(float);
+  auto kern = qg ? knn_metric_kernel<METRIC, H, true> : knn_metric_kernel<METRIC, H, false>;
   HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
   hipLaunchKernelGGL(kern, dim3(blocks_for(m, 64)), dim3(64), bytes, s, q, m, r, n, d, k, p, qs,
-                     qq, rs, rq, out_idx, out_dist);
+                     qq, rs, rq, out_idx, out_dist, qglob);
   KERNEL_CHECK();
 }
 
 template <int H>
 void dispatch_metric(int metric, const float* q, int64_t m, const float* r, int64_t n, int d,
                      int k, float p, const float* qs, const float* qq, const float* rs,
-                     const float* rq, int32_t* out_idx, float* out_dist, hipStream_t s) {
+                     const float* rq, int32_t* out_idx, float* out_dist, float* qg,
+                     hipStream_t s) {
   switch (metric) {
-    case 2: launch_metric<2, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
-    case 3: launch_metric<3, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
-    case 4: launch_metric<4, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
-    case 5: launch_metric<5, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
-    case 6: launch_metric<6, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
-    default: launch_metric<7, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, s); break;
+    case 2: launch_metric<2, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
+    case 3: launch_metric<3, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
+    case 4: launch_metric<4, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
+    case 5: launch_metric<5, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
+    case 6: launch_metric<6, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
+    default: launch_metric<7, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
   }
 }
 
@@ -327,7 +338,8 @@ void knn_metric_device(const float* q, int64_t m, const float* r, int64_t n, int
                        int metric, float p, int32_t* out_idx, float* out_dist, KnnWorkspace& ws,
                        hipStream_t s) {
   if (metric < 2 || metric > 7) throw_invalid("unknown metric code");
-  if (d < 1 || d > 256) throw_invalid("exact GPU kNN supports at most 256 features");
+  if (d < 1 || d > kMetricMaxFeatures)
+    throw_invalid("exact GPU kNN under this metric supports at most 2560 features");
   if (k < 1 || k > 32 || k > n) throw_invalid("n_neighbors must be <= 32 on the GPU path");
   if (n >= (int64_t{1} << 31)) throw_invalid("exact GPU kNN supports fewer than 2^31 rows");
   if (m == 0) return;
@@ -349,10 +361,15 @@ void knn_metric_device(const float* q, int64_t m, const float* r, int64_t n, int
     qq = base + 2 * n + m;
   }
   const int dd = static_cast<int>(d);
+  float* qg = nullptr;  // per-block transposed query tiles of wide rows
+  if (dd > kMetricLdsFeatures) {
+    ws.scores.ensure(static_cast<size_t>(blocks_for(m, 64)) * 64 * dd);
+    qg = ws.scores.get();
+  }
   if (k <= 16)
-    dispatch_metric<16>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, s);
+    dispatch_metric<16>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s);
   else
-    dispatch_metric<32>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, s);
+    dispatch_metric<32>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s);
 }
 
 }  // namespace umapx
