@@ -47,11 +47,15 @@ def test_sparse_fit_matches_dense_fit(gpu):
     # bytes; only the Hogwild SGD's float atomics differ between runs
     from umapers_amd.diagnostics import trustworthiness
 
+    # (one run's trustworthiness on these rows varies by about 0.01, so three
+    # seeds are averaged)
     x = _data(2000, 20, 7, 0.3)
-    kw = dict(n_neighbors=10, n_epochs=100, random_seed=5, ann_mode="exact")
-    ta = trustworthiness(x, Umap(**kw).fit_transform(x))
-    tb = trustworthiness(x, Umap(**kw).fit_transform(sp.csr_matrix(x)))
-    assert abs(ta - tb) < 0.01, (ta, tb)
+    ta, tb = [], []
+    for seed in (5, 6, 7):
+        kw = dict(n_neighbors=10, n_epochs=100, random_seed=seed, ann_mode="exact")
+        ta.append(trustworthiness(x, Umap(**kw).fit_transform(x)))
+        tb.append(trustworthiness(x, Umap(**kw).fit_transform(sp.csr_matrix(x))))
+    assert abs(np.mean(ta) - np.mean(tb)) < 0.01, (ta, tb)
 
 
 def test_sparse_trained_inverse_errors(gpu):
