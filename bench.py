@@ -86,6 +86,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=50000,
                     help="rows of the CPU-baseline sample (0 skips that leg)")
     ap.add_argument("--transform-rows", type=int, default=20000)
+    ap.add_argument("--wide-rows", type=int, default=200000,
+                    help="rows of the 784-feature fit leg (0 skips it)")
     ap.add_argument("--mlp-queries", type=int, default=1_000_000,
                     help="rows sent through the learned encoder (0 skips the learned-transform leg)")
     args = ap.parse_args()
@@ -251,6 +253,24 @@ def main() -> int:
     log(f"bench: inverse transform {m} rows in {np.median(ti) * 1e3:.1f} ms, "
         f"rmse {inverse['rmse_vs_query_rows']} (rows' own rms {inverse['query_rms_about_mean']})")
 
+    # ---- rows wider than 256 features: GEMM score blocks + selection ----
+    wide = None
+    if args.wide_rows > 0:
+        from umapers_amd import Umap
+        n_w, d_w = args.wide_rows, 784
+        log(f"bench: wide-row leg ({n_w} x {d_w}, BF16 + re-rank fit)")
+        xw = datasets.gaussian_mixture(n_w, d_w, 10, 3.0, 1, (0.5, 1.5))
+        mw = Umap(n_epochs=200, random_seed=1, ann_mode="exact", knn_precision="bf16_rerank")
+        mw.profile_fit_transform(xw[:5000])  # loads the GEMM library's kernels
+        pw = mw.profile_fit_transform(xw)
+        tw = pw["timings"]
+        wide = {"rows": n_w, "features": d_w, "knn_sec": round(tw["knn_sec"], 4),
+                "fit_sec": round(tw["total_sec"], 4),
+                "knn_tflops": round(2.0 * n_w * n_w * d_w / tw["knn_sec"] / 1e12, 1)}
+        log(f"bench: wide rows {n_w} x {d_w}: kNN {tw['knn_sec'] * 1e3:.1f} ms, "
+            f"fit {tw['total_sec'] * 1e3:.1f} ms")
+        del xw
+
     # ---- learned transform: 100000 x 192 fit, encoder over fresh queries ----
     mlp = None
     if args.mlp_queries > 0:
@@ -356,6 +376,7 @@ def main() -> int:
         "knn_preservation_15": round(kpres, 5),
         "transform_rows_per_sec": round(xform_rps, 1),
         "inverse_transform": inverse,
+        "wide_rows": wide,
         "learned_transform": mlp,
         "stage_sec": {key: round(prof[key], 4) for key in L.TIMING_KEYS},
         "fp32_knn_kernel_ms": round(float(np.median([t["knn_kernel_ms"] for t in tm32])), 3),

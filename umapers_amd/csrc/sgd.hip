@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void sgd_epoch_kernel(
 // covers edges [e_begin, e_end).
 constexpr int64_t kMirrorMinEdges = int64_t{1} << 20;
 constexpr int kRangeLaunches = 8;
-constexpr int kChunkLen = 32;  // longest run of one head's edges walked by one thread
+constexpr int kChunkLen = 4;  // longest run of one head's edges walked by one thread
 constexpr int kPerLane = 2;
 constexpr int kWaveEdges = 64 * kPerLane;
 constexpr int kEpochWaves = 4;  // waves per block of the epoch kernel
@@ -344,8 +344,10 @@ __global__ __launch_bounds__(64 * kEpochWaves) void sgd_epoch2_kernel(
 // consecutive edges of one head walks them in CSR order as the reference's
 // loop does (lib.rs:5180-5241), keeping the head in registers so every later
 // edge and negative of the chunk sees its earlier moves (Gauss-Seidel on the
-// head; a head has one chunk unless its degree exceeds chunk_len, which
-// keeps high-degree hubs from serialising an epoch).  Tail moves are float
+// head within a chunk).  Rows are cut into chunks of chunk_len edges: a
+// launch lasts as long as its longest chain of dependent gathers, so short
+// chunks keep hubs, and rows of ordinary length, from serialising an
+// epoch.  Tail moves are float
 // atomics; the chunk's net head move is added atomically at the end so
 // concurrent moves from other threads are kept.
 __global__ __launch_bounds__(256) void sgd_vertex2_kernel(
