@@ -49,7 +49,7 @@ namespace {
 
 constexpr int kSelT = 8;        // per-lane list of the first pass
 constexpr int kSelCap = 512;    // gathered entries per wave (LDS)
-constexpr int kUnroll = 4;      // 16-byte chunks per lane in flight per step
+constexpr int kUnroll = 2;      // 16-byte chunks per lane in flight per step
 constexpr int kSelWaves = 4;
 constexpr size_t kScoreBudget = size_t(4) << 30;  // bytes of one score block
 
@@ -143,8 +143,8 @@ __global__ void first_copy_kernel(const float* __restrict__ x, int64_t n, int d,
 
 // Score rows have a stride of n rounded up to 4 (16-byte aligned rows): the
 // two main passes read 4 consecutive entries per lane with vector loads,
-// kUnroll of them issued together, so a wave keeps 4 KB of independent
-// loads in flight per step.
+// kUnroll of them issued together (2 keeps the kernel within 64 VGPRs, 8
+// waves per SIMD).
 template <int KC>
 __global__ __launch_bounds__(kSelWaves * 64) void wide_select_kernel(
     const float* __restrict__ scores, int64_t ld, int64_t rows, int64_t n,
