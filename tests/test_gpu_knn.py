@@ -105,7 +105,7 @@ def test_knn_to_reference_many_queries(gpu, oracle, rng):
 
 # Rows wider than 256 features take the GEMM score-block path with a
 # selection kernel; the re-rank makes FP32 results equal to the oracle's.
-WIDE = [(700, 300, 15), (1100, 784, 15), (400, 520, 30), (257, 257, 5)]
+WIDE = [(700, 300, 15), (1100, 784, 15), (400, 520, 30), (257, 257, 5), (20, 300, 5)]
 
 
 @pytest.mark.parametrize("n,d,k", WIDE)

@@ -211,17 +211,19 @@ __global__ __launch_bounds__(kSelWaves * 64) void wide_select_kernel(
   // appends the chunk's entries >= th to keys, in index order (lane by lane,
   // 4 entries each)
   auto gather = [&](int64_t j0, const float (&xs)[4], float th) {
+    bool take[4];
     int before = 0, mine = 0, all = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const uint64_t b = __ballot(xs[e] >= th);
+      take[e] = j0 + e < n && xs[e] >= th;  // th is -inf while fewer than kc seen
+      const uint64_t b = __ballot(take[e]);
       before += __popcll(b & below);
       all += __popcll(b);
     }
     if (all) {
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        if (xs[e] >= th) {
+        if (take[e]) {
           const int pos = total + before + mine++;
           if (pos < kSelCap) keys[pos] = sel_key(xs[e], j0 + e);
         }
