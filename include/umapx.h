@@ -187,6 +187,22 @@ int32_t umapx_knn(const float* x, int64_t n, int64_t d, int32_t k, int32_t metri
 int32_t umapx_knn_to_reference(const float* q, int64_t m, const float* r, int64_t n,
                                int64_t d, int32_t k, int32_t metric, int32_t precision,
                                int32_t rerank_factor, int64_t* idx, float* dist);
+/* No counterpart in the reference.  umapx_knn_to_reference (and, with
+ * d_q == d_r and m == n, umapx_knn) for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Inputs d_q [m x d] / d_r [n x d] and outputs d_idx /
+ * d_dist [m x k] are caller-owned device buffers; every temporary comes from
+ * one static device array of UMAPX_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends.  Euclidean or cosine, 1 to 256 features; a shape whose
+ * temporaries exceed the array is rejected with UMAPX_ERR_INVALID_PARAMETER.
+ * Work is queued on `stream` (a hipStream_t; NULL is the null stream) and
+ * not waited for.  All calls on one GPU share the array: calls that overlap
+ * on one GPU race. */
+#define UMAPX_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+int32_t umapx_knn_to_reference_device_noalloc(const float* d_q, int64_t m, const float* d_r,
+                                              int64_t n, int64_t d, int32_t k, int32_t metric,
+                                              int32_t precision, int32_t rerank_factor,
+                                              int64_t* d_idx, float* d_dist, void* stream);
 /* approximate_nearest_neighbors (lib.rs:2932-3168, 3213-3281): the
  * reference's neighbours-of-neighbours rounds from random pools under any
  * UMAPX_METRIC_* code, on the reference's RNG stream (seed = random_seed ^

@@ -167,6 +167,9 @@ SIGNATURES = {
     "umapx_knn_to_reference": (ctypes.c_int32, [
         _f32p, ctypes.c_int64, _f32p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i64p, _f32p]),
+    "umapx_knn_to_reference_device_noalloc": (ctypes.c_int32, [
+        _vp, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]),
     "umapx_knn_rows_include_self": (ctypes.c_int32, [
         _i64p, _f32p, ctypes.c_int64, ctypes.c_int32, _i32p]),
     "umapx_smooth_knn_dist": (ctypes.c_int32, [

@@ -48,6 +48,12 @@ void retain_pool_memory() {
   HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
 }
 
+namespace {
+thread_local std::string g_last_error;
+}  // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
 void maybe_inject_fault(const char* stage) {
   static const char* env = std::getenv("UMAPX_FAULT_INJECT");
   if (env && std::strcmp(env, stage) == 0)
@@ -55,8 +61,6 @@ void maybe_inject_fault(const char* stage) {
 }
 
 namespace {
-
-thread_local std::string g_last_error;
 
 double now_sec() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();

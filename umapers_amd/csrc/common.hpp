@@ -36,6 +36,9 @@ enum : int {
   throw Error(kInvalidParameter, "invalid parameter: " + msg);
 }
 
+// Sets the calling thread's umapx_last_error() message.
+void set_last_error(const std::string& msg);
+
 void check_hip(hipError_t e, const char* what, const char* file, int line);
 #define HIP_CHECK(expr) ::umapx::check_hip((expr), #expr, __FILE__, __LINE__)
 #define KERNEL_CHECK() ::umapx::check_hip(hipGetLastError(), "kernel launch", __FILE__, __LINE__)
@@ -69,20 +72,24 @@ struct AllocStreamScope {
 void retain_pool_memory();
 
 // RAII device buffer.  A buffer allocated under an AllocStreamScope is freed
-// on the same stream, which must outlive it.
+// on the same stream, which must outlive it.  A borrowed buffer (borrow())
+// is a fixed-capacity view of memory owned elsewhere: it is never freed, and
+// growing it past its capacity is an error instead of an allocation.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
   hipStream_t s = nullptr;  // allocation stream (stream-ordered), or nullptr
+  bool borrowed = false;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s), borrowed(o.borrowed) {
     o.p = nullptr;
     o.n = 0;
     o.s = nullptr;
+    o.borrowed = false;
   }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
@@ -90,14 +97,29 @@ struct DevBuf {
       p = o.p;
       n = o.n;
       s = o.s;
+      borrowed = o.borrowed;
       o.p = nullptr;
       o.n = 0;
       o.s = nullptr;
+      o.borrowed = false;
     }
     return *this;
   }
   ~DevBuf() { release(); }
+  void borrow(T* ptr, size_t capacity) {
+    release();
+    p = ptr;
+    n = capacity;
+    borrowed = true;
+  }
   void alloc(size_t count) {
+    if (borrowed) {
+      if (count > n)
+        throw Error(kInvalidParameter,
+                    "invalid parameter: borrowed device buffer of " + std::to_string(n) +
+                        " elements cannot hold " + std::to_string(count));
+      return;
+    }
     release();
     if (count == 0) return;
     maybe_inject_fault("alloc");
@@ -113,6 +135,12 @@ struct DevBuf {
     if (count > n) alloc(count);
   }
   void release() {
+    if (borrowed) {
+      p = nullptr;
+      n = 0;
+      borrowed = false;
+      return;
+    }
     if (p) {
       if (s)
         (void)hipFreeAsync(p, s);

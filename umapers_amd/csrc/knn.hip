@@ -674,6 +674,29 @@ int device_cus() {
   return cus > 0 ? cus : 1;
 }
 
+// How a launch of query groups of `queries` rows covers m queries: `full`
+// groups scan every reference row; the rest are split `tail_split` ways over
+// slices of `tail_len` reference rows, leaving `tail_rows` query rows whose
+// split lists (2 * H pairs per row and split) are merged afterwards.
+struct TailPlan {
+  int64_t groups = 0, full = 0, tail_len = 0, tail_rows = 0;
+  int tail_split = 1;
+};
+
+TailPlan plan_tail(int64_t m, int64_t n, int queries, int cus) {
+  TailPlan t;
+  t.groups = (m + queries - 1) / queries;
+  t.full = t.groups / cus * cus;
+  int tail_split = choose_tail_split(t.groups - t.full, n, cus);
+  auto split_rows = [&](int sp) { return ((n + sp - 1) / sp + kTile - 1) / kTile * kTile; };
+  while (tail_split > 1 && (tail_split - 1) * split_rows(tail_split) >= n) --tail_split;  // no empty split
+  if (tail_split == 1) t.full = t.groups;
+  t.tail_split = tail_split;
+  t.tail_len = split_rows(tail_split);
+  t.tail_rows = m - t.full * queries;
+  return t;
+}
+
 template <typename T, int DPAD, int H>
 void launch_candidates(const T* xq, int64_t m, const T* xr, const float* nr_half, int64_t n,
                        int32_t* cand, DevBuf<uint2>& pairs, hipStream_t s) {
@@ -681,15 +704,10 @@ void launch_candidates(const T* xq, int64_t m, const T* xr, const float* nr_half
   auto kern = knn_candidates_kernel<T, DPAD, H>;
   HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, C::kBytes));
-  const int64_t groups = (m + C::kQueries - 1) / C::kQueries;
-  const int cus = device_cus();
-  int64_t full = groups / cus * cus;
-  int tail_split = choose_tail_split(groups - full, n, cus);
-  auto split_rows = [&](int sp) { return ((n + sp - 1) / sp + kTile - 1) / kTile * kTile; };
-  while (tail_split > 1 && (tail_split - 1) * split_rows(tail_split) >= n) --tail_split;  // no empty split
-  if (tail_split == 1) full = groups;
-  const int64_t tail_len = split_rows(tail_split);
-  const int64_t tail_rows = m - full * C::kQueries;
+  const TailPlan tp = plan_tail(m, n, C::kQueries, device_cus());
+  const int64_t groups = tp.groups, full = tp.full, tail_len = tp.tail_len,
+                tail_rows = tp.tail_rows;
+  const int tail_split = tp.tail_split;
   uint2* pp = nullptr;
   if (tail_rows > 0) {
     pairs.ensure(static_cast<size_t>(tail_rows) * tail_split * 2 * H);
@@ -705,6 +723,24 @@ void launch_candidates(const T* xq, int64_t m, const T* xr, const float* nr_half
                        pp, full * C::kQueries, m, tail_split, cand);
     KERNEL_CHECK();
   }
+}
+
+// Query rows per block of the candidate kernel for one shape.
+template <typename T, int H>
+int queries_per_block(int dpad) {
+  switch (dpad) {
+    case 32: return CandShape<T, 32, H>::kQueries;
+    case 64: return CandShape<T, 64, H>::kQueries;
+    case 128: return CandShape<T, 128, H>::kQueries;
+    case 192: return CandShape<T, 192, H>::kQueries;
+    case 256: return CandShape<T, 256, H>::kQueries;
+    default: throw_invalid("unsupported padded feature count");
+  }
+}
+
+__global__ void zero_f64_kernel(double* __restrict__ p, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
 }
 
 template <typename T, int H>
@@ -737,6 +773,31 @@ int knn_candidate_count(int k, int rerank_factor) {
   return -1;
 }
 
+KnnScratchNeed knn_scratch_need(int64_t m, int64_t n, int64_t d, int k, int metric,
+                                int precision, int rerank_factor, bool self) {
+  KnnScratchNeed need;
+  const int dpad = knn_padded_dim(d);
+  const int kc = knn_candidate_count(k, rerank_factor);
+  if (metric > 1 || dpad < 0 || kc < 0 || m == 0) return need;
+  const size_t esz = precision == 1 ? 2 : 4;
+  const int h = kc / 2;
+  const int queries = precision == 1
+                          ? (h == 16 ? queries_per_block<uint16_t, 16>(dpad)
+                                     : queries_per_block<uint16_t, 32>(dpad))
+                          : (h == 16 ? queries_per_block<float, 16>(dpad)
+                                     : queries_per_block<float, 32>(dpad));
+  const TailPlan tp = plan_tail(m, n, queries, device_cus());
+  need.colsum = static_cast<size_t>(d);
+  need.rprep = static_cast<size_t>(n) * dpad * esz;
+  need.rnorm2 = static_cast<size_t>(n);
+  need.qprep = self ? 0 : static_cast<size_t>(m) * dpad * esz;
+  need.cand = static_cast<size_t>(m) * kc;
+  need.pairs = tp.tail_rows > 0 ? static_cast<size_t>(tp.tail_rows) * tp.tail_split * 2 * h : 0;
+  need.rnorm = metric == 1 ? static_cast<size_t>(n) : 0;
+  need.qnorm = metric == 1 && !self ? static_cast<size_t>(m) : 0;
+  return need;
+}
+
 void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int k,
                 int metric, int precision, int rerank_factor, int32_t* out_idx,
                 float* out_dist, KnnWorkspace& ws, hipStream_t s, KnnStats* stats,
@@ -763,7 +824,9 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
 
   // column means of the (normalised) reference rows for centring
   ws.colsum.ensure(static_cast<size_t>(d));
-  HIP_CHECK(hipMemsetAsync(ws.colsum.get(), 0, sizeof(double) * d, s));
+  // a fill kernel rather than a memset node, so the call can be captured
+  hipLaunchKernelGGL(zero_f64_kernel, dim3(blocks_for(static_cast<size_t>(d), 256)), dim3(256), 0,
+                     s, ws.colsum.get(), d);
   const unsigned cs_blocks = static_cast<unsigned>(std::min<int64_t>(n, 1024));
   if (metric == 1) {
     HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(column_sums_normalized_kernel),

@@ -26,6 +26,15 @@ struct KnnStats {
 
 int knn_padded_dim(int64_t d);
 
+// Elements of each KnnWorkspace buffer that knn_device fills for euclidean /
+// cosine rows of at most 256 features (all zero for other shapes); self: q
+// and r are the same rows.
+struct KnnScratchNeed {
+  size_t colsum = 0, rprep = 0, rnorm2 = 0, qprep = 0, cand = 0, pairs = 0, rnorm = 0, qnorm = 0;
+};
+KnnScratchNeed knn_scratch_need(int64_t m, int64_t n, int64_t d, int k, int metric,
+                                int precision, int rerank_factor, bool self);
+
 // Candidates for rows wider than 256 features: score blocks Q R^T by GEMM,
 // then the kc largest q.r + nr_half[r] per query (kc 32 or 64).  qp / rp are
 // the prepared [m x dpad] / [n x dpad] rows (BF16 when bf16, else FP32); r
