@@ -90,6 +90,7 @@ def lib() -> ctypes.CDLL:
         L.orc_smooth_knn_dist.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int32, ctypes.c_float,
                                           ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                                           _f32p, _f32p]
+        L.orc_smooth_knn_dist_fp64.argtypes = L.orc_smooth_knn_dist.argtypes
         L.orc_fuzzy_graph.argtypes = [_i64p, _f32p, _f32p, _f32p, ctypes.c_int64,
                                       ctypes.c_int32, ctypes.c_float, ctypes.c_int32,
                                       ctypes.c_int64, _i64p, _i64p, _f32p]
@@ -234,13 +235,15 @@ def rows_include_self(idx, dist) -> bool:
 
 
 def smooth_knn_dist(dist, k_value: float, local_connectivity: float = 1.0,
-                    include_self: bool = True, bandwidth: float = 1.0):
+                    include_self: bool = True, bandwidth: float = 1.0, fp64_sums: bool = False):
+    """sigma / rho of every row; fp64_sums: the bisection's psum in f64."""
     dist = _f32(dist)
     n, k = dist.shape
     sig = np.empty(n, np.float32)
     rho = np.empty(n, np.float32)
-    lib().orc_smooth_knn_dist(_p(dist, _f32p), n, k, k_value, local_connectivity, bandwidth,
-                              int(include_self), _p(sig, _f32p), _p(rho, _f32p))
+    fn = lib().orc_smooth_knn_dist_fp64 if fp64_sums else lib().orc_smooth_knn_dist
+    fn(_p(dist, _f32p), n, k, k_value, local_connectivity, bandwidth, int(include_self),
+       _p(sig, _f32p), _p(rho, _f32p))
     return sig, rho
 
 

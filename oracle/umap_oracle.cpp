@@ -26,6 +26,7 @@
  *   find_ab_params      bit-exact KAT (lib.rs:5781-5795)          pinned
  *   distances / top-k   FP64 brute force + tie-break KAT           pinned
  *   smooth_knn_dist     psum invariant, self/no-self KAT           pinned
+ *   (f64 psum variant)  psum invariant in f64, closed-form rows    pinned
  *   membership / union  closed forms, symmetry, Laplacian KAT      pinned
  *   schedule counts     closed-form per-edge counts                pinned
  *   spectral init       eigh of I + D^-1/2 W D^-1/2 (subspace)     pinned up to sign/rotation
@@ -416,10 +417,13 @@ bool rows_include_self(const int64_t* idx, const float* dist, int64_t n, int32_t
   return true;
 }
 
-/* lib.rs:3424-3523 smooth_knn_dist */
+/* lib.rs:3424-3523 smooth_knn_dist.  fp64_sums: the same 64-step bisection,
+ * stop rule and f32 ratio (d - rho) / mid, with psum accumulated in f64 from
+ * f64 exponentials and compared with the f64 target (the GPU path's default
+ * form; the reference itself sums in f32). */
 void smooth_knn(const float* distances, int64_t n, int32_t kc, float k,
                 float local_connectivity, float bandwidth, bool include_self,
-                float* sigmas, float* rhos) {
+                float* sigmas, float* rhos, bool fp64_sums = false) {
   const float target = std::log2(k) * bandwidth;
   float mean_distances;
   {
@@ -468,16 +472,29 @@ void smooth_knn(const float* distances, int64_t n, int32_t kc, float k,
     rhos[i] = rho;
     float lo = 0.0f, hi = std::numeric_limits<float>::infinity(), mid = 1.0f;
     for (int it = 0; it < 64; ++it) {
-      float psum = 0.0f;
-      for (int c = 0; c < na; ++c) {
-        const float dd = act[c] - rho;
-        if (dd > 0.0f)
-          psum += std::exp(-(dd / mid));
-        else
-          psum += 1.0f;
+      bool stop, above;
+      if (fp64_sums) {
+        double psum = 0.0;
+        for (int c = 0; c < na; ++c) {
+          const float dd = act[c] - rho;
+          psum += dd > 0.0f ? std::exp(-static_cast<double>(dd / mid)) : 1.0;
+        }
+        stop = std::fabs(psum - static_cast<double>(target)) < 1e-5;
+        above = psum > static_cast<double>(target);
+      } else {
+        float psum = 0.0f;
+        for (int c = 0; c < na; ++c) {
+          const float dd = act[c] - rho;
+          if (dd > 0.0f)
+            psum += std::exp(-(dd / mid));
+          else
+            psum += 1.0f;
+        }
+        stop = std::fabs(psum - target) < SMOOTH_K_TOLERANCE;
+        above = psum > target;
       }
-      if (std::fabs(psum - target) < SMOOTH_K_TOLERANCE) break;
-      if (psum > target) {
+      if (stop) break;
+      if (above) {
         hi = mid;
         mid = (lo + hi) / 2.0f;
       } else {
@@ -1769,6 +1786,13 @@ void orc_smooth_knn_dist(const float* dist, int64_t n, int32_t k, float k_value,
                          float local_connectivity, float bandwidth, int32_t include_self,
                          float* sigmas, float* rhos) {
   smooth_knn(dist, n, k, k_value, local_connectivity, bandwidth, include_self != 0, sigmas, rhos);
+}
+
+void orc_smooth_knn_dist_fp64(const float* dist, int64_t n, int32_t k, float k_value,
+                              float local_connectivity, float bandwidth, int32_t include_self,
+                              float* sigmas, float* rhos) {
+  smooth_knn(dist, n, k, k_value, local_connectivity, bandwidth, include_self != 0, sigmas, rhos,
+             true);
 }
 
 int64_t orc_fuzzy_graph(const int64_t* idx, const float* dist, const float* sigmas,

@@ -78,6 +78,10 @@ int32_t orc_knn_rows_include_self(const int64_t* idx, const float* dist,
 void orc_smooth_knn_dist(const float* dist, int64_t n, int32_t k, float k_value,
                          float local_connectivity, float bandwidth,
                          int32_t include_self, float* sigmas, float* rhos);
+/* The same with the bisection's psum in f64 (the GPU path's default). */
+void orc_smooth_knn_dist_fp64(const float* dist, int64_t n, int32_t k, float k_value,
+                              float local_connectivity, float bandwidth,
+                              int32_t include_self, float* sigmas, float* rhos);
 /* Membership + fuzzy union (+ prune when n_epochs > 0) into head-sorted
  * edges.  Returns the edge count, or -1 when cap is too small. */
 int64_t orc_fuzzy_graph(const int64_t* idx, const float* dist,

@@ -11,6 +11,11 @@ Fixtures (all small):
                  positive / negative sample counts of 200 epochs
   inputs.json    SHA-256 of the synthetic inputs of configs 1-3 and of the
                  100,000-row fit set of config 5
+  reference_smoke.json
+                 the oracle under the reference's CI smoke protocol (digits and
+                 breast_cancer, k = 15, 200 epochs, random init; both gates)
+                 over ten seeds, beside the reference's published seed-42
+                 figures for its Rust CPU path
   quality.json   the oracle's trustworthiness@15 and kNN preservation at
                  config 1, config 2 (densMAP, 500 epochs) and a 50,000-row
                  draw of config 4's mixture (datasets.c4(n=50000), k = 30)
@@ -73,12 +78,64 @@ def _fit_quality(x, seeds, **kw) -> dict:
     return {"seeds": list(seeds), "trust": t, "knn_preservation": k}
 
 
+def _fit_quality_shared_knn(x, seeds, metric: str, **kw) -> dict:
+    """As _fit_quality, with the exact kNN computed once and handed to every
+    seed's fit (the kNN does not depend on the seed; fit_transform_with_knn
+    runs the same stages after it)."""
+    idx, dist = O.knn(x, kw.get("n_neighbors", 15), metric)
+    t, k = [], []
+    for s in seeds:
+        y = O.fit(x, O.OracleParams(random_seed=s, metric=metric, **kw), knn_idx=idx,
+                  knn_dist=dist).embedding
+        t.append(trustworthiness(x, y))
+        k.append(knn_preservation(x, y))
+        print(metric, kw, "seed", s, t[-1], k[-1], flush=True)
+    return {"seeds": list(seeds), "trust": t, "knn_preservation": k}
+
+
+C3_SEEDS = [42, 7, 11]
+
 QUALITY = {
     "c1_plain_200": lambda: _fit_quality(datasets.c1(), C1_SEEDS, n_epochs=200),
+    "c3_euclidean_200": lambda: _fit_quality_shared_knn(datasets.c3(), C3_SEEDS, "euclidean",
+                                                        n_epochs=200),
+    "c3_cosine_200": lambda: _fit_quality_shared_knn(datasets.c3(cosine=True), C3_SEEDS,
+                                                     "cosine", n_epochs=200),
     "c2_densmap_500": lambda: _fit_quality(datasets.c2(), C2_SEEDS, n_epochs=500, densmap=True),
     "c4_sub50k_k30_200": lambda: _fit_quality(datasets.c4(n=50_000), C4_SEEDS, n_epochs=200,
                                               n_neighbors=30),
 }
+
+
+SMOKE_SEEDS = [42, 0, 1, 2, 3, 4, 5, 6, 7, 8]
+
+
+def reference_smoke() -> None:
+    """The oracle under the reference's own CI smoke protocol
+    (tests/smoke_protocol.py), next to the figures the reference published
+    for its Rust CPU path: reference_smoke.json."""
+    from tests import smoke_protocol as P
+
+    out = {"seeds": SMOKE_SEEDS, "k": P.K, "n_epochs": 200, "init": "random", "gates": {}}
+    for gate, (cap, pre) in P.GATES.items():
+        out["gates"][gate] = {"sample_cap": cap, "precomputed_knn": pre, "datasets": {}}
+        for name in ("digits", "breast_cancer"):
+            x = P.load_dataset(name)
+            idx, dist = P.shared_exact_knn(x)
+            ts, rs = [], []
+            for s in SMOKE_SEEDS:
+                prm = O.OracleParams(random_seed=s, n_epochs=200, init="random")
+                y = (O.fit(x, prm, knn_idx=idx, knn_dist=dist) if pre else O.fit(x, prm)).embedding
+                t, r = P.consistency(x, y, idx, cap)
+                ts.append(t)
+                rs.append(r)
+            out["gates"][gate]["datasets"][name] = {
+                "reference": P.REFERENCE[gate][name],
+                "oracle_trust": ts, "oracle_recall": rs,
+                "oracle_trust_mean": float(np.mean(ts)), "oracle_recall_mean": float(np.mean(rs))}
+            print(gate, name, np.mean(ts), np.mean(rs), P.REFERENCE[gate][name], flush=True)
+    with open(os.path.join(HERE, "reference_smoke.json"), "w") as f:
+        json.dump(out, f, indent=1)
 
 
 def quality(only=None) -> None:
@@ -95,9 +152,12 @@ def quality(only=None) -> None:
 
 
 if __name__ == "__main__":
-    if len(sys.argv) > 1:  # named quality sections only, e.g. c4_sub50k_k30_200
+    if sys.argv[1:] == ["reference_smoke"]:
+        reference_smoke()
+    elif len(sys.argv) > 1:  # named quality sections only, e.g. c4_sub50k_k30_200
         quality(set(sys.argv[1:]))
     else:
         stages()
         inputs()
+        reference_smoke()
         quality()

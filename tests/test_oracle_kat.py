@@ -60,6 +60,31 @@ def test_smooth_knn_psum_hits_log2k(oracle, rng):
     assert np.array_equal(rho, d[:, 1])
 
 
+def test_smooth_knn_fp64_sums_variant(oracle, rng):
+    """The f64-psum form of the bisection (the GPU path's default): the same
+    invariant with the sums in f64, the self / no-self equivalence, a
+    closed-form row, and agreement with the f32 form (both solve the same
+    equation, so they differ by the stop tolerance only)."""
+    d = np.sort(rng.uniform(0.5, 3.0, size=(300, 15)).astype(np.float32), axis=1)
+    d[:, 0] = 0.0
+    sig, rho = oracle.smooth_knn_dist(d, 15.0, 1.0, include_self=True, fp64_sums=True)
+    act = d[:, 1:].astype(np.float64)
+    psum = np.where(act - rho[:, None] > 0, np.exp(-(act - rho[:, None]) / sig[:, None]), 1.0).sum(1)
+    assert np.all(np.abs(psum - np.log2(15.0)) < 1.2e-5)
+    s32, r32 = oracle.smooth_knn_dist(d, 15.0, 1.0, include_self=True)
+    assert np.array_equal(rho, r32) and np.max(np.abs(sig - s32) / s32) < 2e-4
+    with_self = np.array([[0.0, 0.2, 0.4, 0.8], [0.0, 0.1, 0.5, 0.9]], np.float32)
+    s1, r1 = oracle.smooth_knn_dist(with_self, 3.0, 1.0, include_self=True, fp64_sums=True)
+    s2, r2 = oracle.smooth_knn_dist(with_self[:, 1:], 3.0, 1.0, include_self=False,
+                                    fp64_sums=True)
+    assert np.array_equal(s1, s2) and np.array_equal(r1, r2)
+    # two neighbours: psum = 1 + exp(-(d2 - d1) / sigma) = log2(3) in closed form
+    row = np.array([[0.0, 0.3, 1.1]], np.float32)
+    s, r = oracle.smooth_knn_dist(row, 3.0, 1.0, include_self=True, fp64_sums=True)
+    closed = (1.1 - 0.3) / -np.log(np.log2(3.0) - 1.0)
+    assert r[0] == np.float32(0.3) and abs(s[0] - closed) / closed < 1e-4
+
+
 def test_distance_closed_forms(oracle):
     x = np.array([1.0, -2.0, 0.0, 4.0], np.float32)
     y = np.array([-1.0, 2.0, 0.0, 1.0], np.float32)
