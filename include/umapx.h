@@ -66,8 +66,9 @@ typedef struct umapx_params {
   int32_t negative_sample_rate; /* 5 */
   int32_t init;                 /* UMAPX_INIT_SPECTRAL */
   uint64_t random_seed;         /* 42 */
-  int32_t use_approximate_knn;  /* with approx_knn_threshold 0 ("approximate" mode): the
-                                   reference's approximate rounds; otherwise the exact kNN */
+  int32_t use_approximate_knn;  /* 1 (default): the approximate rounds past 4 x
+                                   approx_knn_threshold rows, as the reference's auto mode
+                                   (threshold 0: always); 0: the exact kNN */
   int32_t approx_knn_candidates;
   int32_t approx_knn_iters;
   int32_t approx_knn_threshold;
@@ -115,6 +116,18 @@ typedef struct umapx_handle umapx_handle;
 
 void umapx_default_params(umapx_params* out);
 const char* umapx_last_error(void);
+/* The structured fields of the calling thread's last failure, so that
+ * UmapError (rust_umap/src/lib.rs:198-216) can be rebuilt exactly: row /
+ * expected / got of InconsistentDimensions, expected / got of
+ * FeatureMismatch and EmbeddingDimensionMismatch, -1 where the variant has
+ * no such field.  Returns that failure's code (UMAPX_OK before any). */
+int32_t umapx_last_error_detail(int64_t* row, int64_t* expected, int64_t* got);
+/* should_use_approximate_knn (rust_umap/src/lib.rs:2285-2311): 1 when a fit
+ * of n_samples x n_features rows under these parameters builds its kNN with
+ * the approximate rounds (auto mode past 4 x approx_knn_threshold rows;
+ * threshold 0 forces them), 0 when it runs the exact kNN. */
+int32_t umapx_uses_approximate_knn(const umapx_params* params, int64_t n_samples,
+                                   int64_t n_features);
 const char* umapx_version(void);
 /* Number of visible GPUs (0 when none); never fails. */
 int32_t umapx_device_count(void);

@@ -91,6 +91,10 @@ def lib() -> ctypes.CDLL:
                                           ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                                           _f32p, _f32p]
         L.orc_smooth_knn_dist_fp64.argtypes = L.orc_smooth_knn_dist.argtypes
+        L.orc_normalized_laplacian.argtypes = [ctypes.c_int64, _i64p, _i64p, _f32p,
+                                               ctypes.c_int64, ctypes.c_int32,
+                                               ctypes.POINTER(ctypes.c_double)]
+        L.orc_normalized_laplacian.restype = ctypes.c_int32
         L.orc_fuzzy_graph.argtypes = [_i64p, _f32p, _f32p, _f32p, ctypes.c_int64,
                                       ctypes.c_int32, ctypes.c_float, ctypes.c_int32,
                                       ctypes.c_int64, _i64p, _i64p, _f32p]
@@ -263,6 +267,17 @@ def fuzzy_graph(idx, dist, sig, rho, mix: float = 1.0, n_epochs: int = 0):
                                _p(t, _i64p), _p(w, _f32p))
     assert ne >= 0
     return h[:ne].copy(), t[:ne].copy(), w[:ne].copy()
+
+
+def normalized_laplacian(n: int, heads, tails, w, from_affinity: bool = False):
+    """I - D^-1/2 W D^-1/2 of the max-symmetrised graph, from the edge list
+    or from the dense affinity matrix; None when no weight is positive."""
+    heads, tails, w = _i64(heads), _i64(tails), _f32(w)
+    out = np.zeros((n, n), np.float64)
+    ok = lib().orc_normalized_laplacian(n, _p(heads, _i64p), _p(tails, _i64p), _p(w, _f32p),
+                                        heads.shape[0], int(from_affinity),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out if ok else None
 
 
 def connected_components(n: int, heads, tails):

@@ -1879,6 +1879,36 @@ void orc_optimize_layout(float* y, int64_t n, int32_t dim, const int64_t* heads,
                   radii, dens_start, dens_lambda, var_shift);
 }
 
+int32_t orc_normalized_laplacian(int64_t n, const int64_t* heads, const int64_t* tails,
+                                 const float* w, int64_t ne, int32_t from_affinity,
+                                 double* out) {
+  vector<double> lap;
+  bool ok;
+  if (from_affinity) {
+    /* the dense affinity path: max-combined, symmetrised (lib.rs:7492-7519) */
+    vector<double> aff(static_cast<size_t>(n * n), 0.0);
+    for (int64_t e = 0; e < ne; ++e) {
+      if (heads[e] == tails[e]) continue;
+      const double v = static_cast<double>(std::max(w[e], 0.0f));
+      double& a = aff[heads[e] * n + tails[e]];
+      if (v > a) a = v;
+    }
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t j = i + 1; j < n; ++j) {
+        const double sym = std::max(aff[i * n + j], aff[j * n + i]);
+        aff[i * n + j] = aff[j * n + i] = sym;
+      }
+    ok = laplacian_from_affinity(aff, static_cast<size_t>(n), lap);
+  } else {
+    vector<Edge> edges(ne);
+    for (int64_t e = 0; e < ne; ++e) edges[e] = {heads[e], tails[e], w[e]};
+    ok = laplacian_from_edges(static_cast<size_t>(n), edges, lap);
+  }
+  if (!ok) return 0;
+  std::copy(lap.begin(), lap.end(), out);
+  return 1;
+}
+
 int32_t orc_density_start_epoch(int32_t n_epochs, float dens_frac) {
   return density_start(n_epochs, dens_frac);
 }

@@ -78,6 +78,12 @@ int32_t orc_knn_rows_include_self(const int64_t* idx, const float* dist,
 void orc_smooth_knn_dist(const float* dist, int64_t n, int32_t k, float k_value,
                          float local_connectivity, float bandwidth,
                          int32_t include_self, float* sigmas, float* rhos);
+/* normalized_laplacian_from_edges (lib.rs:3966-4023), or with from_affinity
+ * the dense max-symmetrised affinity route (lib.rs:3928-3964): n x n, row-major.
+ * Returns 0 when the graph has no positive weight. */
+int32_t orc_normalized_laplacian(int64_t n, const int64_t* heads, const int64_t* tails,
+                                 const float* w, int64_t ne, int32_t from_affinity,
+                                 double* out);
 /* The same with the bisection's psum in f64 (the GPU path's default). */
 void orc_smooth_knn_dist_fp64(const float* dist, int64_t n, int32_t k, float k_value,
                               float local_connectivity, float bandwidth,

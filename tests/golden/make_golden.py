@@ -18,8 +18,9 @@ Fixtures (all small):
                  figures for its Rust CPU path
   quality.json   the oracle's trustworthiness@15 and kNN preservation at
                  config 1, config 2 (densMAP, 500 epochs) and a 50,000-row
-                 draw of config 4's mixture (datasets.c4(n=50000), k = 30)
-                 over fixed seeds (the statistical
+                 draw of config 4's mixture (datasets.c4(n=50000), k = 30),
+                 configs 3 (euclidean and cosine) and 5 (with the embedding's
+                 extent) over fixed seeds (the statistical
                  parity targets of the GPU fit tests)
 
 The reference (Rust) is not run: it holds no array fixtures of its own for
@@ -43,8 +44,9 @@ from oracle import oracle as O  # noqa: E402
 from umapers_amd import datasets  # noqa: E402
 from umapers_amd.diagnostics import knn_preservation, trustworthiness  # noqa: E402
 
-C1_SEEDS = list(range(10))
-C2_SEEDS = [42, 7, 11, 23, 5, 9, 13, 17, 19, 29, 31, 37, 41, 43, 47, 53]
+C1_SEEDS = list(range(40))
+C2_SEEDS = [42, 7, 11, 23, 5, 9, 13, 17, 19, 29, 31, 37, 41, 43, 47, 53,
+            59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127, 131]
 C4_SEEDS = [42, 7, 11, 23]
 
 
@@ -94,6 +96,36 @@ def _fit_quality_shared_knn(x, seeds, metric: str, **kw) -> dict:
 
 
 C3_SEEDS = [42, 7, 11]
+C5_SEEDS = [42, 7, 11]
+
+
+def _radius_max(y) -> float:
+    """Largest distance of an embedded point from the embedding's median."""
+    return float(np.linalg.norm(y - np.median(y, 0), axis=1).max())
+
+
+def _radius_q(y, q) -> float:
+    """The q-th percentile of the embedded points' distances from the median."""
+    return float(np.percentile(np.linalg.norm(y - np.median(y, 0), axis=1), q))
+
+
+def _fit_quality_c5(seeds) -> dict:
+    """Config 5's 100,000 fit rows: quality and the embedding's extent (its
+    high-dimensional kNN graph has hub rows of about a thousand entries)."""
+    x = datasets.c5(n_fit=100_000, n_query=1)[0]
+    idx, dist = O.knn(x, 15)
+    t, k, r, r50, r99 = [], [], [], [], []
+    for s in seeds:
+        y = O.fit(x, O.OracleParams(random_seed=s, n_epochs=200), knn_idx=idx,
+                  knn_dist=dist).embedding
+        t.append(trustworthiness(x, y))
+        k.append(knn_preservation(x, y))
+        r.append(_radius_max(y))
+        r50.append(_radius_q(y, 50))
+        r99.append(_radius_q(y, 99))
+        print("c5 seed", s, t[-1], k[-1], r[-1], r50[-1], r99[-1], flush=True)
+    return {"seeds": list(seeds), "trust": t, "knn_preservation": k, "radius_max": r,
+            "radius_p50": r50, "radius_p99": r99}
 
 QUALITY = {
     "c1_plain_200": lambda: _fit_quality(datasets.c1(), C1_SEEDS, n_epochs=200),
@@ -101,6 +133,7 @@ QUALITY = {
                                                         n_epochs=200),
     "c3_cosine_200": lambda: _fit_quality_shared_knn(datasets.c3(cosine=True), C3_SEEDS,
                                                      "cosine", n_epochs=200),
+    "c5_fit_200": lambda: _fit_quality_c5(C5_SEEDS),
     "c2_densmap_500": lambda: _fit_quality(datasets.c2(), C2_SEEDS, n_epochs=500, densmap=True),
     "c4_sub50k_k30_200": lambda: _fit_quality(datasets.c4(n=50_000), C4_SEEDS, n_epochs=200,
                                               n_neighbors=30),
@@ -147,8 +180,8 @@ def quality(only=None) -> None:
     for key, make in QUALITY.items():
         if only is None or key in only:
             res[key] = make()
-    with open(path, "w") as f:
-        json.dump(res, f, indent=1)
+            with open(path, "w") as f:  # after each section, so a long run keeps its progress
+                json.dump(res, f, indent=1)
 
 
 if __name__ == "__main__":

@@ -73,6 +73,9 @@ def _fit_error(umapx, x, **overrides):
     (np.zeros((20, 3)), {"n_neighbors": 5, "negative_sample_rate": 0}, 7, "negative_sample_rate must be >= 1"),
     (np.zeros((20, 3)), {"n_neighbors": 5, "dens_frac": 2.0}, 7, "dens_frac must be finite and in [0, 1]"),
     (np.zeros((20, 3)), {"n_neighbors": 5, "dens_var_shift": 0.0}, 7, "dens_var_shift must be finite and > 0"),
+    # the GPU path's own bound, checked before any device work
+    (np.zeros((20, 3)), {"n_neighbors": 5, "n_components": 9}, 7, "n_components must be <= 8"),
+    (np.zeros((20, 3)), {"n_neighbors": 5, "approx_knn_threshold": -1}, 7, "approx_knn_threshold must be >= 0"),
 ])
 def test_validation_messages(umapx, x, overrides, code, msg):
     rc, err = _fit_error(umapx, x, **overrides)
@@ -122,3 +125,36 @@ def test_inverse_transform_requires_fit(umapx):
     rc = umapx.lib().umapx_inverse_transform(h, umapx.ptr(np.zeros((1, 2), np.float32)), 1, 2,
                                              umapx.ptr(np.zeros((1, 2), np.float32)))
     assert rc == umapx.ERR_NOT_FITTED
+
+
+def test_auto_ann_dispatch_matches_reference_predicate(umapx):
+    """should_use_approximate_knn (rust_umap/src/lib.rs:2285-2311) on the
+    cases of the reference's own test (lib.rs:5816-5840)."""
+    L = umapx.lib()
+    p = umapx.default_params()
+    p.use_approximate_knn = 1
+    p.approx_knn_threshold = 4096
+    f = lambda n, d: L.umapx_uses_approximate_knn(ctypes.byref(p), n, d)  # noqa: E731
+    assert not f(4200, 96)
+    assert not f(16_384, 160)
+    assert f(16_385, 160)
+    assert not f(16_384, 16)
+    assert f(16_385, 16)
+    p.approx_knn_threshold = 0  # ann_mode="approximate"
+    assert f(4200, 96)
+    p.approx_knn_threshold = 4096
+    p.use_approximate_knn = 0  # ann_mode="exact"
+    assert not f(100_000, 160)
+
+
+def test_last_error_detail_fields(umapx):
+    L = umapx.lib()
+    row, exp, got = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    x = np.zeros((4, 3), np.float32)
+    idx = np.zeros((4, 1), np.int64)
+    dist = np.zeros((4, 1), np.float32)
+    rc = L.umapx_knn(umapx.ptr(x), 4, 3, 0, 0, 0, 2, umapx.ptr(idx, umapx._i64p), umapx.ptr(dist))
+    assert rc == umapx.ERR_INVALID_PARAMETER
+    code = L.umapx_last_error_detail(ctypes.byref(row), ctypes.byref(exp), ctypes.byref(got))
+    assert code == rc
+    assert (row.value, exp.value, got.value) == (-1, -1, -1)

@@ -106,8 +106,8 @@ def test_oracle_reproduces_fixtures(oracle, c1):
 def test_quality_targets_recorded():
     with open(os.path.join(HERE, "quality.json")) as f:
         q = json.load(f)
-    assert len(q["c1_plain_200"]["trust"]) == 10
-    assert len(q["c2_densmap_500"]["trust"]) == 16
+    assert len(q["c1_plain_200"]["trust"]) == 40
+    assert len(q["c2_densmap_500"]["trust"]) == 32
     assert 0.6 < np.mean(q["c1_plain_200"]["trust"]) < 0.7
     assert 0.8 < np.mean(q["c2_densmap_500"]["trust"]) < 0.9
     assert len(q["c4_sub50k_k30_200"]["trust"]) == 4

@@ -5,6 +5,8 @@ import pytest
 from umapers_amd import Umap, datasets
 from umapers_amd.diagnostics import knn_preservation, trustworthiness
 
+from . import capi
+
 pytestmark = pytest.mark.gpu
 
 
@@ -23,8 +25,8 @@ def test_c1_fit_parity_with_oracle(gpu, oracle):
         to.append(trustworthiness(x, yo))
         kg.append(knn_preservation(x, y))
         ko.append(knn_preservation(x, yo))
-    assert np.mean(tg) >= np.mean(to) - 0.01, (tg, to)
-    assert np.mean(kg) >= np.mean(ko) - 0.01, (kg, ko)
+    assert abs(np.mean(tg) - np.mean(to)) <= 0.01, (tg, to)
+    assert abs(np.mean(kg) - np.mean(ko)) <= 0.01, (kg, ko)
 
 
 def test_clustered_fit_parity_and_profile(gpu, oracle):
@@ -37,15 +39,16 @@ def test_clustered_fit_parity_and_profile(gpu, oracle):
     assert prof["n_epochs"] == 200 and prof["n_edges"] > 0
     assert prof["used_approximate_knn"] is False
     fo = oracle.fit(x, oracle.OracleParams(n_epochs=200))
-    assert trustworthiness(x, y) >= trustworthiness(x, fo.embedding) - 0.01
-    assert knn_preservation(x, y) >= knn_preservation(x, fo.embedding) - 0.01
+    assert abs(trustworthiness(x, y) - trustworthiness(x, fo.embedding)) <= 0.01
+    assert abs(knn_preservation(x, y) - knn_preservation(x, fo.embedding)) <= 0.01
 
 
 def test_bf16_rerank_fit(gpu, oracle):
     x = datasets.gaussian_mixture(5000, 128, 8, 3.0, 11, (0.5, 1.5))
     y = Umap(n_epochs=200, knn_precision="bf16_rerank").fit_transform(x)
     yo = oracle.fit(x, oracle.OracleParams(n_epochs=200)).embedding
-    assert trustworthiness(x, y) >= trustworthiness(x, yo) - 0.01
+    assert abs(trustworthiness(x, y) - trustworthiness(x, yo)) <= 0.01
+    assert abs(knn_preservation(x, y) - knn_preservation(x, yo)) <= 0.01
 
 
 def test_cosine_fit(gpu, oracle):
@@ -53,7 +56,8 @@ def test_cosine_fit(gpu, oracle):
     y = Umap(n_epochs=200, metric="cosine").fit_transform(x)
     yo = oracle.fit(x, oracle.OracleParams(n_epochs=200, metric="cosine")).embedding
     xn = x / np.linalg.norm(x, axis=1, keepdims=True)
-    assert trustworthiness(xn, y) >= trustworthiness(xn, yo) - 0.01
+    assert abs(trustworthiness(xn, y) - trustworthiness(xn, yo)) <= 0.01
+    assert abs(knn_preservation(xn, y) - knn_preservation(xn, yo)) <= 0.01
 
 
 def test_precomputed_knn_matches_direct(gpu, oracle):
@@ -110,7 +114,11 @@ def test_plain_transform_parity(gpu, oracle):
     both_x = np.concatenate([x, q])
     tg = trustworthiness(both_x, np.concatenate([y, yq]))
     to = trustworthiness(both_x, np.concatenate([y, yo]))
-    assert tg >= to - 0.01
+    qg, qo = trustworthiness(q, yq), trustworthiness(q, yo)
+    print(f"transform: trust (fit + queries) gpu {tg:.4f} oracle {to:.4f}; queries alone gpu "
+          f"{qg:.4f} oracle {qo:.4f}", flush=True)
+    assert abs(tg - to) <= 0.01, (tg, to)
+    assert abs(qg - qo) <= 0.02, (qg, qo)
     # a query lands near the training rows it resembles
     d = np.linalg.norm(yq - yo, axis=1)
     assert np.median(d) < 1.0
@@ -124,6 +132,17 @@ def test_transform_errors(gpu):
     m.fit(x)
     with pytest.raises(ValueError, match="feature mismatch: expected 10, got 9"):
         m.transform(np.zeros((3, 9), np.float32))
+    # UmapError::FeatureMismatch { expected: 10, got: 9 } through the ABI
+    import ctypes
+
+    from umapers_amd import _lib as L
+    row, exp, got = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    code = L.lib().umapx_last_error_detail(ctypes.byref(row), ctypes.byref(exp), ctypes.byref(got))
+    assert (code, row.value, exp.value, got.value) == (L.ERR_FEATURE_MISMATCH, -1, 10, 9)
+    with pytest.raises(ValueError, match="embedding dimension mismatch: expected 2, got 3"):
+        m.inverse_transform(np.zeros((2, 3), np.float32))
+    code = L.lib().umapx_last_error_detail(ctypes.byref(row), ctypes.byref(exp), ctypes.byref(got))
+    assert (code, row.value, exp.value, got.value) == (L.ERR_EMBEDDING_DIMENSION_MISMATCH, -1, 2, 3)
     q = np.zeros((4, 10), np.float32)
     q[2, 3] = np.inf
     with pytest.raises(ValueError, match="non-finite value at row 2"):
@@ -140,14 +159,24 @@ def test_output_dens_radii(gpu, oracle):
 
 
 def test_densmap_fit(gpu, oracle):
+    # three seeds each way: mean trustworthiness within 0.01 both ways and the
+    # original / embedded radius correlation within 0.05 both ways
     x = datasets.gaussian_mixture(3000, 96, 5, 4.0, 16, 1.0)
-    m = Umap(n_epochs=200, densmap=True, dens_lambda=2.0, dens_frac=0.3, output_dens=True)
-    emb, ro, re = m.fit_transform(x)
-    fo = oracle.fit(x, oracle.OracleParams(n_epochs=200, densmap=True, output_dens=True))
-    assert trustworthiness(x, emb) >= trustworthiness(x, fo.embedding) - 0.01
-    cg = np.corrcoef(ro, re)[0, 1]
-    co = np.corrcoef(fo.rad_orig, fo.rad_emb)[0, 1]
-    assert cg >= co - 0.05, (cg, co)
+    tg, to, cg, co = [], [], [], []
+    for s in (42, 7, 11):
+        m = Umap(n_epochs=200, densmap=True, dens_lambda=2.0, dens_frac=0.3, output_dens=True,
+                 random_seed=s)
+        emb, ro, re = m.fit_transform(x)
+        fo = oracle.fit(x, oracle.OracleParams(n_epochs=200, densmap=True, output_dens=True,
+                                               random_seed=s))
+        tg.append(trustworthiness(x, emb))
+        to.append(trustworthiness(x, fo.embedding))
+        cg.append(np.corrcoef(ro, re)[0, 1])
+        co.append(np.corrcoef(fo.rad_orig, fo.rad_emb)[0, 1])
+    print(f"densMAP fit: trust gpu {np.mean(tg):.4f} oracle {np.mean(to):.4f}; radius corr gpu "
+          f"{np.mean(cg):.3f} oracle {np.mean(co):.3f}", flush=True)
+    assert abs(np.mean(tg) - np.mean(to)) <= 0.01, (tg, to)
+    assert abs(np.mean(cg) - np.mean(co)) <= 0.05, (cg, co)
 
 
 def test_out_buffer_and_params(gpu):
@@ -159,3 +188,25 @@ def test_out_buffer_and_params(gpu):
     with pytest.raises(ValueError, match="output buffer shape mismatch"):
         m.fit_transform(x, out=np.empty((10, 2), np.float32))
     assert m.get_params()["n_epochs"] == 50
+
+
+def test_transform_launch_configuration_determinism(gpu):
+    # the analog of lib.rs:6193-6236 (the same transform whatever the rayon
+    # thread count).  A query's kNN into the training rows is row-local, so it
+    # comes out bit for bit the same alone, in a short batch or in a long one
+    # (different grids, block splits and merges); the whole transform of one
+    # batch is the same on a repeat and after unrelated GPU work.  (Across
+    # batches the transform itself differs, in the reference too: pruning and
+    # the sampling schedule use the batch's largest edge weight.)
+    x = datasets.gaussian_mixture(3000, 40, 5, 3.0, 21, 1.0)
+    q = datasets.gaussian_mixture(700, 40, 5, 3.0, 21, 1.0) + np.float32(0.05)
+    i_all, d_all = capi.knn_to_reference(q, x, 15)
+    for rows in (slice(0, 1), slice(0, 37), slice(300, 700)):
+        i_part, d_part = capi.knn_to_reference(q[rows], x, 15)
+        assert np.array_equal(i_part, i_all[rows]) and np.array_equal(d_part, d_all[rows])
+    m = Umap(n_epochs=100, random_seed=4, ann_mode="exact")
+    m.fit(x)
+    full = m.transform(q)
+    assert np.array_equal(full, m.transform(q))
+    Umap(n_epochs=30, random_seed=9, ann_mode="exact").fit_transform(x[:900])
+    assert np.array_equal(full, m.transform(q))

@@ -112,3 +112,34 @@ def test_spectral_init_components(gpu, oracle):
         assert np.abs(means_g[c] - means_o[c] * flip).max() < 0.05 * span
         assert abs(np.ptp(yg[m], axis=0).max() / span - 1.0) < 0.02
         assert _subspace_cos(yg[m] - yg[m].mean(0), yo[m] - yo[m].mean(0)) > 0.99
+
+
+@pytest.mark.parametrize("dim,scales", [(1, (4.0,)), (3, (4.0, 3.2, 2.6))])
+def test_spectral_init_other_dims(gpu, oracle, dim, scales):
+    # (dim + 1) * 6 iterated columns (lib.rs:4231-4234) for 1-D and 3-D output:
+    # the leading axes of a blob stretched along dim axes give the smoothest
+    # non-trivial eigenvectors, with a gap to the next
+    rng = np.random.default_rng(31 + dim)
+    sc = np.ones(24)
+    sc[:dim] = scales
+    x = (rng.normal(size=(3000, 24)) * sc).astype(np.float32)
+    h, t, w = _graph(oracle, x)
+    yo, ok_o = oracle.initialize_embedding(x, dim, h, t, w, "spectral", 42)
+    yg, ok_g = capi.init_embedding(x, dim, h, t, w, init=1, seed=42)
+    assert ok_o and ok_g and yg.shape == (3000, dim) and np.isfinite(yg).all()
+    assert abs(np.abs(yg).max() - 10.0) < 1e-3
+    assert _subspace_cos(yg, yo) > 0.99
+
+
+def test_three_component_fit_uses_spectral_init(gpu, oracle):
+    from umapers_amd import Umap, datasets
+    from umapers_amd.diagnostics import trustworthiness
+
+    x = datasets.gaussian_mixture(3000, 32, 5, 3.0, 8, 1.0)
+    m = Umap(n_components=3, n_epochs=200, random_seed=7, ann_mode="exact")
+    prof = m.profile_fit_transform(x)
+    assert prof["gpu"]["init_used_spectral"] == 1
+    y = prof["embedding"]
+    assert y.shape == (3000, 3) and np.isfinite(y).all()
+    yo = oracle.fit(x, oracle.OracleParams(n_components=3, n_epochs=200, random_seed=7)).embedding
+    assert abs(trustworthiness(x, y) - trustworthiness(x, yo)) <= 0.01

@@ -157,3 +157,19 @@ def test_wide_rows_fit_parity(gpu, oracle):
     g = Umap(n_epochs=200, random_seed=5, ann_mode="exact").fit_transform(x)
     fit = oracle.fit(x, oracle.OracleParams(n_epochs=200, random_seed=5))
     assert abs(trustworthiness(x, g) - trustworthiness(x, fit.embedding)) <= 0.01
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_wide_rows_several_score_blocks(gpu, oracle, rng, monkeypatch, metric):
+    # a score block of 192 query rows (the test hook lowers the block budget),
+    # so 3000 rows run 15 full blocks and a partial one; FP32 stays bit-exact
+    monkeypatch.setenv("UMAPX_TEST_WIDE_SCORE_BYTES", str(192 * 3000 * 4))
+    x = rng.normal(size=(3000, 300)).astype(np.float32)
+    idx, dist = capi.knn(x, 15, metric, 0)
+    oi, od = oracle.knn(x, 15, "cosine" if metric else "euclidean")
+    assert np.array_equal(idx, oi)
+    assert np.array_equal(dist.view(np.int32), od.view(np.int32))
+    q = rng.normal(size=(700, 300)).astype(np.float32)
+    idx, dist = capi.knn_to_reference(q, x, 15, metric, 1)
+    oi, od = oracle.knn_to_reference(q, x, 15, "cosine" if metric else "euclidean")
+    assert recall_at_k(idx, oi) >= 0.99

@@ -57,7 +57,8 @@ def test_batch_gradients_match_fp64_oracle(gpu, d, h, dim):
 def test_learned_transform_trains_and_gains_over_plain(gpu):
     x_fit, x_q = datasets.c5(n_fit=20000, n_query=100000)
     judge = x_q[:2000]
-    model = Umap(n_epochs=200, transform_mode="learned", mlp_steps=600, random_seed=3)
+    model = Umap(n_epochs=200, transform_mode="learned", mlp_steps=600, random_seed=3,
+                 ann_mode="exact")
     y_fit = model.fit_transform(x_fit)
     first, last, steps = model.learned_transform_loss()
     assert steps == 600 and np.isfinite(last) and last < 0.5 * first, (first, last)
@@ -72,7 +73,8 @@ def test_learned_transform_trains_and_gains_over_plain(gpu):
     assert t_mlp >= t_plain - 0.02, (t_mlp, t_plain)
 
     # the same encoder after a single step: training is what buys the match
-    raw = Umap(n_epochs=200, transform_mode="learned", mlp_steps=1, random_seed=3)
+    raw = Umap(n_epochs=200, transform_mode="learned", mlp_steps=1, random_seed=3,
+               ann_mode="exact")
     raw.fit(x_fit)
     y_raw = raw.transform_learned(judge)
     assert np.median(np.linalg.norm(y_raw - y_plain, axis=1)) > 4 * np.median(per_row)

@@ -47,16 +47,26 @@ def test_single_edge_closed_form_attraction(gpu):
 
 
 def test_sgd_statistical_parity(gpu, oracle, graph):
+    # the same graph and oracle start over six seeds: mean trustworthiness and
+    # kNN preservation of the GPU epochs within 0.01 of the oracle's, both
+    # ways (one run moves by ~0.006 between seeds)
     x, h, t, w = graph
     a, b = oracle.find_ab_params(1.0, 0.1)
-    y0, _ = oracle.initialize_embedding(x, 2, h, t, w, "spectral", 42)
-    yo = oracle.optimize_layout(y0, h, t, w, 200, a, b)
-    yg = capi.optimize(y0, h, t, w, 200, a, b)
-    assert np.isfinite(yg).all()
-    to, tg = trustworthiness(x, yo), trustworthiness(x, yg)
-    ko, kg = knn_preservation(x, yo), knn_preservation(x, yg)
-    assert tg >= to - 0.01, (tg, to)
-    assert kg >= ko - 0.01, (kg, ko)
+    to, tg, ko, kg = [], [], [], []
+    for s in range(6):
+        y0, _ = oracle.initialize_embedding(x, 2, h, t, w, "spectral", s)
+        ss = s ^ 0x9E3779B97F4A7C15
+        yo = oracle.optimize_layout(y0, h, t, w, 200, a, b, seed=ss)
+        yg = capi.optimize(y0, h, t, w, 200, a, b, seed=ss)
+        assert np.isfinite(yg).all()
+        to.append(trustworthiness(x, yo))
+        tg.append(trustworthiness(x, yg))
+        ko.append(knn_preservation(x, yo))
+        kg.append(knn_preservation(x, yg))
+    print(f"sgd parity: trust gpu {np.mean(tg):.4f} oracle {np.mean(to):.4f}; knn-pres gpu "
+          f"{np.mean(kg):.4f} oracle {np.mean(ko):.4f}", flush=True)
+    assert abs(np.mean(tg) - np.mean(to)) <= 0.01, (tg, to)
+    assert abs(np.mean(kg) - np.mean(ko)) <= 0.01, (kg, ko)
 
 
 def test_densmap_term_changes_local_scale(gpu, oracle, graph):
@@ -64,22 +74,29 @@ def test_densmap_term_changes_local_scale(gpu, oracle, graph):
     a, b = oracle.find_ab_params(1.0, 0.1)
     idx, dist = oracle.knn(x, 15)
     rad = oracle.local_radii(idx, dist)
-    y0, _ = oracle.initialize_embedding(x, 2, h, t, w, "spectral", 42)
     start = oracle.density_start_epoch(200, 0.3)
-    yo = oracle.optimize_layout(y0, h, t, w, 200, a, b, radii=rad, dens_start=start)
-    yg = capi.optimize(y0, h, t, w, 200, a, b, radii=rad, dens_start=start)
-    yg0 = capi.optimize(y0, h, t, w, 200, a, b)
-    assert np.isfinite(yg).all()
+    ro, rg, rg0, to, tg = [], [], [], [], []
+    for s in range(3):
+        y0, _ = oracle.initialize_embedding(x, 2, h, t, w, "spectral", s)
+        ss = s ^ 0x9E3779B97F4A7C15
+        yo = oracle.optimize_layout(y0, h, t, w, 200, a, b, radii=rad, dens_start=start, seed=ss)
+        yg = capi.optimize(y0, h, t, w, 200, a, b, radii=rad, dens_start=start, seed=ss)
+        yg0 = capi.optimize(y0, h, t, w, 200, a, b, seed=ss)
+        assert np.isfinite(yg).all()
+        ro.append(np.corrcoef(rad, oracle.embedding_radii(yo, idx))[0, 1])
+        rg.append(np.corrcoef(rad, oracle.embedding_radii(yg, idx))[0, 1])
+        rg0.append(np.corrcoef(rad, oracle.embedding_radii(yg0, idx))[0, 1])
+        to.append(trustworthiness(x, yo))
+        tg.append(trustworthiness(x, yg))
+    print(f"densMAP term: radius corr gpu {np.mean(rg):.3f} (without {np.mean(rg0):.3f}) oracle "
+          f"{np.mean(ro):.3f}; trust gpu {np.mean(tg):.4f} oracle {np.mean(to):.4f}", flush=True)
     # the density term ties embedding radii to the original ones: the GPU
-    # run must show the effect (vs its own run without the term) at a size
-    # comparable to the oracle's; this correlation moves by ~0.01 between
-    # seeds on the CPU and more between Hogwild runs
-    ro = np.corrcoef(rad, oracle.embedding_radii(yo, idx))[0, 1]
-    rg = np.corrcoef(rad, oracle.embedding_radii(yg, idx))[0, 1]
-    rg0 = np.corrcoef(rad, oracle.embedding_radii(yg0, idx))[0, 1]
-    assert rg >= rg0 + 0.15, (rg, rg0)
-    assert rg >= ro - 0.1, (rg, ro)
-    assert trustworthiness(x, yg) >= trustworthiness(x, yo) - 0.01
+    # runs show the effect (vs their own runs without the term) at the size
+    # the oracle shows it; this correlation moves by ~0.01 between seeds on
+    # the CPU and more between Hogwild runs
+    assert np.mean(rg) >= np.mean(rg0) + 0.15, (rg, rg0)
+    assert abs(np.mean(rg) - np.mean(ro)) <= 0.1, (rg, ro)
+    assert abs(np.mean(tg) - np.mean(to)) <= 0.01, (tg, to)
 
 
 def test_hub_vertices_stay_in_their_cluster(gpu):

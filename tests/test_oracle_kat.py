@@ -215,3 +215,41 @@ def test_components_ordered_by_smallest_member(oracle):
     nc, lab = oracle.connected_components(7, h, t)
     assert nc == 4
     assert lab.tolist() == [0, 1, 2, 0, 1, 2, 3]
+
+
+def test_edge_laplacian_equals_dense_affinity_laplacian(oracle, rng):
+    # lib.rs:7457-7538: the edge-list Laplacian equals the one built from the
+    # dense, max-symmetrised affinity matrix (to 1e-12), on the reference's
+    # 5-node graph and on a random one; and both equal I - D^-1/2 W D^-1/2
+    # formed here in numpy
+    graphs = [(5, [0, 1, 1, 2, 3, 4, 0], [1, 0, 2, 3, 4, 3, 4],
+               [0.4, 0.7, 0.8, 0.6, 0.2, 0.5, 0.3])]
+    n = 40
+    h = rng.integers(0, n, 200)
+    t = rng.integers(0, n, 200)
+    graphs.append((n, h.tolist(), t.tolist(), rng.uniform(0, 1, 200).astype(np.float32).tolist()))
+    for n, h, t, w in graphs:
+        a = oracle.normalized_laplacian(n, h, t, w)
+        b = oracle.normalized_laplacian(n, h, t, w, from_affinity=True)
+        assert np.abs(a - b).max() < 1e-12
+        W = np.zeros((n, n))
+        for hh, tt, ww in zip(h, t, w):
+            if hh != tt:
+                W[hh, tt] = max(W[hh, tt], float(np.float32(ww)))
+        W = np.maximum(W, W.T)
+        deg = W.sum(1)
+        inv = np.where(deg > 0, 1.0 / np.sqrt(np.where(deg > 0, deg, 1.0)), 0.0)
+        L = np.eye(n) - inv[:, None] * W * inv[None, :]
+        assert np.abs(a - L).max() < 1e-12
+
+
+def test_densmap_zero_lambda_is_the_standard_embedding(oracle):
+    # lib.rs:6078-6106: densMAP with lambda = 0 leaves the embedding unchanged
+    x = np.random.default_rng(3).normal(size=(400, 12)).astype(np.float32)
+    plain = oracle.fit(x, oracle.OracleParams(n_epochs=60, random_seed=9)).embedding
+    dm0 = oracle.fit(x, oracle.OracleParams(n_epochs=60, random_seed=9, densmap=True,
+                                            dens_lambda=0.0)).embedding
+    assert np.array_equal(plain, dm0)
+    dm = oracle.fit(x, oracle.OracleParams(n_epochs=60, random_seed=9, densmap=True,
+                                           dens_lambda=2.0)).embedding
+    assert not np.array_equal(plain, dm)

@@ -115,6 +115,9 @@ class Timings(ctypes.Structure):
 SIGNATURES = {
     "umapx_default_params": (None, [ctypes.POINTER(Params)]),
     "umapx_last_error": (ctypes.c_char_p, []),
+    "umapx_last_error_detail": (ctypes.c_int32, [_i64p, _i64p, _i64p]),
+    "umapx_uses_approximate_knn": (ctypes.c_int32, [ctypes.POINTER(Params), ctypes.c_int64,
+                                                    ctypes.c_int64]),
     "umapx_version": (ctypes.c_char_p, []),
     "umapx_device_count": (ctypes.c_int32, []),
     "umapx_create": (ctypes.c_int32, [ctypes.POINTER(Params), ctypes.POINTER(_vp)]),

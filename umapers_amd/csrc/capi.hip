@@ -50,9 +50,18 @@ void retain_pool_memory() {
 
 namespace {
 thread_local std::string g_last_error;
+struct LastErrorDetail {
+  int32_t code = 0;
+  int64_t row = -1, expected = -1, got = -1;
+};
+thread_local LastErrorDetail g_last_detail;
 }  // namespace
 
-void set_last_error(const std::string& msg) { g_last_error = msg; }
+int32_t set_last_error(const Error& e) {
+  g_last_error = e.what();
+  g_last_detail = LastErrorDetail{e.code, e.row, e.expected, e.got};
+  return e.code;
+}
 
 void maybe_inject_fault(const char* stage) {
   static const char* env = std::getenv("UMAPX_FAULT_INJECT");
@@ -149,6 +158,9 @@ void validate_params(const umapx_params& p, int64_t n_samples) {
     throw_invalid("n_neighbors (" + std::to_string(p.n_neighbors) +
                   ") must be smaller than number of samples (" + std::to_string(n_samples) + ")");
   if (p.n_components == 0 || p.n_components < 0) throw_invalid("n_components must be >= 1");
+  // checked before any device work: the GPU layout kernels hold at most 8
+  // coordinates per row
+  if (p.n_components > 8) throw_invalid("n_components must be <= 8 on the GPU path");
   if (!std::isfinite(p.learning_rate) || p.learning_rate <= 0.0f)
     throw_invalid("learning_rate must be finite and > 0");
   if (!std::isfinite(p.min_dist) || p.min_dist < 0.0f) throw_invalid("min_dist must be finite and >= 0");
@@ -163,6 +175,7 @@ void validate_params(const umapx_params& p, int64_t n_samples) {
     throw_invalid("local_connectivity must be finite and >= 0");
   if (p.approx_knn_candidates <= 0) throw_invalid("approx_knn_candidates must be >= 1");
   if (p.approx_knn_iters <= 0) throw_invalid("approx_knn_iters must be >= 1");
+  if (p.approx_knn_threshold < 0) throw_invalid("approx_knn_threshold must be >= 0");
   if (!std::isfinite(p.dens_lambda) || p.dens_lambda < 0.0f) throw_invalid("dens_lambda must be finite and >= 0");
   if (!std::isfinite(p.dens_frac) || !(p.dens_frac >= 0.0f && p.dens_frac <= 1.0f))
     throw_invalid("dens_frac must be finite and in [0, 1]");
@@ -171,6 +184,21 @@ void validate_params(const umapx_params& p, int64_t n_samples) {
   if (p.knn_precision != UMAPX_KNN_FP32 && p.knn_precision != UMAPX_KNN_BF16_RERANK)
     throw_invalid("knn_precision must be fp32 or bf16_rerank");
   if (p.rerank_factor < 1) throw_invalid("rerank_factor must be >= 1");
+}
+
+// should_use_approximate_knn (lib.rs:2285-2311, constants lib.rs:24-26):
+// auto mode switches to the approximate rounds only past 4 x the threshold
+// (16 features or fewer: the same multiplier); a threshold of 0 is the
+// forced "approximate" mode.
+bool should_use_approximate_knn(const umapx_params& p, int64_t n_samples, int64_t n_features) {
+  constexpr int64_t kLowDimMaxFeatures = 16, kLowDimMultiplier = 4, kGeneralMultiplier = 4;
+  const int64_t threshold = std::max<int64_t>(p.approx_knn_threshold, 0);
+  if (!p.use_approximate_knn || n_samples <= threshold) return false;
+  if (threshold > 0) {
+    const int64_t mult = n_features <= kLowDimMaxFeatures ? kLowDimMultiplier : kGeneralMultiplier;
+    if (n_samples <= threshold * mult) return false;
+  }
+  return true;
 }
 
 int32_t default_epochs(const umapx_params& p, int64_t n) {
@@ -312,10 +340,7 @@ struct umapx_handle {
     if (pre_idx == nullptr) {
       idx.alloc(static_cast<size_t>(n_) * k);
       dist.alloc(static_cast<size_t>(n_) * k);
-      // ann_mode "approximate" (use_approximate_knn with threshold 0) runs the
-      // reference's approximate rounds; auto mode keeps the exact kNN, which
-      // on this GPU is faster than the rounds and exact
-      const bool approx = p.use_approximate_knn && p.approx_knn_threshold == 0;
+      const bool approx = should_use_approximate_knn(p, n_, d_);
       if (approx)
         knn_approx_device(d_x, n_, d_, k, p.metric, p.metric_p, p.approx_knn_candidates,
                           p.approx_knn_iters, p.random_seed ^ 0xC0FEFEED1234ABCDULL, idx.get(),
@@ -499,7 +524,8 @@ struct umapx_handle {
     if (!fitted) throw Error(kNotFitted, "model is not fitted yet");
     if (m > 0 && dq != d)
       throw Error(kFeatureMismatch,
-                  "feature mismatch: expected " + std::to_string(d) + ", got " + std::to_string(dq));
+                  "feature mismatch: expected " + std::to_string(d) + ", got " + std::to_string(dq),
+                  -1, d, dq);
   }
 
   void transform_host(const float* q, int64_t m, int64_t dq, float* out, int mode) {
@@ -541,7 +567,8 @@ struct umapx_handle {
     if (m > 0 && dq != p.n_components)
       throw Error(kEmbeddingDimensionMismatch,
                   "embedding dimension mismatch: expected " + std::to_string(p.n_components) +
-                      ", got " + std::to_string(dq));
+                      ", got " + std::to_string(dq),
+                  -1, p.n_components, dq);
   }
 
   void inverse_host(const float* q, int64_t m, int64_t dq, float* out) {
@@ -582,11 +609,9 @@ int32_t guarded(F&& f) {
     f();
     return UMAPX_OK;
   } catch (const Error& e) {
-    g_last_error = e.what();
-    return e.code;
+    return set_last_error(e);
   } catch (const std::exception& e) {
-    g_last_error = std::string("device error: ") + e.what();
-    return UMAPX_ERR_DEVICE;
+    return set_last_error(Error(kDeviceError, std::string("device error: ") + e.what()));
   }
 }
 
@@ -640,6 +665,17 @@ void umapx_default_params(umapx_params* o) {
 }
 
 const char* umapx_last_error(void) { return g_last_error.c_str(); }
+
+int32_t umapx_last_error_detail(int64_t* row, int64_t* expected, int64_t* got) {
+  if (row) *row = g_last_detail.row;
+  if (expected) *expected = g_last_detail.expected;
+  if (got) *got = g_last_detail.got;
+  return g_last_detail.code;
+}
+
+int32_t umapx_uses_approximate_knn(const umapx_params* p, int64_t n_samples, int64_t n_features) {
+  return p && should_use_approximate_knn(*p, n_samples, n_features) ? 1 : 0;
+}
 
 const char* umapx_version(void) { return "umapx 0.1.0 (gfx950)"; }
 

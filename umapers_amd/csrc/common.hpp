@@ -16,7 +16,14 @@ namespace umapx {
 // return code plus the thread-local message (umapx_last_error).
 struct Error : std::runtime_error {
   int code;
+  // The sizes of UmapError's structured variants (lib.rs:198-216):
+  // InconsistentDimensions{row, expected, got}, FeatureMismatch{expected,
+  // got}, EmbeddingDimensionMismatch{expected, got}; -1 where a variant has
+  // no such field.
+  int64_t row = -1, expected = -1, got = -1;
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+  Error(int c, const std::string& m, int64_t r, int64_t e, int64_t g)
+      : std::runtime_error(m), code(c), row(r), expected(e), got(g) {}
 };
 
 // Codes mirror UmapError (rust_umap/src/lib.rs:198-244) plus a device error.
@@ -36,8 +43,9 @@ enum : int {
   throw Error(kInvalidParameter, "invalid parameter: " + msg);
 }
 
-// Sets the calling thread's umapx_last_error() message.
-void set_last_error(const std::string& msg);
+// Records e as the calling thread's last failure (umapx_last_error and
+// umapx_last_error_detail) and returns its code.
+int32_t set_last_error(const Error& e);
 
 void check_hip(hipError_t e, const char* what, const char* file, int line);
 #define HIP_CHECK(expr) ::umapx::check_hip((expr), #expr, __FILE__, __LINE__)

@@ -593,9 +593,9 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     int metric, const float* __restrict__ qnorm, const float* __restrict__ rnorm,
     const int32_t* __restrict__ cand, int k, int32_t* __restrict__ out_idx,
     float* __restrict__ out_dist) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t row = gid / KC;
-  const int j = gid % KC;
+  const int j = static_cast<int>(gid % KC);
   const bool live = row < m;
   const int64_t rr = live ? row : m - 1;
   int32_t c = live ? cand[rr * KC + j] : -1;

@@ -6,7 +6,19 @@
 namespace umapx {
 
 // Reusable device scratch for knn_device (grown on demand).
+// A GEMM library handle owned by one workspace (so concurrent handles on
+// different streams never share one); created on first use by the wide-row
+// path, released with the workspace.
+struct BlasHandle {
+  void* h = nullptr;
+  BlasHandle() = default;
+  BlasHandle(const BlasHandle&) = delete;
+  BlasHandle& operator=(const BlasHandle&) = delete;
+  ~BlasHandle();
+};
+
 struct KnnWorkspace {
+  BlasHandle blas;
   DevBuf<double> colsum;
   DevBuf<unsigned char> qprep, rprep;
   DevBuf<float> rnorm2, qnorm, rnorm;  // rnorm2: -||y||^2/2 of the compute rows

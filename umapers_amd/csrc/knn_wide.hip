@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <climits>
 #include <map>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 
@@ -332,29 +333,12 @@ __global__ __launch_bounds__(kSelWaves * 64) void wide_select_kernel(
     cand[i * KC + t] = t < got ? static_cast<int32_t>(keys[t] & 0xFFFFFFFFu) : -1;
 }
 
-struct BlasHandles {
-  std::mutex mu;
-  std::map<int, rocblas_handle> by_device;
-};
-
-BlasHandles& blas_handles() {
-  static BlasHandles* h = new BlasHandles();  // kept until the process ends
-  return *h;
-}
-
-// The caller holds blas_handles().mu: one handle per device, whose stream is
-// set per call.
-rocblas_handle blas_handle() {
-  BlasHandles* h = &blas_handles();
-  int dev = 0;
-  HIP_CHECK(hipGetDevice(&dev));
-  auto it = h->by_device.find(dev);
-  if (it != h->by_device.end()) return it->second;
-  rocblas_handle b = nullptr;
-  if (rocblas_create_handle(&b) != rocblas_status_success)
-    throw Error(kDeviceError, "device error: rocblas_create_handle failed");
-  h->by_device[dev] = b;
-  return b;
+// The score block's size in bytes: kScoreBudget, or (tests only) the value
+// of UMAPX_TEST_WIDE_SCORE_BYTES, so that small inputs run several blocks.
+size_t score_budget() {
+  const char* e = std::getenv("UMAPX_TEST_WIDE_SCORE_BYTES");
+  const long long b = e ? std::atoll(e) : 0;
+  return b > 0 ? static_cast<size_t>(b) : kScoreBudget;
 }
 
 }  // namespace
@@ -392,12 +376,19 @@ void knn_wide_candidates(const void* qp, int64_t m, const void* rp, const float*
                        rep.get());
     KERNEL_CHECK();
   }
-  std::lock_guard<std::mutex> lock(blas_handles().mu);
-  rocblas_handle h = blas_handle();
+  // the workspace's own handle: calls on other workspaces (other umapx
+  // handles, other streams) never share it
+  if (!ws.blas.h) {
+    rocblas_handle b = nullptr;
+    if (rocblas_create_handle(&b) != rocblas_status_success)
+      throw Error(kDeviceError, "device error: rocblas_create_handle failed");
+    ws.blas.h = b;
+  }
+  rocblas_handle h = static_cast<rocblas_handle>(ws.blas.h);
   if (rocblas_set_stream(h, s) != rocblas_status_success)
     throw Error(kDeviceError, "device error: rocblas_set_stream failed");
   const int64_t ld = (n + 3) / 4 * 4;  // 16-byte aligned score rows
-  int64_t mb = static_cast<int64_t>(kScoreBudget / (sizeof(float) * static_cast<size_t>(ld)));
+  int64_t mb = static_cast<int64_t>(score_budget() / (sizeof(float) * static_cast<size_t>(ld)));
   mb = std::max<int64_t>(64, mb / 64 * 64);
   mb = std::min<int64_t>(mb, (m + 63) / 64 * 64);
   ws.scores.ensure(static_cast<size_t>(mb) * ld);
@@ -428,6 +419,10 @@ void knn_wide_candidates(const void* qp, int64_t m, const void* rp, const float*
                          ws.scores.get(), ld, rows, n, nr_half, rep.get(), cand + q0 * kc);
     KERNEL_CHECK();
   }
+}
+
+BlasHandle::~BlasHandle() {
+  if (h) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(h));
 }
 
 }  // namespace umapx

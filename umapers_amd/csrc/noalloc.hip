@@ -138,10 +138,8 @@ extern "C" int32_t umapx_knn_to_reference_device_noalloc(
     KERNEL_CHECK();
     return UMAPX_OK;
   } catch (const Error& e) {
-    set_last_error(e.what());
-    return e.code;
+    return set_last_error(e);
   } catch (const std::exception& e) {
-    set_last_error(std::string("device error: ") + e.what());
-    return UMAPX_ERR_DEVICE;
+    return set_last_error(Error(kDeviceError, std::string("device error: ") + e.what()));
   }
 }

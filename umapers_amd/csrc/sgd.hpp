@@ -34,8 +34,6 @@ struct SgdArgs {
 struct SgdWorkspace {
   DevBuf<float> eps, eons, eonns;
   DevBuf<int32_t> chunk_row, chunk_start;  // (head, first edge) of each walked chunk
-  DevBuf<int32_t> ends;     // edge ends per vertex (+ hub count)
-  DevBuf<float> mscale;     // per-vertex move scale (hubs below 1)
 };
 
 struct SgdStats {

@@ -27,13 +27,17 @@
 #include <cstring>
 #include <numeric>
 #include <thread>
+#include <type_traits>
 
 #include "hostmath.hpp"
 
 namespace umapx {
 namespace {
 
-constexpr int kSub = 18;            // (n_components + 1) * 6 for 2-D output
+// The iterated block has KS = (n_components + 1) * 6 columns
+// (lib.rs:4231-4234): the kernels below are instantiated for n_components
+// 1 .. 8, i.e. KS = 12, 18, .., 54.
+constexpr int kMaxDim = 8;
 constexpr int kIters = 256;         // SPECTRAL_ITERATIVE_MAX_ITERS
 // The block is re-orthonormalised after every kOrthEvery products instead of
 // after each one.  For an upper-triangular R with a positive diagonal,
@@ -180,6 +184,7 @@ __global__ void fill_i32_kernel(int32_t* __restrict__ a, int64_t n, int32_t v) {
 
 // Start block: column 0 = sqrt(degree), the rest U(-1, 1) (lib.rs:4252-4260;
 // a counter-based stream replaces SmallRng).
+template <int KS>
 __global__ void start_block_kernel(const int32_t* __restrict__ perm, int64_t rows,
                                    const int32_t* __restrict__ slot_of_pos,
                                    const int32_t* __restrict__ local_of_pos,
@@ -190,11 +195,11 @@ __global__ void start_block_kernel(const int32_t* __restrict__ perm, int64_t row
   const int32_t v = perm[p];
   const uint64_t key = seeds[slot_of_pos[p]] ^ 0x7A5B3D91CE42A117ULL;
   const uint64_t lr = static_cast<uint64_t>(local_of_pos[p]);
-  float* row = yp + p * kSub;
+  float* row = yp + p * KS;
   row[0] = static_cast<float>(sqrt(deg[v]));
-  for (int c = 1; c < kSub; ++c)
+  for (int c = 1; c < KS; ++c)
     row[c] = static_cast<float>(
-        2.0 * uniform01(key + lr * kSub + static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ULL) - 1.0);
+        2.0 * uniform01(key + lr * KS + static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ULL) - 1.0);
 }
 
 // Work items are laid out component by component.  Hardware hands block b to
@@ -206,20 +211,22 @@ __device__ __forceinline__ int xcd_item(int b, int nblocks) {
   return x * q + (x < r ? x : r) + b / 8;
 }
 
+template <int KS>
 __device__ __forceinline__ void pair_of(int tid, int& pi, int& pj) {
   int t = tid, i = 0;
-  while (i < kSub && t >= kSub - i) {
-    t -= kSub - i;
+  while (i < KS && t >= KS - i) {
+    t -= KS - i;
     ++i;
   }
   pi = i;
   pj = i + t;
 }
 
-// z = y + D^-1/2 W D^-1/2 y, one launch per product.  A wave holds 7 lane
-// groups (lanes 9g .. 9g+8; lane 63 idles), lane j of a group carrying
-// columns 2j, 2j+1.  Rows of at most kSeg entries are packed 7 to a wave, one
-// per group.  A longer row (hub rows run to hundreds of entries after the
+// z = y + D^-1/2 W D^-1/2 y, one launch per product.  A wave holds
+// SpmmShape<KS>::kGroups lane groups of KS / 2 lanes (lanes past the last
+// group idle), lane j of a group carrying columns 2j, 2j+1 (for 2-D output,
+// KS = 18: 7 groups of 9).  Rows of at most kSeg entries are packed one per
+// group.  A longer row (hub rows run to hundreds of entries after the
 // union) takes a whole wave: its kSeg-entry pieces are dealt to the groups
 // in turn, and the groups' partial sums are added in group order through
 // LDS, so the result does not depend on scheduling.  For every piece the
@@ -233,8 +240,12 @@ __device__ __forceinline__ void pair_of(int tid, int& pi, int& pj) {
 // normalised weights are formed in f64 and rounded once; the Gram matrices
 // and the Cholesky factors below stay in f64.
 constexpr int kSeg = 64;
-constexpr int kGroups = 7;
 constexpr int kSpmmWaves = 4;
+template <int KS>
+struct SpmmShape {
+  static constexpr int kLanes = KS / 2;        // lanes of one group
+  static constexpr int kGroups = 64 / kLanes;  // groups of one wave
+};
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -242,18 +253,20 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Waves 0 .. n_long-1 take the long rows, the rest 7 short rows each.
+// Waves 0 .. n_long-1 take the long rows, the rest kGroups short rows each.
+template <int KS>
 __global__ __launch_bounds__(64 * kSpmmWaves) void spmm_kernel(
     int64_t n_long, int64_t n_short, const int32_t* __restrict__ long_list,
     const int32_t* __restrict__ short_list, const int32_t* __restrict__ prow_ptr,
     const int32_t* __restrict__ pcol, const float* __restrict__ pnw,
     const float* __restrict__ yp, float* __restrict__ zp) {
+  constexpr int kLanes = SpmmShape<KS>::kLanes, kGroups = SpmmShape<KS>::kGroups;
   __shared__ int2 staged[kSpmmWaves][kGroups][kSeg];
-  __shared__ float2 partial[kSpmmWaves][kGroups][9];
+  __shared__ float2 partial[kSpmmWaves][kGroups][kLanes];
   const int wv = threadIdx.x >> 6;
   const int wl = threadIdx.x & 63;
-  const int g = wl / 9;
-  const int lj = wl - 9 * g;
+  const int g = wl / kLanes;
+  const int lj = wl - kLanes * g;
   const int64_t w = static_cast<int64_t>(blockIdx.x) * kSpmmWaves + wv;
   const float2* y2 = reinterpret_cast<const float2*>(yp);
   float2* z2 = reinterpret_cast<float2*>(zp);
@@ -261,7 +274,7 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void spmm_kernel(
 
   // sum of w * y[col] over entries [e0, e0 + cnt) of one piece (cnt <= kSeg)
   auto piece = [&](int32_t e0, int32_t cnt, float2 acc) {
-    for (int k = lj; k < cnt; k += 9) st[k] = make_int2(pcol[e0 + k], __float_as_int(pnw[e0 + k]));
+    for (int k = lj; k < cnt; k += kLanes) st[k] = make_int2(pcol[e0 + k], __float_as_int(pnw[e0 + k]));
     wave_sync();
     int k = 0;
     for (; k + 8 <= cnt; k += 8) {
@@ -270,7 +283,7 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void spmm_kernel(
 #pragma unroll
       for (int j = 0; j < 8; ++j) c[j] = st[k + j];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) q[j] = y2[static_cast<int64_t>(c[j].x) * (kSub / 2) + lj];
+      for (int j = 0; j < 8; ++j) q[j] = y2[static_cast<int64_t>(c[j].x) * kLanes + lj];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const float wt = __int_as_float(c[j].y);
@@ -280,7 +293,7 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void spmm_kernel(
     }
     for (; k < cnt; ++k) {
       const int2 c = st[k];
-      const float2 q = y2[static_cast<int64_t>(c.x) * (kSub / 2) + lj];
+      const float2 q = y2[static_cast<int64_t>(c.x) * kLanes + lj];
       const float wt = __int_as_float(c.y);
       acc.x = fmaf(wt, q.x, acc.x);
       acc.y = fmaf(wt, q.y, acc.y);
@@ -299,21 +312,21 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void spmm_kernel(
     if (g < kGroups) partial[wv][g][lj] = acc;
     wave_sync();
     if (g == 0) {
-      float2 sum = y2[static_cast<int64_t>(p) * (kSub / 2) + lj];
+      float2 sum = y2[static_cast<int64_t>(p) * kLanes + lj];
 #pragma unroll
       for (int q = 0; q < kGroups; ++q) {
         sum.x += partial[wv][q][lj].x;
         sum.y += partial[wv][q][lj].y;
       }
-      z2[static_cast<int64_t>(p) * (kSub / 2) + lj] = sum;
+      z2[static_cast<int64_t>(p) * kLanes + lj] = sum;
     }
   } else {
     const int64_t i = (w - n_long) * kGroups + g;
     if (g >= kGroups || i >= n_short) return;
     const int32_t p = short_list[i];
     const int32_t r0 = prow_ptr[p], r1 = prow_ptr[p + 1];
-    const float2 acc = piece(r0, r1 - r0, y2[static_cast<int64_t>(p) * (kSub / 2) + lj]);
-    z2[static_cast<int64_t>(p) * (kSub / 2) + lj] = acc;
+    const float2 acc = piece(r0, r1 - r0, y2[static_cast<int64_t>(p) * kLanes + lj]);
+    z2[static_cast<int64_t>(p) * kLanes + lj] = acc;
   }
 }
 
@@ -338,27 +351,27 @@ __global__ void short_fill_kernel(int64_t rows, const int32_t* __restrict__ flag
 
 // The item's share of Z^T Z (or Y^T Z when CROSS) added to gram[slot], with
 // f64 products of the stored f32 values; the rows are already computed.
-template <bool CROSS>
+template <bool CROSS, int KS>
 __global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ items,
                                                    const float* __restrict__ yp,
                                                    const float* __restrict__ zp,
                                                    double* __restrict__ gram) {
-  __shared__ float sz[kItemRows][kSub + 1];
-  __shared__ float sy[CROSS ? kItemRows : 1][kSub + 1];
+  __shared__ float sz[kItemRows][KS + 1];
+  __shared__ float sy[CROSS ? kItemRows : 1][KS + 1];
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int32_t slot = items[item * 3 + 0];
   const int32_t p0 = items[item * 3 + 1];
   const int32_t cnt = items[item * 3 + 2] - p0;
   const int tid = threadIdx.x;
-  for (int q = tid; q < cnt * kSub; q += 256) {
-    const int r = q / kSub, c = q % kSub;
-    sz[r][c] = zp[(static_cast<int64_t>(p0) + r) * kSub + c];
-    if (CROSS) sy[r][c] = yp[(static_cast<int64_t>(p0) + r) * kSub + c];
+  for (int q = tid; q < cnt * KS; q += 256) {
+    const int r = q / KS, c = q % KS;
+    sz[r][c] = zp[(static_cast<int64_t>(p0) + r) * KS + c];
+    if (CROSS) sy[r][c] = yp[(static_cast<int64_t>(p0) + r) * KS + c];
   }
   __syncthreads();
-  if (tid < kSub * (kSub + 1) / 2) {
+  for (int pair = tid; pair < KS * (KS + 1) / 2; pair += 256) {
     int pi, pj;
-    pair_of(tid, pi, pj);
+    pair_of<KS>(pair, pi, pj);
     double acc = 0.0;
     if (CROSS) {
       for (int r = 0; r < cnt; ++r)
@@ -367,7 +380,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const int32_t* __restrict__ i
       for (int r = 0; r < cnt; ++r)
         acc = fma(static_cast<double>(sz[r][pi]), static_cast<double>(sz[r][pj]), acc);
     }
-    atomicAdd(&gram[static_cast<int64_t>(slot) * kSub * kSub + pi * kSub + pj], acc);
+    atomicAdd(&gram[static_cast<int64_t>(slot) * KS * KS + pi * KS + pj], acc);
   }
 }
 
@@ -390,17 +403,18 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 // with row j's entries broadcast by readlane.  The subtractions reach each
 // entry in the order k = 0, 1, .., j-1, i.e. G[j][c] - sum_k R[k][j] R[k][c]
 // in index order, and R^-1 is formed column by column by back substitution.
+template <int KS>
 __global__ __launch_bounds__(64) void chol_kernel(double* __restrict__ gram,
                                                   double* __restrict__ rinv,
                                                   int32_t* __restrict__ fail) {
   const int c = threadIdx.x;
-  double* g = gram + static_cast<int64_t>(blockIdx.x) * kSub * kSub;
-  double a[kSub];
+  double* g = gram + static_cast<int64_t>(blockIdx.x) * KS * KS;
+  double a[KS];
 #pragma unroll
-  for (int i = 0; i < kSub; ++i) a[i] = (c < kSub && i <= c) ? g[i * kSub + c] : 0.0;
+  for (int i = 0; i < KS; ++i) a[i] = (c < KS && i <= c) ? g[i * KS + c] : 0.0;
   bool bad = false;
 #pragma unroll
-  for (int j = 0; j < kSub; ++j) {
+  for (int j = 0; j < KS; ++j) {
     double piv = lane_value(a[j], j);
     if (!(piv > kOrthoEps) || !isfinite(piv)) {
       bad = true;
@@ -409,27 +423,27 @@ __global__ __launch_bounds__(64) void chol_kernel(double* __restrict__ gram,
     const double rjj = sqrt(piv);
     a[j] = c == j ? rjj : (c > j ? a[j] / rjj : a[j]);  // row j of R
 #pragma unroll
-    for (int i = j + 1; i < kSub; ++i) {
+    for (int i = j + 1; i < KS; ++i) {
       const double rji = lane_value(a[j], i);
       if (c >= i) a[i] = fma(-rji, a[j], a[i]);
     }
   }
   // column c of R^-1: x[i] = -(sum_{k=i+1..c} R[i][k] x[k]) / R[i][i], x[c] = 1 / R[c][c]
-  double x[kSub];
+  double x[KS];
 #pragma unroll
-  for (int i = kSub - 1; i >= 0; --i) {
+  for (int i = KS - 1; i >= 0; --i) {
     double v = 0.0;
 #pragma unroll
-    for (int k = i + 1; k < kSub; ++k) v = fma(lane_value(a[i], k), x[k], v);
+    for (int k = i + 1; k < KS; ++k) v = fma(lane_value(a[i], k), x[k], v);
     const double rii = lane_value(a[i], i);
     x[i] = c == i ? 1.0 / rii : (c > i ? -v / rii : 0.0);
   }
-  if (c < kSub) {
-    double* out = rinv + static_cast<int64_t>(blockIdx.x) * kSub * kSub;
+  if (c < KS) {
+    double* out = rinv + static_cast<int64_t>(blockIdx.x) * KS * KS;
 #pragma unroll
-    for (int i = 0; i < kSub; ++i) out[i * kSub + c] = x[i];
+    for (int i = 0; i < KS; ++i) out[i * KS + c] = x[i];
   }
-  for (int q = c; q < kSub * kSub; q += 64) g[q] = 0.0;
+  for (int q = c; q < KS * KS; q += 64) g[q] = 0.0;
   if (c == 0 && bad) fail[blockIdx.x] = 1;
 }
 
@@ -438,28 +452,29 @@ __global__ __launch_bounds__(64) void chol_kernel(double* __restrict__ gram,
 // stored f32 rows, so the factor is accurate to about kappa^2 * 2^-53, far
 // below the 2^-24 of storing the result; a second pass could not take the
 // block's orthogonality past that storage rounding.
+template <int KS>
 __global__ __launch_bounds__(kItemRows) void rotate_kernel(const int32_t* __restrict__ items,
                                                            const double* __restrict__ rinv,
                                                            const float* __restrict__ src,
                                                            float* __restrict__ dst) {
-  __shared__ double Ri[kSub][kSub];
+  __shared__ double Ri[KS][KS];
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int32_t slot = items[item * 3 + 0];
   const int32_t p0 = items[item * 3 + 1];
   const int32_t cnt = items[item * 3 + 2] - p0;
   const int tid = threadIdx.x;
-  const double* r = rinv + static_cast<int64_t>(slot) * kSub * kSub;
-  for (int q = tid; q < kSub * kSub; q += kItemRows) Ri[q / kSub][q % kSub] = r[q];
+  const double* r = rinv + static_cast<int64_t>(slot) * KS * KS;
+  for (int q = tid; q < KS * KS; q += kItemRows) Ri[q / KS][q % KS] = r[q];
   __syncthreads();
   if (tid >= cnt) return;
   const int64_t p = p0 + tid;
-  const float* row = src + p * kSub;
-  double x[kSub];
+  const float* row = src + p * KS;
+  double x[KS];
 #pragma unroll
-  for (int k = 0; k < kSub; ++k) x[k] = row[k];
-  float* out = dst + p * kSub;
+  for (int k = 0; k < KS; ++k) x[k] = row[k];
+  float* out = dst + p * KS;
 #pragma unroll
-  for (int j = 0; j < kSub; ++j) {
+  for (int j = 0; j < KS; ++j) {
     double v = 0.0;
 #pragma unroll
     for (int k = 0; k <= j; ++k) v = fma(x[k], Ri[k][j], v);
@@ -467,8 +482,9 @@ __global__ __launch_bounds__(kItemRows) void rotate_kernel(const int32_t* __rest
   }
 }
 
-// coords = Y V_slot (kSub x dim) in f64, stored f32; per-slot max |coord|
+// coords = Y V_slot (KS x dim) in f64, stored f32; per-slot max |coord|
 // (reduced over the lanes of a wave that share a slot, then one atomic).
+template <int KS>
 __global__ __launch_bounds__(256) void project_kernel(const int32_t* __restrict__ perm, int64_t rows,
                                                       const int32_t* __restrict__ slot_of_pos,
                                                       const float* __restrict__ y,
@@ -482,11 +498,11 @@ __global__ __launch_bounds__(256) void project_kernel(const int32_t* __restrict_
   if (p < rows) {
     const int32_t v = perm[p];
     slot = slot_of_pos[p];
-    const float* row = y + p * kSub;
-    const double* vm = vmat + static_cast<int64_t>(slot) * kSub * dim;
+    const float* row = y + p * KS;
+    const double* vm = vmat + static_cast<int64_t>(slot) * KS * dim;
     for (int c = 0; c < dim; ++c) {
       double s = 0.0;
-      for (int b = 0; b < kSub; ++b) s += static_cast<double>(row[b]) * vm[b * dim + c];
+      for (int b = 0; b < KS; ++b) s += static_cast<double>(row[b]) * vm[b * dim + c];
       const float f = static_cast<float>(s);
       out[static_cast<int64_t>(v) * dim + c] = f;
       mx = fmaxf(mx, fabsf(f));
@@ -717,8 +733,8 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
   maybe_inject_fault("init");
   SpectralReport rep;
   if (n <= dim + 1 || g.n_edges == 0) return false;
-  if ((dim + 1) * 6 != kSub)
-    throw_invalid("spectral init on the GPU supports n_components = 2; use init='random'");
+  if (dim < 1 || dim > kMaxDim)
+    throw_invalid("spectral init on the GPU supports n_components 1 to 8");
   // components, ordered by smallest member, members ascending
   std::vector<int32_t> labels;
   connected_components_device(g, labels, ws, s);
@@ -873,7 +889,10 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
   for (int64_t c = 0; c < nc; ++c)
     if (plan[c] == 2) gpu_comps.push_back(c);
   const int n_slots = static_cast<int>(gpu_comps.size());
-  if (n_slots > 0) {
+  // the iterated components, for the block width KS = (dim + 1) * 6
+  auto iterate = [&](auto ks_tag) -> bool {
+    constexpr int KS = decltype(ks_tag)::value;
+    constexpr int kGroups = SpmmShape<KS>::kGroups;
     std::vector<int32_t> perm, slot_of_pos, local_of_pos, items;
     std::vector<uint64_t> seeds(n_slots), noise_seeds(n_slots);
     size_t total_rows = 0;
@@ -887,7 +906,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
       const int64_t c = gpu_comps[sl];
       const int32_t start = static_cast<int32_t>(at);
       const auto& mem = comps[c].members;
-      if (static_cast<int64_t>(mem.size()) <= kSub) return false;
+      if (static_cast<int64_t>(mem.size()) <= KS) return false;
       std::copy(mem.begin(), mem.end(), perm.begin() + at);
       std::fill(slot_of_pos.begin() + at, slot_of_pos.begin() + at + mem.size(), sl);
       std::iota(local_of_pos.begin() + at, local_of_pos.begin() + at + mem.size(), 0);
@@ -920,9 +939,9 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
 
     ws.deg.ensure(n);
     ws.inv.ensure(n);
-    ws.y.ensure(static_cast<size_t>(rows) * kSub);
-    ws.z.ensure(static_cast<size_t>(rows) * kSub);
-    ws.gram.ensure(static_cast<size_t>(n_slots) * kSub * kSub);
+    ws.y.ensure(static_cast<size_t>(rows) * KS);
+    ws.z.ensure(static_cast<size_t>(rows) * KS);
+    ws.gram.ensure(static_cast<size_t>(n_slots) * KS * KS);
     ws.fail.ensure(n_slots);
     ws.label.ensure(n);  // reused as pos_of
     hipLaunchKernelGGL(degree_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.row_ptr.get(),
@@ -946,10 +965,10 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     hipLaunchKernelGGL(prow_fill_kernel, dim3(blocks_for(g.n_edges, 256)), dim3(256), 0, s,
                        g.head.get(), g.n_edges, g.row_ptr.get(), g.tail.get(), g.weight.get(),
                        ws.inv.get(), ws.label.get(), prow_ptr.get(), pcol.get(), pnw.get());
-    hipLaunchKernelGGL(start_block_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
+    hipLaunchKernelGGL(start_block_kernel<KS>, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm,
                        rows, d_slot, d_local, ws.seeds.get(), ws.deg.get(), ws.y.get());
     KERNEL_CHECK();
-    const int glen = n_slots * kSub * kSub;
+    const int glen = n_slots * KS * KS;
     double* g1 = ws.gram.get();
     HIP_CHECK(hipMemsetAsync(ws.fail.get(), 0, 4 * n_slots, s));
     HIP_CHECK(hipMemsetAsync(ws.gram.get(), 0, sizeof(double) * glen, s));
@@ -960,14 +979,14 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     // Orthonormalise Z into Y (CholeskyQR): g1 holds Z^T Z on entry and is
     // cleared for the next accumulation.
     auto orth_into_y = [&]() {
-      hipLaunchKernelGGL(chol_kernel, dim3(n_slots), dim3(64), 0, s, g1, ws.rmat.get(), ws.fail.get());
-      hipLaunchKernelGGL(rotate_kernel, dim3(n_items), dim3(kItemRows), 0, s, it_items, ws.rmat.get(),
+      hipLaunchKernelGGL(chol_kernel<KS>, dim3(n_slots), dim3(64), 0, s, g1, ws.rmat.get(), ws.fail.get());
+      hipLaunchKernelGGL(rotate_kernel<KS>, dim3(n_items), dim3(kItemRows), 0, s, it_items, ws.rmat.get(),
                          zb, yb);
       KERNEL_CHECK();
     };
     // start block: Z = Y, g1 = Y^T Y
-    HIP_CHECK(hipMemcpyAsync(zb, yb, sizeof(float) * rows * kSub, hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
+    HIP_CHECK(hipMemcpyAsync(zb, yb, sizeof(float) * rows * KS, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL((gram_kernel<false, KS>), dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
     KERNEL_CHECK();
     orth_into_y();
     // rows of more than kSeg entries (one wave each) and the others in order
@@ -988,7 +1007,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     HIP_CHECK(hipStreamSynchronize(s));
     const int64_t spmm_waves = n_long + (n_short + kGroups - 1) / kGroups;
     auto product = [&](const float* src, float* dst) {
-      hipLaunchKernelGGL(spmm_kernel, dim3(blocks_for(spmm_waves, kSpmmWaves)),
+      hipLaunchKernelGGL(spmm_kernel<KS>, dim3(blocks_for(spmm_waves, kSpmmWaves)),
                          dim3(64 * kSpmmWaves), 0, s, static_cast<int64_t>(n_long), n_short,
                          long_list.get(), short_list.get(), prow_ptr.get(), pcol.get(), pnw.get(),
                          src, dst);
@@ -997,7 +1016,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     for (int it = 0; it < kIters; ++it) {
       product(yb, zb);
       if ((it + 1) % kOrthEvery == 0 || it + 1 == kIters) {
-        hipLaunchKernelGGL(gram_kernel<false>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
+        hipLaunchKernelGGL((gram_kernel<false, KS>), dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
         KERNEL_CHECK();
         orth_into_y();
       } else {
@@ -1006,14 +1025,14 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     }
     // Ritz matrix Y^T (I + N) Y, upper triangle
     product(yb, zb);
-    hipLaunchKernelGGL(gram_kernel<true>, dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
+    hipLaunchKernelGGL((gram_kernel<true, KS>), dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
     KERNEL_CHECK();
-    std::vector<double> ritz(static_cast<size_t>(n_slots) * kSub * kSub);
+    std::vector<double> ritz(static_cast<size_t>(n_slots) * KS * KS);
     std::vector<int32_t> fail(n_slots);
     HIP_CHECK(hipMemcpyAsync(ritz.data(), g1, 8 * ritz.size(), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(fail.data(), ws.fail.get(), 4 * n_slots, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    std::vector<double> vmat(static_cast<size_t>(n_slots) * kSub * dim);
+    std::vector<double> vmat(static_cast<size_t>(n_slots) * KS * dim);
     std::vector<float> anchor(static_cast<size_t>(n_slots) * dim);
     std::vector<int> ok_slots;
     for (int sl = 0; sl < n_slots; ++sl) {
@@ -1041,8 +1060,8 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
           for (int k = 0; k < dim; ++k) y_host_rows.push_back(lc[i * dim + k] + layout[c * dim + k]);
         }
         ++rep.n_dense;
-        for (int b = 0; b < kSub; ++b)
-          for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * kSub + b) * dim + k] = 0.0;
+        for (int b = 0; b < KS; ++b)
+          for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * KS + b) * dim + k] = 0.0;
         continue;
       }
       ok_slots.push_back(sl);
@@ -1052,17 +1071,17 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     // is the same whichever thread solves it).
     auto ritz_slot = [&](int sl) {
       const int64_t c = gpu_comps[sl];
-      std::vector<double> a(ritz.begin() + static_cast<size_t>(sl) * kSub * kSub,
-                            ritz.begin() + static_cast<size_t>(sl + 1) * kSub * kSub);
-      for (int i = 0; i < kSub; ++i)
-        for (int j = 0; j < i; ++j) a[i * kSub + j] = a[j * kSub + i];
+      std::vector<double> a(ritz.begin() + static_cast<size_t>(sl) * KS * KS,
+                            ritz.begin() + static_cast<size_t>(sl + 1) * KS * KS);
+      for (int i = 0; i < KS; ++i)
+        for (int j = 0; j < i; ++j) a[i * KS + j] = a[j * KS + i];
       std::vector<double> ev, vec;
-      symmetric_eigen(a, kSub, ev, vec);
-      std::vector<int> order(kSub);
+      symmetric_eigen(a, KS, ev, vec);
+      std::vector<int> order(KS);
       std::iota(order.begin(), order.end(), 0);
       std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return ev[q] < ev[p]; });
-      for (int b = 0; b < kSub; ++b)
-        for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * kSub + b) * dim + k] = vec[b * kSub + order[k + 1]];
+      for (int b = 0; b < KS; ++b)
+        for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * KS + b) * dim + k] = vec[b * KS + order[k + 1]];
       for (int k = 0; k < dim; ++k) anchor[sl * dim + k] = layout[c * dim + k];
     };
     const int n_ok = static_cast<int>(ok_slots.size());
@@ -1098,7 +1117,7 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     HIP_CHECK(hipMemcpyAsync(ws.vmat.get(), vmat.data(), 8 * vmat.size(), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(ws.anchor.get(), anchor.data(), 4 * anchor.size(), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemsetAsync(ws.maxabs.get(), 0, 4 * n_slots, s));
-    hipLaunchKernelGGL(project_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
+    hipLaunchKernelGGL(project_kernel<KS>, dim3(blocks_for(rows, 256)), dim3(256), 0, s, d_perm, rows,
                        d_slot, yb, ws.vmat.get(), dim, coords.get(), ws.maxabs.get());
     hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(n_slots, 64)), dim3(64), 0, s,
                        ws.maxabs.get(), ws.anchor.get() + static_cast<size_t>(n_slots) * dim, n_slots,
@@ -1109,6 +1128,22 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
                        coords.get(), y);
     KERNEL_CHECK();
     HIP_CHECK(hipStreamSynchronize(s));
+    return true;
+  };
+  if (n_slots > 0) {
+    bool ok = false;
+    switch (dim) {
+      case 1: ok = iterate(std::integral_constant<int, 12>{}); break;
+      case 2: ok = iterate(std::integral_constant<int, 18>{}); break;
+      case 3: ok = iterate(std::integral_constant<int, 24>{}); break;
+      case 4: ok = iterate(std::integral_constant<int, 30>{}); break;
+      case 5: ok = iterate(std::integral_constant<int, 36>{}); break;
+      case 6: ok = iterate(std::integral_constant<int, 42>{}); break;
+      case 7: ok = iterate(std::integral_constant<int, 48>{}); break;
+      case 8: ok = iterate(std::integral_constant<int, 54>{}); break;
+      default: throw_invalid("spectral init on the GPU supports n_components 1 to 8");
+    }
+    if (!ok) return false;
   }
   if (!y_host_index.empty()) {
     DevBuf<int32_t> ridx(y_host_index.size());
