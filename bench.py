@@ -108,6 +108,124 @@ def knn_accuracy(lib, L, x, k, precision, metric, rows):
     return recall, float(rel[same].max()) if np.any(same) else float("nan")
 
 
+def leg_learned(lib, L, dev, m_q: int, k: int) -> dict:
+    """The learned-transform configuration: 100000 x 192 fit with encoder
+    training, then the encoder over m_q fresh device-resident rows."""
+    import torch
+
+    from umapers_amd import datasets
+    from umapers_amd.diagnostics import trustworthiness
+
+    log(f"bench: learned transform leg (100000 x 192 fit, {m_q} queries)")
+    x5, q5 = datasets.c5(n_fit=100_000, n_query=m_q)
+    p5 = L.default_params()
+    p5.use_approximate_knn = 0
+    p5.n_neighbors = k
+    p5.transform_mode = L.TRANSFORM_LEARNED
+    p5.random_seed = 42
+    h5 = ctypes.c_void_p()
+    L.check(lib.umapx_create(ctypes.byref(p5), ctypes.byref(h5)))
+    t0 = time.perf_counter()
+    y5 = np.empty((x5.shape[0], 2), np.float32)
+    L.check(lib.umapx_fit_transform(h5, L.ptr(x5), x5.shape[0], x5.shape[1], L.ptr(y5)))
+    fit5 = time.perf_counter() - t0
+    first, last, steps = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+    L.check(lib.umapx_get_mlp_loss(h5, ctypes.byref(first), ctypes.byref(last), ctypes.byref(steps)))
+    q5_d = torch.from_numpy(q5).to(dev)
+    yq5_d = torch.empty((m_q, 2), dtype=torch.float32, device=dev)
+    args_l = (h5, ctypes.c_void_p(q5_d.data_ptr()), m_q, q5.shape[1], ctypes.c_void_p(yq5_d.data_ptr()))
+    L.check(lib.umapx_transform_learned_device(*args_l))
+    torch.cuda.synchronize()
+    tl = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        L.check(lib.umapx_transform_learned_device(*args_l))
+        torch.cuda.synchronize()
+        tl.append(time.perf_counter() - t0)
+    mlp_sec = float(np.median(tl))
+    check_rows = np.ascontiguousarray(q5[:10000])
+    y_mlp = yq5_d[:10000].cpu().numpy()
+    y_plain = np.empty((check_rows.shape[0], 2), np.float32)
+    t0 = time.perf_counter()
+    L.check(lib.umapx_transform_plain(h5, L.ptr(check_rows), check_rows.shape[0], check_rows.shape[1], L.ptr(y_plain)))
+    plain_sec = time.perf_counter() - t0
+    per_row = np.linalg.norm(y_mlp - y_plain, axis=1)
+    extent = float(np.ptp(y5, axis=0).max())
+    flops_row = 2.0 * (192 * 256 + 256 * 256 + 256 * 2)
+    mlp = {"workload": "c5: 100000x192 fit (20 clusters, seed 31), encoder 192-256-256-2 "
+                       f"trained {steps.value} steps x 4096 edges x 5 negatives; {m_q} fresh "
+                       "queries (seed 32), device resident; plain transform of the first 10000 as the check",
+           "fit_sec_incl_training": round(fit5, 3),
+           "train_loss_first_last": [round(first.value, 4), round(last.value, 4)],
+           "learned_rows_per_sec": round(m_q / mlp_sec, 1),
+           "learned_tflops": round(flops_row * m_q / mlp_sec / 1e12, 2),
+           "plain_check_rows_per_sec": round(check_rows.shape[0] / plain_sec, 1),
+           "per_row_dist_median": round(float(np.median(per_row)), 4),
+           "per_row_dist_p90": round(float(np.percentile(per_row, 90)), 4),
+           "embedding_extent": round(extent, 3),
+           "trust15_learned": round(trustworthiness(check_rows, y_mlp, k), 5),
+           "trust15_plain": round(trustworthiness(check_rows, y_plain, k), 5)}
+    log(f"bench: learned transform {m_q / mlp_sec:.0f} rows/s vs plain {check_rows.shape[0] / plain_sec:.0f} rows/s")
+    lib.umapx_destroy(h5)
+    del q5_d, yq5_d
+    return mlp
+
+
+def leg_k30(lib, L, dev, n4: int, epochs: int) -> dict:
+    """The SGD-dominated k = 30 configuration: n4 x 64 anisotropic mixture."""
+    import torch
+
+    from umapers_amd import datasets
+    from umapers_amd.diagnostics import trustworthiness
+
+    log(f"bench: k=30 leg ({n4} x 64 anisotropic mixture, BF16 + re-rank)")
+    x4 = datasets.c4(n=n4)
+    x4_d = torch.from_numpy(x4).to(dev)
+    y4_d = torch.empty((n4, 2), dtype=torch.float32, device=dev)
+    p4 = L.default_params()
+    p4.use_approximate_knn = 0
+    p4.n_neighbors = 30
+    p4.n_epochs = epochs
+    p4.knn_precision = L.KNN_BF16_RERANK
+    p4.random_seed = 42
+    h4 = ctypes.c_void_p()
+    L.check(lib.umapx_create(ctypes.byref(p4), ctypes.byref(h4)))
+    t4 = []
+    for i in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.check(lib.umapx_fit_transform_device(h4, ctypes.c_void_p(x4_d.data_ptr()), n4, 64,
+                                               ctypes.c_void_p(y4_d.data_ptr())))
+        torch.cuda.synchronize()
+        if i > 0:
+            t4.append(time.perf_counter() - t0)
+    tm4 = L.Timings()
+    L.check(lib.umapx_get_timings(h4, ctypes.byref(tm4)))
+    tm4 = tm4.as_dict()
+    y4 = y4_d.cpu().numpy()
+    p4t = L.Timings()  # exact schedule totals (untimed profile run, host buffers)
+    L.check(lib.umapx_profile_fit_transform(h4, L.ptr(x4), n4, 64, L.ptr(y4.copy()),
+                                            ctypes.byref(p4t)))
+    p4t = p4t.as_dict()
+    e4 = int(p4t["n_edges"])
+    b4 = 4.0 * e4 * epochs + 48.0 * p4t["sgd_positive_updates"] + 8.0 * p4t["sgd_negative_samples"]
+    gbs4 = b4 / (tm4["sgd_kernel_ms"] * 1e-3) / 1e9
+    c4 = {"workload": f"c4 anisotropic mixture {n4}x64, 50 clusters, seed 23; k=30, "
+                      f"{epochs} epochs, BF16 candidates + exact FP32 re-rank",
+          "fit_sec": round(float(np.median(t4)), 4),
+          "stage_sec": {key: round(tm4[key], 4) for key in ("knn_sec", "init_sec", "optimize_sec")},
+          "n_edges": e4, "knn_kernel_tflops": round(2.0 * n4 * n4 * 64 / (tm4["knn_kernel_ms"] * 1e-3) / 1e12, 1),
+          "sgd_ms_per_epoch": round(tm4["sgd_kernel_ms"] / epochs, 4),
+          "sgd_bytes_per_epoch": round(b4 / epochs),
+          "sgd_gbs_per_epoch": round(gbs4, 1), "sgd_hbm_frac": round(gbs4 / HBM_PEAK_GBS, 4),
+          "trustworthiness_15": round(trustworthiness(x4, y4, 15), 5)}
+    log(f"bench: c4 fit {c4['fit_sec']} s, sgd {c4['sgd_ms_per_epoch']} ms/epoch "
+        f"({gbs4:.0f} GB/s)")
+    lib.umapx_destroy(h4)
+    del x4_d, y4_d, x4
+    return c4
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +243,9 @@ def main() -> int:
                     help="1: the cosine variant of the workload (rows L2-normalised), both precisions")
     ap.add_argument("--c4-rows", type=int, default=1_000_000,
                     help="rows of the k=30 SGD-dominated leg (1M x 64 mixture; 0 skips it)")
+    ap.add_argument("--only", choices=("k30", "learned"), default=None,
+                    help="run only that leg and print its figures as one JSON line (for "
+                         "profiling one configuration)")
     ap.add_argument("--cpu-full", action="store_true",
                     help="time the CPU baseline on the full workload once (minutes of CPU "
                          "work) instead of the bounded sample")
@@ -146,6 +267,12 @@ def main() -> int:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
     n, k, epochs = args.n, 15, 200
+    if args.only:
+        dev = torch.device("cuda:0")
+        res = (leg_k30(lib, L, dev, args.c4_rows, epochs) if args.only == "k30"
+               else leg_learned(lib, L, dev, args.mlp_queries, k))
+        print(json.dumps(res), flush=True)
+        return 0
     log(f"bench: generating {n} x 128 rows")
     x = datasets.c3(n=n)
     d = x.shape[1]
@@ -351,112 +478,8 @@ def main() -> int:
             f"fit {tw['total_sec'] * 1e3:.1f} ms")
         del xw
 
-    # ---- learned transform: 100000 x 192 fit, encoder over fresh queries ----
-    mlp = None
-    if args.mlp_queries > 0:
-        m_q = args.mlp_queries
-        log(f"bench: learned transform leg (100000 x 192 fit, {m_q} queries)")
-        x5, q5 = datasets.c5(n_fit=100_000, n_query=m_q)
-        p5 = L.default_params()
-        p5.use_approximate_knn = 0
-        p5.n_neighbors = k
-        p5.transform_mode = L.TRANSFORM_LEARNED
-        p5.random_seed = 42
-        h5 = ctypes.c_void_p()
-        L.check(lib.umapx_create(ctypes.byref(p5), ctypes.byref(h5)))
-        t0 = time.perf_counter()
-        y5 = np.empty((x5.shape[0], 2), np.float32)
-        L.check(lib.umapx_fit_transform(h5, L.ptr(x5), x5.shape[0], x5.shape[1], L.ptr(y5)))
-        fit5 = time.perf_counter() - t0
-        first, last, steps = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
-        L.check(lib.umapx_get_mlp_loss(h5, ctypes.byref(first), ctypes.byref(last), ctypes.byref(steps)))
-        q5_d = torch.from_numpy(q5).to(dev)
-        yq5_d = torch.empty((m_q, 2), dtype=torch.float32, device=dev)
-        args_l = (h5, ctypes.c_void_p(q5_d.data_ptr()), m_q, q5.shape[1], ctypes.c_void_p(yq5_d.data_ptr()))
-        L.check(lib.umapx_transform_learned_device(*args_l))
-        torch.cuda.synchronize()
-        tl = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            L.check(lib.umapx_transform_learned_device(*args_l))
-            torch.cuda.synchronize()
-            tl.append(time.perf_counter() - t0)
-        mlp_sec = float(np.median(tl))
-        check_rows = np.ascontiguousarray(q5[:10000])
-        y_mlp = yq5_d[:10000].cpu().numpy()
-        y_plain = np.empty((check_rows.shape[0], 2), np.float32)
-        t0 = time.perf_counter()
-        L.check(lib.umapx_transform_plain(h5, L.ptr(check_rows), check_rows.shape[0], check_rows.shape[1], L.ptr(y_plain)))
-        plain_sec = time.perf_counter() - t0
-        per_row = np.linalg.norm(y_mlp - y_plain, axis=1)
-        extent = float(np.ptp(y5, axis=0).max())
-        flops_row = 2.0 * (192 * 256 + 256 * 256 + 256 * 2)
-        mlp = {"workload": "c5: 100000x192 fit (20 clusters, seed 31), encoder 192-256-256-2 "
-                           f"trained {steps.value} steps x 4096 edges x 5 negatives; {m_q} fresh "
-                           "queries (seed 32), device resident; plain transform of the first 10000 as the check",
-               "fit_sec_incl_training": round(fit5, 3),
-               "train_loss_first_last": [round(first.value, 4), round(last.value, 4)],
-               "learned_rows_per_sec": round(m_q / mlp_sec, 1),
-               "learned_tflops": round(flops_row * m_q / mlp_sec / 1e12, 2),
-               "plain_check_rows_per_sec": round(check_rows.shape[0] / plain_sec, 1),
-               "per_row_dist_median": round(float(np.median(per_row)), 4),
-               "per_row_dist_p90": round(float(np.percentile(per_row, 90)), 4),
-               "embedding_extent": round(extent, 3),
-               "trust15_learned": round(trustworthiness(check_rows, y_mlp, k), 5),
-               "trust15_plain": round(trustworthiness(check_rows, y_plain, k), 5)}
-        log(f"bench: learned transform {m_q / mlp_sec:.0f} rows/s vs plain {check_rows.shape[0] / plain_sec:.0f} rows/s")
-        lib.umapx_destroy(h5)
-        del q5_d, yq5_d
-
-    # ---- SGD-dominated large graph: 1M x 64, k = 30 ---------------------
-    c4 = None
-    if args.c4_rows > 0:
-        n4 = args.c4_rows
-        log(f"bench: k=30 leg ({n4} x 64 anisotropic mixture, BF16 + re-rank)")
-        x4 = datasets.c4(n=n4)
-        x4_d = torch.from_numpy(x4).to(dev)
-        y4_d = torch.empty((n4, 2), dtype=torch.float32, device=dev)
-        p4 = L.default_params()
-        p4.use_approximate_knn = 0
-        p4.n_neighbors = 30
-        p4.n_epochs = epochs
-        p4.knn_precision = L.KNN_BF16_RERANK
-        p4.random_seed = 42
-        h4 = ctypes.c_void_p()
-        L.check(lib.umapx_create(ctypes.byref(p4), ctypes.byref(h4)))
-        t4 = []
-        for i in range(2):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            L.check(lib.umapx_fit_transform_device(h4, ctypes.c_void_p(x4_d.data_ptr()), n4, 64,
-                                                   ctypes.c_void_p(y4_d.data_ptr())))
-            torch.cuda.synchronize()
-            if i > 0:
-                t4.append(time.perf_counter() - t0)
-        tm4 = L.Timings()
-        L.check(lib.umapx_get_timings(h4, ctypes.byref(tm4)))
-        tm4 = tm4.as_dict()
-        y4 = y4_d.cpu().numpy()
-        p4t = L.Timings()  # exact schedule totals (untimed profile run, host buffers)
-        L.check(lib.umapx_profile_fit_transform(h4, L.ptr(x4), n4, 64, L.ptr(y4.copy()),
-                                                ctypes.byref(p4t)))
-        p4t = p4t.as_dict()
-        e4 = int(p4t["n_edges"])
-        b4 = 4.0 * e4 * epochs + 48.0 * p4t["sgd_positive_updates"] + 8.0 * p4t["sgd_negative_samples"]
-        gbs4 = b4 / (tm4["sgd_kernel_ms"] * 1e-3) / 1e9
-        c4 = {"workload": f"c4 anisotropic mixture {n4}x64, 50 clusters, seed 23; k=30, "
-                          f"{epochs} epochs, BF16 candidates + exact FP32 re-rank",
-              "fit_sec": round(float(np.median(t4)), 4),
-              "stage_sec": {key: round(tm4[key], 4) for key in ("knn_sec", "init_sec", "optimize_sec")},
-              "n_edges": e4, "knn_kernel_tflops": round(2.0 * n4 * n4 * 64 / (tm4["knn_kernel_ms"] * 1e-3) / 1e12, 1),
-              "sgd_ms_per_epoch": round(tm4["sgd_kernel_ms"] / epochs, 4),
-              "sgd_bytes_per_epoch": round(b4 / epochs),
-              "sgd_gbs_per_epoch": round(gbs4, 1), "sgd_hbm_frac": round(gbs4 / HBM_PEAK_GBS, 4),
-              "trustworthiness_15": round(trustworthiness(x4, y4, 15), 5)}
-        log(f"bench: c4 fit {c4['fit_sec']} s, sgd {c4['sgd_ms_per_epoch']} ms/epoch "
-            f"({gbs4:.0f} GB/s)")
-        lib.umapx_destroy(h4)
-        del x4_d, y4_d, x4
+    mlp = leg_learned(lib, L, dev, args.mlp_queries, k) if args.mlp_queries > 0 else None
+    c4 = leg_k30(lib, L, dev, args.c4_rows, epochs) if args.c4_rows > 0 else None
 
     # ---- CPU baseline: the C++ restatement of the reference's CPU path -----
     cpu = None
