@@ -568,6 +568,8 @@ def main() -> int:
         "fp32_fit_sec": round(fp32_sec, 4),
         "speedup_bf16_vs_fp32": round(fp32_sec / fit_sec, 3),
         "speedup_vs_cpu": round(cpu["value"] / fit_sec, 1) if cpu else None,
+        "speedup_vs_cpu_full_run": (round(cpu["full_run"]["value"] / fit_sec, 1)
+                                    if cpu and "full_run" in cpu else None),
         "recall_at_15": round(recall, 5),
         "max_rel_dist_err": max_err,
         "trustworthiness_15": round(trust, 5),
