@@ -153,6 +153,26 @@ int32_t umapx_fit_transform_with_knn(umapx_handle* h, const float* x, int64_t n,
 /* profile_fit_transform_dense (lib.rs:796-962). */
 int32_t umapx_profile_fit_transform(umapx_handle* h, const float* x, int64_t n, int64_t d,
                                     float* y_out, umapx_timings* t);
+/* fit_transform_sparse_csr (lib.rs:1015-1036; fit_sparse_csr :998-1013 when
+ * y_out is NULL) on CSR rows: indptr [n_rows + 1], column indices strictly
+ * increasing within a row, f32 data, nnz = indptr[n_rows].  The CSR checks
+ * of SparseCsrMatrix::new (rust_umap/src/sparse.rs:20-90) come first, then
+ * validate_sparse_metric / validate_sparse_density_params (lib.rs:2241-2258):
+ * canberra / braycurtis, densMAP and output_dens are dense-only.  The kNN is
+ * the reference's sparse exact kNN (sparse.rs:605-626, its arithmetic bit
+ * for bit except minkowski); n_neighbors <= 64. */
+int32_t umapx_fit_transform_csr(umapx_handle* h, int64_t n_rows, int64_t n_cols,
+                                const int64_t* indptr, const int64_t* indices, const float* data,
+                                int64_t nnz, float* y_out);
+/* transform_sparse_csr (lib.rs:1687-1737): CSR queries on a sparse fit.
+ * umapx_transform on a sparse fit takes dense queries through the
+ * reference's dense-query sparse kNN (sparse.rs:753-900, lib.rs:1655-1672). */
+int32_t umapx_transform_csr(umapx_handle* h, int64_t m, int64_t n_cols, const int64_t* indptr,
+                            const int64_t* indices, const float* data, int64_t nnz, float* out);
+/* 1 when the handle was fitted on CSR rows (no counterpart in the reference,
+ * whose model keeps training_data_sparse, lib.rs:1571-1590). */
+int32_t umapx_is_sparse_fit(const umapx_handle* h, int32_t* sparse);
+
 /* Timings of the last fit on this handle. */
 int32_t umapx_get_timings(const umapx_handle* h, umapx_timings* t);
 
@@ -223,6 +243,25 @@ int32_t umapx_knn_to_reference_device_noalloc(const float* d_q, int64_t m, const
 int32_t umapx_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_t metric,
                          float metric_p, int32_t candidate_pool, int32_t n_iters, uint64_t seed,
                          int64_t* idx, float* dist);
+/* Sparse exact kNN stages (rust_umap/src/sparse.rs) on host CSR buffers;
+ * metric codes 0-5 (canberra / braycurtis are dense-only), k <= 64:
+ * umapx_knn_csr: exact_nearest_neighbors (:605-626), the fit's kNN, every
+ *   row at distance 0.0 from itself;
+ * umapx_knn_csr_query: exact_nearest_neighbors_sparse_query (:628-657);
+ * umapx_knn_dense_to_csr: exact_nearest_neighbors_dense_query (:753-900),
+ *   dense q [m x n_cols] against CSR rows. */
+int32_t umapx_knn_csr(int64_t n, int64_t n_cols, const int64_t* indptr, const int64_t* indices,
+                      const float* data, int64_t nnz, int32_t k, int32_t metric, float metric_p,
+                      int64_t* idx, float* dist);
+int32_t umapx_knn_csr_query(int64_t m, const int64_t* q_indptr, const int64_t* q_indices,
+                            const float* q_data, int64_t q_nnz, int64_t n,
+                            const int64_t* r_indptr, const int64_t* r_indices,
+                            const float* r_data, int64_t r_nnz, int64_t n_cols, int32_t k,
+                            int32_t metric, float metric_p, int64_t* idx, float* dist);
+int32_t umapx_knn_dense_to_csr(const float* q, int64_t m, int64_t n, const int64_t* r_indptr,
+                               const int64_t* r_indices, const float* r_data, int64_t r_nnz,
+                               int64_t n_cols, int32_t k, int32_t metric, float metric_p,
+                               int64_t* idx, float* dist);
 /* exact_nearest_neighbors_by / _to_reference_by (lib.rs:2891-2928, 3346-3399)
  * under any UMAPX_METRIC_* code with minkowski's p (self included when q == r
  * and m == n). */

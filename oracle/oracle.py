@@ -144,6 +144,27 @@ def lib() -> ctypes.CDLL:
                                             ctypes.c_float, _f32p, ctypes.c_int64, ctypes.c_int32,
                                             _f32p]
         L.orc_inverse_transform.restype = ctypes.c_int32
+        L.orc_transform_with_knn.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_int64, _f32p,
+                                             ctypes.c_float, ctypes.c_float, ctypes.c_int64,
+                                             _i64p, _f32p, ctypes.c_int32, _f32p]
+        L.orc_transform_with_knn.restype = ctypes.c_int32
+        L.orc_knn_csr.argtypes = [ctypes.c_int64, ctypes.c_int64, _i64p, _i64p, _f32p,
+                                  ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_int32,
+                                  _i64p, _f32p]
+        L.orc_knn_csr.restype = None
+        L.orc_knn_csr_query.argtypes = [ctypes.c_int64, _i64p, _i64p, _f32p, ctypes.c_int64,
+                                        _i64p, _i64p, _f32p, ctypes.c_int64, ctypes.c_int32,
+                                        ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _i64p,
+                                        _f32p]
+        L.orc_knn_csr_query.restype = None
+        L.orc_knn_dense_to_csr.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
+                                           _f32p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_float, ctypes.c_int32, _i64p, _f32p]
+        L.orc_knn_dense_to_csr.restype = None
+        L.orc_csr_check.argtypes = [ctypes.c_int64, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p,
+                                    ctypes.c_int64, _f32p, ctypes.c_int64, ctypes.c_char_p,
+                                    ctypes.c_int64]
+        L.orc_csr_check.restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -445,6 +466,88 @@ def transform(x_train, y_train, q, params: OracleParams | None = None):
     rc = lib().orc_transform(ctypes.byref(cp), _p(x_train, _f32p), x_train.shape[0],
                              x_train.shape[1], _p(y_train, _f32p), a, b, _p(q, _f32p),
                              q.shape[0], _p(out, _f32p))
+    assert rc == 0
+    return out
+
+
+class Csr:
+    """CSR rows as the reference's SparseCsrMatrix holds them: int64 indptr
+    [rows + 1] and column indices, f32 data (from a scipy matrix or arrays)."""
+
+    def __init__(self, indptr, indices, data, n_cols: int):
+        self.indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+        self.indices = np.ascontiguousarray(indices, dtype=np.int64)
+        self.data = np.ascontiguousarray(data, dtype=np.float32)
+        self.n_rows = self.indptr.shape[0] - 1
+        self.n_cols = int(n_cols)
+
+    @classmethod
+    def of(cls, m) -> "Csr":
+        m = m.tocsr()
+        return cls(m.indptr, m.indices, m.data, m.shape[1])
+
+    def ptrs(self):
+        return _p(self.indptr, _i64p), _p(self.indices, _i64p), _p(self.data, _f32p)
+
+
+def csr_check(c: Csr) -> str | None:
+    """SparseCsrMatrix::new's checks (sparse.rs:20-90): None when valid, else
+    the reference's message."""
+    buf = ctypes.create_string_buffer(512)
+    rc = lib().orc_csr_check(c.n_rows, c.n_cols, _p(c.indptr, _i64p), c.indptr.shape[0],
+                             _p(c.indices, _i64p), c.indices.shape[0], _p(c.data, _f32p),
+                             c.data.shape[0], buf, 512)
+    return None if rc == 0 else buf.value.decode()
+
+
+def knn_csr(c, k: int, metric: str = "euclidean", p: float = 2.0, n_threads: int = 0):
+    """exact_nearest_neighbors (sparse.rs:605-626): the fit's sparse kNN,
+    every row itself at distance 0.0."""
+    c = c if isinstance(c, Csr) else Csr.of(c)
+    idx = np.empty((c.n_rows, k), np.int64)
+    dist = np.empty((c.n_rows, k), np.float32)
+    lib().orc_knn_csr(c.n_rows, c.n_cols, *c.ptrs(), k, METRICS[metric], p, n_threads,
+                      _p(idx, _i64p), _p(dist, _f32p))
+    return idx, dist
+
+
+def knn_csr_query(q, r, k: int, metric: str = "euclidean", p: float = 2.0, n_threads: int = 0):
+    """exact_nearest_neighbors_sparse_query (sparse.rs:628-657)."""
+    q = q if isinstance(q, Csr) else Csr.of(q)
+    r = r if isinstance(r, Csr) else Csr.of(r)
+    idx = np.empty((q.n_rows, k), np.int64)
+    dist = np.empty((q.n_rows, k), np.float32)
+    lib().orc_knn_csr_query(q.n_rows, *q.ptrs(), r.n_rows, *r.ptrs(), r.n_cols, k,
+                            METRICS[metric], p, n_threads, _p(idx, _i64p), _p(dist, _f32p))
+    return idx, dist
+
+
+def knn_dense_to_csr(q, r, k: int, metric: str = "euclidean", p: float = 2.0,
+                     n_threads: int = 0):
+    """exact_nearest_neighbors_dense_query (sparse.rs:753-900): dense rows
+    against CSR rows."""
+    q = _f32(q)
+    r = r if isinstance(r, Csr) else Csr.of(r)
+    idx = np.empty((q.shape[0], k), np.int64)
+    dist = np.empty((q.shape[0], k), np.float32)
+    lib().orc_knn_dense_to_csr(_p(q, _f32p), q.shape[0], r.n_rows, *r.ptrs(), r.n_cols, k,
+                               METRICS[metric], p, n_threads, _p(idx, _i64p), _p(dist, _f32p))
+    return idx, dist
+
+
+def transform_with_knn(y_train, idx, dist, params: OracleParams | None = None):
+    """transform_from_knn (lib.rs:1738-1780) on a query kNN among the fitted
+    rows (how the sparse transforms finish, lib.rs:1655-1737)."""
+    params = params or OracleParams()
+    y_train = _f32(y_train)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    dist = _f32(dist)
+    a, b = find_ab_params(params.spread, params.min_dist)
+    out = np.empty((idx.shape[0], params.n_components), np.float32)
+    cp = params.c()
+    rc = lib().orc_transform_with_knn(ctypes.byref(cp), y_train.shape[0], _p(y_train, _f32p), a,
+                                      b, idx.shape[0], _p(idx, _i64p), _p(dist, _f32p),
+                                      idx.shape[1], _p(out, _f32p))
     assert rc == 0
     return out
 

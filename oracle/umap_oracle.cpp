@@ -2024,15 +2024,25 @@ int32_t orc_transform(const orc_params* p, const float* x_train, int64_t n, int6
   if (m == 0) return 0;
   const int nt = p->n_threads > 0 ? p->n_threads : omp_get_max_threads();
   const int32_t k = std::min<int64_t>(p->n_neighbors, n);
-  const int32_t n_epochs = p->n_epochs > 0 ? p->n_epochs : (m <= 10000 ? 100 : 30);
   vector<int64_t> idx(static_cast<size_t>(m) * k);
   vector<float> dist(static_cast<size_t>(m) * k);
   knn_impl(q, m, x_train, n, d, k, p->metric, nt, idx.data(), dist.data(), p->metric_p);
+  return orc_transform_with_knn(p, n, y_train, a, b, m, idx.data(), dist.data(), k, out);
+}
+
+/* transform_from_knn (lib.rs:1738-1780) on a query kNN [m x k] among the n
+ * fitted rows (the sparse transforms, lib.rs:1655-1737, reach it this way). */
+int32_t orc_transform_with_knn(const orc_params* p, int64_t n, const float* y_train, float a,
+                               float b, int64_t m, const int64_t* idx, const float* dist,
+                               int32_t k, float* out) {
+  if (m == 0) return 0;
+  const int nt = p->n_threads > 0 ? p->n_threads : omp_get_max_threads();
+  const int32_t n_epochs = p->n_epochs > 0 ? p->n_epochs : (m <= 10000 ? 100 : 30);
   vector<float> sig(m), rho(m);
   const float lc = std::max(p->local_connectivity - 1.0f, 0.0f);
-  smooth_knn(dist.data(), m, k, static_cast<float>(k), lc, DEFAULT_BANDWIDTH, false,
-             sig.data(), rho.data());
-  auto edges = membership_bipartite(idx.data(), dist.data(), sig.data(), rho.data(), m, k);
+  smooth_knn(dist, m, k, static_cast<float>(k), lc, DEFAULT_BANDWIDTH, false, sig.data(),
+             rho.data());
+  auto edges = membership_bipartite(idx, dist, sig.data(), rho.data(), m, k);
   prune(edges, n_epochs);
   const int dim = p->n_components;
   init_graph_transform(m, dim, edges, y_train, out);

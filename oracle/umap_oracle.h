@@ -137,6 +137,34 @@ int32_t orc_fit(const orc_params* p, const float* x, int64_t n, int64_t d,
 int32_t orc_transform(const orc_params* p, const float* x_train, int64_t n,
                       int64_t d, const float* y_train, float a, float b,
                       const float* q, int64_t m, float* out);
+/* transform_from_knn (lib.rs:1738-1780) on a query kNN [m x k] among the n
+ * fitted rows, e.g. from the sparse kNN below. */
+int32_t orc_transform_with_knn(const orc_params* p, int64_t n, const float* y_train, float a,
+                               float b, int64_t m, const int64_t* idx, const float* dist,
+                               int32_t k, float* out);
+/* Sparse (CSR) exact kNN, sparse_oracle.cpp (rust_umap/src/sparse.rs):
+ * indptr [rows + 1], strictly increasing column indices per row, f32 data.
+ * orc_knn_csr: every row against every row, the row itself at 0.0 (:605-626);
+ * orc_knn_csr_query: CSR queries against CSR rows (:628-657);
+ * orc_knn_dense_to_csr: dense queries [m x n_cols] against CSR rows
+ * (:753-900).  Metric codes as above (canberra / braycurtis are dense-only). */
+void orc_knn_csr(int64_t n, int64_t n_cols, const int64_t* indptr, const int64_t* indices,
+                 const float* data, int32_t k, int32_t metric, float p, int32_t n_threads,
+                 int64_t* out_idx, float* out_dist);
+void orc_knn_csr_query(int64_t m, const int64_t* q_indptr, const int64_t* q_indices,
+                       const float* q_data, int64_t n, const int64_t* r_indptr,
+                       const int64_t* r_indices, const float* r_data, int64_t n_cols, int32_t k,
+                       int32_t metric, float p, int32_t n_threads, int64_t* out_idx,
+                       float* out_dist);
+void orc_knn_dense_to_csr(const float* q, int64_t m, int64_t n, const int64_t* r_indptr,
+                          const int64_t* r_indices, const float* r_data, int64_t n_cols,
+                          int32_t k, int32_t metric, float p, int32_t n_threads,
+                          int64_t* out_idx, float* out_dist);
+/* SparseCsrMatrix::new's checks (sparse.rs:20-90): 0 if valid, else 1 with
+ * the reference's message in msg. */
+int32_t orc_csr_check(int64_t n_rows, int64_t n_cols, const int64_t* indptr, int64_t indptr_len,
+                      const int64_t* indices, int64_t n_indices, const float* data,
+                      int64_t n_data, char* msg, int64_t cap);
 /* Inverse transform (lib.rs:1800-1915): embedded rows [m x n_components]
  * back to the feature space [m x d]; optimize = 0 stops after the local
  * linear initialisation (init_inverse_points, lib.rs:4838-4935). */

@@ -173,3 +173,44 @@ def knn_approx(x, k, metric="euclidean", candidate_pool=50, n_iters=14,
                                      n_iters, seed & 0xFFFFFFFFFFFFFFFF, L.ptr(idx, L._i64p),
                                      L.ptr(dist)))
     return idx, dist
+
+
+def _csr_parts(m):
+    m = m.tocsr()
+    m.sort_indices()
+    return (np.ascontiguousarray(m.indptr, np.int64), np.ascontiguousarray(m.indices, np.int64),
+            np.ascontiguousarray(m.data, np.float32), m.shape)
+
+
+def knn_csr(x, k, metric=0, p=2.0):
+    """umapx_knn_csr: the fit's sparse kNN (row i at 0.0 from itself)."""
+    ip, ix, dt, (n, c) = _csr_parts(x)
+    idx = np.empty((n, k), np.int64)
+    dist = np.empty((n, k), np.float32)
+    L.check(L.lib().umapx_knn_csr(n, c, L.ptr(ip, L._i64p), L.ptr(ix, L._i64p), L.ptr(dt),
+                                  dt.shape[0], k, metric, p, L.ptr(idx, L._i64p), L.ptr(dist)))
+    return idx, dist
+
+
+def knn_csr_query(q, r, k, metric=0, p=2.0):
+    qi, qx, qd, (m, c) = _csr_parts(q)
+    ri, rx, rd, (n, _) = _csr_parts(r)
+    idx = np.empty((m, k), np.int64)
+    dist = np.empty((m, k), np.float32)
+    L.check(L.lib().umapx_knn_csr_query(m, L.ptr(qi, L._i64p), L.ptr(qx, L._i64p), L.ptr(qd),
+                                        qd.shape[0], n, L.ptr(ri, L._i64p), L.ptr(rx, L._i64p),
+                                        L.ptr(rd), rd.shape[0], c, k, metric, p,
+                                        L.ptr(idx, L._i64p), L.ptr(dist)))
+    return idx, dist
+
+
+def knn_dense_to_csr(q, r, k, metric=0, p=2.0):
+    q = _f32(q)
+    ri, rx, rd, (n, c) = _csr_parts(r)
+    m = q.shape[0]
+    idx = np.empty((m, k), np.int64)
+    dist = np.empty((m, k), np.float32)
+    L.check(L.lib().umapx_knn_dense_to_csr(L.ptr(q), m, n, L.ptr(ri, L._i64p), L.ptr(rx, L._i64p),
+                                           L.ptr(rd), rd.shape[0], c, k, metric, p,
+                                           L.ptr(idx, L._i64p), L.ptr(dist)))
+    return idx, dist

@@ -38,6 +38,20 @@ class FakeCore:
         self.fit_transform_into(arr, out)
         return out
 
+    def fit_transform_csr_into(self, c, out):
+        # the oracle's sparse kNN, then the fit from that kNN
+        self.calls.append("fit_transform_csr")
+        ref = self.O.Csr(c.indptr, c.indices, c.data, c.shape[1])
+        idx, dist = self.O.knn_csr(ref, self.p.n_neighbors)
+        dense = np.zeros(c.shape, np.float32)
+        for r in range(c.shape[0]):
+            lo, hi = c.indptr[r], c.indptr[r + 1]
+            dense[r, c.indices[lo:hi]] = c.data[lo:hi]
+        emb = self.O.fit(dense, self._op(), knn_idx=idx, knn_dist=dist).embedding
+        if out is not None:
+            out[...] = emb
+        self.emb = emb.copy()
+
     @property
     def embedding(self):
         return self.emb
@@ -119,15 +133,18 @@ def test_sparse_input_rules(fake):
     for conv in (sp.csr_matrix, sp.csc_matrix, sp.coo_matrix):
         m = Umap(n_neighbors=5, n_epochs=10)
         assert m.fit_transform(conv(x)).shape == (60, 2)
-        assert m._core.calls[-1] == "fit_transform" and m._fitted_sparse
+        assert m._core.calls[-1] == "fit_transform_csr" and m._fitted_sparse
     with pytest.raises(ValueError, match="dense input only"):
         Umap(densmap=True).fit(sp.csr_matrix(x))
     with pytest.raises(ValueError, match="dense input only"):
         Umap(output_dens=True).fit(sp.csr_matrix(x))
     with pytest.raises(ValueError, match="Canberra metric is currently supported for dense input only"):
         Umap(metric="canberra").fit(sp.csr_matrix(x))
-    with pytest.raises(ValueError, match="at most 2560 features"):
-        Umap().fit(sp.csr_matrix((10, 3000), dtype=np.float32))
+    # no feature cap: a 3000-column CSR input reaches the sparse entry point
+    wide = sp.random(30, 3000, density=0.01, random_state=4, format="csr", dtype=np.float32)
+    wm = Umap(n_neighbors=5, n_epochs=10)
+    wm.fit(wide)
+    assert wm._core.calls[-1] == "fit_transform_csr" and wm._fitted_sparse
     dense = Umap(n_neighbors=5, n_epochs=10)
     dense.fit(x)
     with pytest.raises(ValueError, match="supported only for sparse-fitted models"):

@@ -1,65 +1,206 @@
-"""Sparse input (CSR / CSC / COO) on the GPU path, after
-umap_rs/tests/test_binding.py:1314-1530: the rows are densified and fitted
-like their dense copy, so a sparse query and its dense copy transform to the
-same embedding, and the dense-only rules raise the reference's errors."""
+"""Sparse (CSR) input on the GPU: the reference's sparse exact kNN
+(rust_umap/src/sparse.rs) against the oracle's restatement, bit for bit
+(minkowski to a tolerance), with no densifying and no feature cap; the sparse
+fit and both transforms; and the reference's sparse-path rules
+(umap_rs/tests/test_binding.py:1314-1530, lib.rs:6376-6660)."""
+import ctypes
+
 import numpy as np
 import pytest
 
 from umapers_amd import Umap
+from umapers_amd import _lib as L
+from umapers_amd.diagnostics import knn_preservation, trustworthiness
+
+from . import capi
 
 pytestmark = pytest.mark.gpu
 sp = pytest.importorskip("scipy.sparse")
 
+METRICS = {"euclidean": 0, "cosine": 1, "manhattan": 2, "chebyshev": 3, "minkowski": 4,
+           "correlation": 5}
 
-def _data(n, d, seed, cut):
-    x = np.random.default_rng(seed).normal(size=(n, d)).astype(np.float32)
-    x[x < cut] = 0.0
+
+def sparse_like_data(n, dim):
+    """The reference's sparse test rows (lib.rs:5767-5778): five nonzeros per
+    row at (17 i + 13 t + 7 (i mod 3)) mod dim, many near-ties."""
+    x = np.zeros((n, dim), np.float32)
+    for i in range(n):
+        for t in range(5):
+            col = (i * 17 + t * 13 + (i % 3) * 7) % dim
+            x[i, col] = np.float32(np.float32(i + 1) * np.float32(t + 2)) * np.float32(0.017) + \
+                np.float32(col) * np.float32(1e-4)
     return x
+
+
+def _random_csr(n, cols, density, seed, scale=1.0):
+    m = sp.random(n, cols, density=density, random_state=seed, format="csr", dtype=np.float32,
+                  data_rvs=lambda k: np.random.default_rng(seed).normal(size=k) * scale)
+    m = m.tolil()
+    m[1] = 0  # an empty row
+    return m.tocsr()
+
+
+CASES = {
+    "reference_rows_61": (lambda: sp.csr_matrix(sparse_like_data(200, 61)), 12),
+    "3000_columns": (lambda: _random_csr(500, 3000, 0.01, 1), 15),
+    "rows_over_1024_nonzeros": (lambda: _random_csr(150, 5000, 0.3, 2), 10),
+    "k_64": (lambda: _random_csr(300, 50, 0.2, 3), 64),
+}
+
+
+def _check(gi, gd, oi, od, metric):
+    if metric == "minkowski":  # |x|^p from the hardware log2 / exp2
+        assert np.mean(gi == oi) > 0.98
+        assert np.allclose(gd, od, rtol=1e-5, atol=1e-6)
+    else:
+        assert np.array_equal(gi, oi)
+        assert np.array_equal(gd, od)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("metric", sorted(METRICS))
+def test_fit_knn_matches_oracle(gpu, oracle, case, metric):
+    make, k = CASES[case]
+    x = make()
+    print(f"sparse fit kNN {case} {metric}", flush=True)
+    gi, gd = capi.knn_csr(x, k, METRICS[metric], 3.0)
+    oi, od = oracle.knn_csr(x, k, metric, p=3.0)
+    _check(gi, gd, oi, od, metric)
+    assert np.all(gd[:, 0] == 0)  # every row is its own nearest at exactly 0.0 ...
+    assert np.all(gi[np.arange(x.shape[0]), 0] <= np.arange(x.shape[0]))  # ... unless tied lower
+
+
+@pytest.mark.parametrize("metric", sorted(METRICS))
+def test_query_knn_matches_oracle(gpu, oracle, metric):
+    r = _random_csr(400, 2600, 0.02, 5)
+    q = _random_csr(70, 2600, 0.02, 6)
+    qd = q.toarray()
+    gi, gd = capi.knn_csr_query(q, r, 9, METRICS[metric], 3.0)
+    oi, od = oracle.knn_csr_query(q, r, 9, metric, p=3.0)
+    _check(gi, gd, oi, od, metric)
+    gi, gd = capi.knn_dense_to_csr(qd, r, 9, METRICS[metric], 3.0)
+    oi, od = oracle.knn_dense_to_csr(qd, r, 9, metric, p=3.0)
+    _check(gi, gd, oi, od, metric)
+
+
+def test_large_norm_near_duplicate_is_bit_exact(gpu, oracle):
+    # |a|^2 + |b|^2 - 2 a.b at norm ~1e4 cancels to a value far from the direct
+    # difference (the oracle's test pins which one the reference computes)
+    rng = np.random.default_rng(9)
+    base = (rng.normal(size=40) * 3000).astype(np.float32)
+    base[rng.random(40) > 0.5] = 0
+    x = np.stack([base, base.copy(), rng.normal(size=40).astype(np.float32),
+                  (base * np.float32(1.5)).astype(np.float32)])
+    nz = np.flatnonzero(base)
+    x[1, nz[3]] = np.nextafter(x[1, nz[3]], np.float32(np.inf)) * np.float32(1.0000005)
+    m = sp.csr_matrix(x)
+    gi, gd = capi.knn_csr(m, 4, 0)
+    oi, od = oracle.knn_csr(m, 4, "euclidean")
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    gi, gd = capi.knn_csr_query(m, m, 4, 0)  # no self case: row 0 against itself is not forced to 0
+    oi, od = oracle.knn_csr_query(m, m, 4, "euclidean")
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+def test_csr_errors_through_the_abi(gpu):
+    ip = np.array([0, 2, 3], np.int64)
+    dt = np.array([1.0, 2.0, 3.0], np.float32)
+    idx = np.empty((2, 1), np.int64)
+    dist = np.empty((2, 1), np.float32)
+
+    def call(indices, cols=3):
+        ix = np.asarray(indices, np.int64)
+        return L.lib().umapx_knn_csr(2, cols, L.ptr(ip, L._i64p), L.ptr(ix, L._i64p), L.ptr(dt), 3,
+                                     1, 0, 2.0, L.ptr(idx, L._i64p), L.ptr(dist))
+
+    assert call([0, 2, 1]) == L.OK
+    assert call([2, 0, 1]) == L.ERR_INVALID_PARAMETER
+    assert "csr row 0 indices must be strictly increasing" in L.last_error()
+    assert call([0, 5, 1]) == L.ERR_INVALID_PARAMETER
+    assert "csr row 0 has column index 5 out of bounds for n_cols=3" in L.last_error()
+    assert call([0, 2, 1], cols=0) == L.ERR_INVALID_PARAMETER
+    assert "csr n_cols must be >= 1" in L.last_error()
 
 
 @pytest.mark.parametrize("metric,kw", [("chebyshev", None), ("minkowski", {"p": 3.0}),
                                        ("correlation", None)])
 def test_sparse_fit_other_metrics(gpu, metric, kw):
-    x = _data(64, 12, 56, 0.15)
+    x = sparse_like_data(68, 90)
     m = Umap(n_neighbors=7, n_epochs=30, metric=metric, metric_kwds=kw, init="random",
              random_seed=91, ann_mode="exact")
     emb = m.fit_transform(sp.csr_matrix(x))
     t = m.transform(x[:8])
-    assert emb.shape == (64, 2) and t.shape == (8, 2)
+    assert emb.shape == (68, 2) and t.shape == (8, 2)
     assert np.all(np.isfinite(emb)) and np.all(np.isfinite(t))
+    s = ctypes.c_int32()
+    L.check(L.lib().umapx_is_sparse_fit(m._core.handle, ctypes.byref(s)))
+    assert s.value == 1
 
 
-def test_sparse_query_equals_dense_query(gpu):
-    x = _data(84, 16, 58, 0.2)
-    m = Umap(n_neighbors=7, n_epochs=40, metric="cosine", init="random", random_seed=37,
+def test_sparse_query_transform_matches_dense_query_transform(gpu):
+    # lib.rs:6429-6471 (cosine): a query's CSR form and its dense form give the
+    # same kNN distances, hence the same transform
+    x = sparse_like_data(72, 80)
+    m = Umap(n_neighbors=10, n_epochs=50, metric="cosine", init="random", random_seed=2027,
              ann_mode="exact")
     m.fit_transform(sp.csr_matrix(x))
-    dense = m.transform(x[:11])
+    dense = m.transform(x[4:14])
     for conv in (sp.csr_matrix, sp.coo_matrix, sp.csc_matrix):
-        assert np.array_equal(m.transform(conv(x[:11])), dense)
-    out = np.empty((11, 2), np.float32)
-    assert m.transform(sp.csr_matrix(x[:11]), out=out) is out
-    assert np.array_equal(out, dense)
+        assert np.allclose(m.transform(conv(x[4:14])), dense, rtol=0, atol=1e-6)
+    out = np.empty((10, 2), np.float32)
+    assert m.transform(sp.csr_matrix(x[4:14]), out=out) is out
 
 
-def test_sparse_fit_matches_dense_fit(gpu):
-    # the same rows fitted dense and sparse run the same kernels on the same
-    # bytes; only the Hogwild SGD's float atomics differ between runs
-    from umapers_amd.diagnostics import trustworthiness
+def test_sparse_fit_and_transforms_parity_with_oracle(gpu, oracle):
+    # 3000 x 4000 topic-like sparse rows, about 100 nonzeros each (no feature
+    # cap): the sparse fit's quality against the oracle's sparse fit (its
+    # sparse kNN, then the same stages), both ways, over three seeds; then the
+    # CSR and dense query transforms against the oracle's transform from its
+    # own sparse query kNN
+    rng = np.random.default_rng(12)
+    topics = [rng.choice(4000, 300, replace=False) for _ in range(8)]  # each cluster's columns
+    lab = rng.integers(0, 8, 3500)
+    dense = np.zeros((3500, 4000), np.float32)
+    for i, c in enumerate(lab):
+        cols = np.unique(np.concatenate([rng.choice(topics[c], 80, replace=False),
+                                         rng.choice(4000, 20, replace=False)]))
+        dense[i, cols] = np.abs(rng.normal(1.0, 0.3, cols.size)).astype(np.float32)
+    x, q = sp.csr_matrix(dense[:3000]), sp.csr_matrix(dense[3000:])
+    xd = dense[:3000]
+    tg, to, kg, ko = [], [], [], []
+    for s in (3, 4, 5):
+        m = Umap(n_neighbors=15, n_epochs=200, random_seed=s, ann_mode="exact")
+        y = m.fit_transform(x)
+        oi, od = oracle.knn_csr(x, 15)
+        yo = oracle.fit(xd, oracle.OracleParams(n_epochs=200, random_seed=s), knn_idx=oi,
+                        knn_dist=od).embedding
+        tg.append(trustworthiness(xd, y))
+        to.append(trustworthiness(xd, yo))
+        kg.append(knn_preservation(xd, y))
+        ko.append(knn_preservation(xd, yo))
+    print(f"sparse fit: trust gpu {np.mean(tg):.4f} oracle {np.mean(to):.4f}; knn-pres gpu "
+          f"{np.mean(kg):.4f} oracle {np.mean(ko):.4f}", flush=True)
+    assert abs(np.mean(tg) - np.mean(to)) <= 0.01, (tg, to)
+    assert abs(np.mean(kg) - np.mean(ko)) <= 0.01, (kg, ko)
+    yq = m.transform(q)
+    yqd = m.transform(dense[3000:])
+    qi, qdist = oracle.knn_csr_query(q, x, 15)
+    yo = oracle.transform_with_knn(y, qi, qdist, oracle.OracleParams(n_epochs=200, random_seed=5))
+    both = dense
+    t_csr = trustworthiness(both, np.concatenate([y, yq]))
+    t_dense = trustworthiness(both, np.concatenate([y, yqd]))
+    t_orc = trustworthiness(both, np.concatenate([y, yo]))
+    print(f"sparse transforms: trust csr {t_csr:.4f} dense {t_dense:.4f} oracle {t_orc:.4f}",
+          flush=True)
+    assert abs(t_csr - t_orc) <= 0.01 and abs(t_dense - t_orc) <= 0.01
 
-    # (one run's trustworthiness on these rows varies by about 0.01, so three
-    # seeds are averaged)
-    x = _data(2000, 20, 7, 0.3)
-    ta, tb = [], []
-    for seed in (5, 6, 7):
-        kw = dict(n_neighbors=10, n_epochs=100, random_seed=seed, ann_mode="exact")
-        ta.append(trustworthiness(x, Umap(**kw).fit_transform(x)))
-        tb.append(trustworthiness(x, Umap(**kw).fit_transform(sp.csr_matrix(x))))
-    assert abs(np.mean(ta) - np.mean(tb)) < 0.01, (ta, tb)
 
-
-def test_sparse_trained_inverse_errors(gpu):
-    x = _data(64, 9, 71, 0.2)
+def test_sparse_rules(gpu):
+    x = sparse_like_data(64, 9)
+    dense = Umap(n_neighbors=6, n_epochs=20, init="random", ann_mode="exact").fit(x)
+    with pytest.raises(ValueError, match="supported only for sparse-fitted models"):
+        dense.transform(sp.csr_matrix(x[:3]))
     m = Umap(n_neighbors=6, n_epochs=40, metric="manhattan", init="random", random_seed=29,
              ann_mode="exact")
     m.fit(sp.csr_matrix(x))
@@ -67,3 +208,10 @@ def test_sparse_trained_inverse_errors(gpu):
         m.inverse_transform(np.zeros((5, 2), np.float32), out=np.empty((5, 10), np.float32))
     with pytest.raises(ValueError, match="not supported for sparse-trained models"):
         m.inverse_transform(np.zeros((5, 2), np.float32))
+    with pytest.raises(ValueError, match="feature mismatch: expected 9, got 8"):
+        m.transform(sp.csr_matrix(x[:3, :8]))
+    for kw in ({"densmap": True}, {"output_dens": True}):
+        with pytest.raises(ValueError, match="dense input only"):
+            Umap(**kw).fit(sp.csr_matrix(x))
+    with pytest.raises(ValueError, match="BrayCurtis metric is currently supported for dense"):
+        Umap(metric="braycurtis").fit(sp.csr_matrix(x))

@@ -64,16 +64,18 @@ def _is_sparse(x: Any) -> bool:
     return getattr(x, "format", None) in {"csr", "csc", "coo"}
 
 
-# Sparse rows are densified on the host and run the dense path: the
-# reference's own sparse kNN equals its dense kNN on the same values
-# (lib.rs sparse_csr_knn_matches_dense_euclidean), and the GPU kNN under
-# every sparse-capable metric takes at most this many features.
-_SPARSE_MAX_FEATURES = 2560
+class _CsrParts:
+    """Canonical CSR arrays of a sparse input: int64 indptr and column indices,
+    float32 data, with the matrix's shape."""
+
+    def __init__(self, indptr, indices, data, n_rows: int, n_cols: int):
+        self.indptr, self.indices, self.data = indptr, indices, data
+        self.shape = (n_rows, n_cols)
 
 
-def _sparse_to_dense(x: Any, name: str) -> np.ndarray:
-    """_api.py:143-190 (CSR/CSC/COO normalised to canonical CSR), then dense
-    float32 rows."""
+def _as_csr_parts(x: Any, name: str) -> _CsrParts:
+    """_api.py:143-190: CSR / CSC / COO normalised to canonical CSR (summed
+    duplicates, sorted indices); the rows stay sparse all the way to the GPU."""
     fmt = getattr(x, "format", None)
     matrix = x
     if fmt in {"csc", "coo"}:
@@ -86,11 +88,24 @@ def _sparse_to_dense(x: Any, name: str) -> np.ndarray:
     if getattr(matrix, "has_canonical_format", True) is False:
         matrix = matrix.copy()
         matrix.sum_duplicates()
-    n_rows, n_cols = matrix.shape
-    if n_cols > _SPARSE_MAX_FEATURES:
-        raise ValueError(f"sparse input on the GPU path is densified and supports at most "
-                         f"{_SPARSE_MAX_FEATURES} features, got {n_cols}")
-    return np.ascontiguousarray(matrix.toarray(), dtype=np.float32)
+        matrix.sort_indices()
+    elif getattr(matrix, "has_sorted_indices", True) is False:
+        matrix = matrix.copy()
+        matrix.sort_indices()
+    shape = getattr(matrix, "shape", None)
+    if shape is None or len(shape) != 2:
+        raise ValueError(f"{name} must be a 2D CSR matrix")
+    n_rows, n_cols = int(shape[0]), int(shape[1])
+    if n_cols <= 0:
+        raise ValueError(f"{name} must have at least one column")
+    indptr = np.ascontiguousarray(matrix.indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(matrix.indices, dtype=np.int64)
+    data = np.ascontiguousarray(matrix.data, dtype=np.float32)
+    if indptr.ndim != 1 or indices.ndim != 1 or data.ndim != 1:
+        raise ValueError(f"{name} CSR arrays must be 1D")
+    if indices.shape[0] != data.shape[0]:
+        raise ValueError(f"{name} CSR indices/data length mismatch")
+    return _CsrParts(indptr, indices, data, n_rows, n_cols)
 
 
 def _as_out_buffer(out: Any, shape: tuple[int, int]) -> np.ndarray:
@@ -240,6 +255,16 @@ class _Core:
             "gpu": {k: td[k] for k in ("knn_kernel_ms", "knn_flops", "sgd_kernel_ms",
                                        "init_used_spectral", "n_graph_components")},
         }
+
+    def fit_transform_csr_into(self, c: "_CsrParts", out: np.ndarray | None) -> None:
+        L.check(self._lib.umapx_fit_transform_csr(
+            self._h, c.shape[0], c.shape[1], L.ptr(c.indptr, L._i64p), L.ptr(c.indices, L._i64p),
+            L.ptr(c.data), c.data.shape[0], None if out is None else L.ptr(out)))
+
+    def transform_csr_into(self, c: "_CsrParts", out: np.ndarray) -> None:
+        L.check(self._lib.umapx_transform_csr(
+            self._h, c.shape[0], c.shape[1], L.ptr(c.indptr, L._i64p), L.ptr(c.indices, L._i64p),
+            L.ptr(c.data), c.data.shape[0], L.ptr(out)))
 
     def timings(self) -> dict[str, Any]:
         t = L.Timings()
@@ -475,11 +500,10 @@ class Umap:
         self.radii_original_, self.radii_embedding_ = radii
         self.rad_orig_, self.rad_emb_ = radii
 
-    def _fit_rows(self, data: Any) -> np.ndarray:
-        """Dense rows for fit; sparse input follows the reference's sparse-fit
-        rules (_api.py:562-570, lib.rs:2241-2249) and is densified."""
+    def _fit_rows(self, data: Any):
+        """Dense rows, or canonical CSR parts for sparse input, which follows
+        the reference's sparse-fit rules (_api.py:562-570, lib.rs:2241-2258)."""
         if not _is_sparse(data):
-            self._fitted_sparse = False
             return _as_f32_matrix(data, "data")
         if self.densmap or self.output_dens:
             raise ValueError("densmap and output_dens currently support dense input only")
@@ -487,37 +511,34 @@ class Umap:
             name = {"canberra": "Canberra", "braycurtis": "BrayCurtis"}[self.metric]
             raise ValueError(f"invalid parameter: {name} metric is currently supported for dense "
                              f"input only")
-        arr = _sparse_to_dense(data, "data")
-        self._fitted_sparse = True
-        return arr
-
-    def _query_rows(self, query: Any) -> np.ndarray:
-        if not _is_sparse(query):
-            return _as_f32_matrix(query, "query")
-        if not getattr(self, "_fitted_sparse", False):
-            raise ValueError("sparse query transform is supported only for sparse-fitted models")
-        return _sparse_to_dense(query, "query")
+        return _as_csr_parts(data, "data")
 
     def fit(self, data: Any, y: Any | None = None) -> "Umap":
         arr = self._fit_rows(data)
-        self._core.fit(arr)
+        if isinstance(arr, _CsrParts):
+            self._core.fit_transform_csr_into(arr, None)
+        else:
+            self._core.fit(arr)
+        self._fitted_sparse = isinstance(arr, _CsrParts)
         self._sync_fitted(self._core.embedding, arr.shape[1], arr.shape[0])
         return self
 
     def fit_transform(self, data: Any, y: Any | None = None, *,
                       out: np.ndarray | None = None):
         arr = self._fit_rows(data)
+        sparse = isinstance(arr, _CsrParts)
         if out is not None and self.output_dens:
             raise ValueError("out cannot be used when output_dens=True")
-        if out is None:
-            embedding = self._core.fit_transform(arr)
-            self._sync_fitted(embedding, arr.shape[1], arr.shape[0])
-            if self.output_dens:
-                return embedding, self.radii_original_, self.radii_embedding_
-            return embedding
-        out_buf = _as_out_buffer(out, (arr.shape[0], self.n_components))
-        self._core.fit_transform_into(arr, out_buf)
-        self._sync_fitted(out_buf, arr.shape[1], arr.shape[0])
+        shape = (arr.shape[0], self.n_components)
+        out_buf = np.empty(shape, np.float32) if out is None else _as_out_buffer(out, shape)
+        if sparse:
+            self._core.fit_transform_csr_into(arr, out_buf)
+        else:
+            self._core.fit_transform_into(arr, out_buf)
+        self._fitted_sparse = sparse
+        self._sync_fitted(out_buf if out is None else out_buf.copy(), arr.shape[1], arr.shape[0])
+        if out is None and self.output_dens:
+            return out_buf, self.radii_original_, self.radii_embedding_
         return out_buf
 
     def profile_fit_transform(self, data: Any) -> dict[str, Any]:
@@ -546,7 +567,16 @@ class Umap:
         return out_buf
 
     def transform(self, query: Any, *, out: np.ndarray | None = None) -> np.ndarray:
-        arr = self._query_rows(query)
+        if _is_sparse(query):
+            if getattr(self, "_fitted_sparse", None) is False:  # fitted on dense rows
+                raise ValueError("sparse query transform is supported only for sparse-fitted "
+                                 "models")
+            c = _as_csr_parts(query, "query")
+            shape = (c.shape[0], self.n_components)
+            out_buf = np.empty(shape, np.float32) if out is None else _as_out_buffer(out, shape)
+            self._core.transform_csr_into(c, out_buf)
+            return out_buf
+        arr = _as_f32_matrix(query, "query")
         shape = (arr.shape[0], self.n_components)
         if out is None:
             return self._core.transform(arr)

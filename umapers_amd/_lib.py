@@ -131,6 +131,22 @@ SIGNATURES = {
     "umapx_profile_fit_transform": (ctypes.c_int32, [
         _vp, _f32p, ctypes.c_int64, ctypes.c_int64, _f32p, ctypes.POINTER(Timings)]),
     "umapx_get_timings": (ctypes.c_int32, [_vp, ctypes.POINTER(Timings)]),
+    "umapx_fit_transform_csr": (ctypes.c_int32, [_vp, ctypes.c_int64, ctypes.c_int64, _i64p,
+                                                 _i64p, _f32p, ctypes.c_int64, _f32p]),
+    "umapx_transform_csr": (ctypes.c_int32, [_vp, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
+                                             _f32p, ctypes.c_int64, _f32p]),
+    "umapx_is_sparse_fit": (ctypes.c_int32, [_vp, ctypes.POINTER(ctypes.c_int32)]),
+    "umapx_knn_csr": (ctypes.c_int32, [ctypes.c_int64, ctypes.c_int64, _i64p, _i64p, _f32p,
+                                       ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_float, _i64p, _f32p]),
+    "umapx_knn_csr_query": (ctypes.c_int32, [ctypes.c_int64, _i64p, _i64p, _f32p, ctypes.c_int64,
+                                             ctypes.c_int64, _i64p, _i64p, _f32p, ctypes.c_int64,
+                                             ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_float, _i64p, _f32p]),
+    "umapx_knn_dense_to_csr": (ctypes.c_int32, [_f32p, ctypes.c_int64, ctypes.c_int64, _i64p,
+                                                _i64p, _f32p, ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                                _i64p, _f32p]),
     "umapx_transform": (ctypes.c_int32, [_vp, _f32p, ctypes.c_int64, ctypes.c_int64, _f32p]),
     "umapx_transform_device": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp]),
     "umapx_transform_plain": (ctypes.c_int32, [_vp, _f32p, ctypes.c_int64, ctypes.c_int64, _f32p]),

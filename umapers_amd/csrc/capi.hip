@@ -186,6 +186,18 @@ void validate_params(const umapx_params& p, int64_t n_samples) {
   if (p.rerank_factor < 1) throw_invalid("rerank_factor must be >= 1");
 }
 
+// validate_sparse_metric and validate_sparse_density_params
+// (lib.rs:2241-2258), plus what the GPU path holds for sparse fits.
+void validate_sparse_params(const umapx_params& p) {
+  if (p.metric == UMAPX_METRIC_CANBERRA || p.metric == UMAPX_METRIC_BRAYCURTIS)
+    throw_invalid(metric_debug(p.metric, p.metric_p) +
+                  " metric is currently supported for dense input only");
+  if (p.densmap || p.output_dens)
+    throw_invalid("densmap and output_dens are currently supported for dense input only");
+  if (p.transform_mode == UMAPX_TRANSFORM_LEARNED)
+    throw_invalid("the learned transform supports dense input only");
+}
+
 // should_use_approximate_knn (lib.rs:2285-2311, constants lib.rs:24-26):
 // auto mode switches to the approximate rounds only past 4 x the threshold
 // (16 features or fewer: the same multiplier); a threshold of 0 is the
@@ -295,6 +307,8 @@ struct umapx_handle {
   int64_t n = 0, d = 0;
   int k_fit = 0;
   DevBuf<float> x, y, sig, rho, rad_orig, rad_emb, knn_dist;
+  bool sparse = false;  // fitted on CSR rows (held in csr instead of x)
+  CsrDevice csr;
   DevBuf<int32_t> knn_idx;
   KnnWorkspace kws;
   GraphWorkspace gws;
@@ -321,8 +335,11 @@ struct umapx_handle {
   // build_dense_fit_artifacts_with_target (lib.rs:1367-1464) on device data.
   // d_x stays valid for the whole call; pre_idx/pre_dist are device kNN when
   // the caller supplied it (already validated and trimmed to k columns).
+  // csr_in: a sparse fit (build_sparse_fit_artifacts, lib.rs:1491-1550); d_x is
+  // then null, the kNN is the sparse one and the rows are kept as CSR.
   void fit_device(const float* d_x, int64_t n_, int64_t d_, const int32_t* pre_idx,
-                  const float* pre_dist, bool validated_on_host, double t_start) {
+                  const float* pre_dist, bool validated_on_host, double t_start,
+                  CsrDevice* csr_in = nullptr) {
     ensure_stream();
     hipStream_t s = stream;
     AllocStreamScope alloc_scope(s);
@@ -337,7 +354,11 @@ struct umapx_handle {
     KnnStats kst;
     kst.cand_start = ev[0];
     kst.cand_stop = ev[1];
-    if (pre_idx == nullptr) {
+    if (csr_in) {  // exact_nearest_neighbors over the CSR rows (sparse.rs:605-626)
+      idx.alloc(static_cast<size_t>(n_) * k);
+      dist.alloc(static_cast<size_t>(n_) * k);
+      csr_knn_device(*csr_in, *csr_in, true, k, p.metric, p.metric_p, idx.get(), dist.get(), s);
+    } else if (pre_idx == nullptr) {
       idx.alloc(static_cast<size_t>(n_) * k);
       dist.alloc(static_cast<size_t>(n_) * k);
       const bool approx = should_use_approximate_knn(p, n_, d_);
@@ -386,7 +407,7 @@ struct umapx_handle {
     bool spectral_ok = false;
     if (p.init == UMAPX_INIT_SPECTRAL)
       spectral_ok = spectral_init_device(d_x, n_, d_, p.n_components, g, p.random_seed, yy.get(),
-                                         spws, s, &rep);
+                                         spws, s, &rep, csr_in);
     if (!spectral_ok) {
       auto r = random_init(n_, p.n_components, p.random_seed, -10.0f, 10.0f);
       HIP_CHECK(hipMemcpyAsync(yy.get(), r.data(), 4 * r.size(), hipMemcpyHostToDevice, s));
@@ -431,9 +452,16 @@ struct umapx_handle {
     }
     t.density_embedding_sec = clk.lap();
     t.output_copy_sec = 0.0;
-    // store_dense_fit_state (lib.rs:1551-1569)
-    x.alloc(static_cast<size_t>(n_) * d_);
-    HIP_CHECK(hipMemcpyAsync(x.get(), d_x, 4 * n_ * d_, hipMemcpyDeviceToDevice, s));
+    // store_dense_fit_state / store_sparse_fit_state (lib.rs:1551-1590)
+    if (csr_in) {
+      csr = std::move(*csr_in);
+      x.release();
+    } else {
+      x.alloc(static_cast<size_t>(n_) * d_);
+      HIP_CHECK(hipMemcpyAsync(x.get(), d_x, 4 * n_ * d_, hipMemcpyDeviceToDevice, s));
+      csr = CsrDevice{};
+    }
+    sparse = csr_in != nullptr;
     knn_idx.alloc(static_cast<size_t>(n_) * k);
     knn_dist.alloc(static_cast<size_t>(n_) * k);
     HIP_CHECK(hipMemcpyAsync(knn_idx.get(), kidx, 4 * n_ * k, hipMemcpyDeviceToDevice, s));
@@ -488,23 +516,87 @@ struct umapx_handle {
     }
   }
 
+  // fit_sparse_csr / fit_transform_sparse_csr (lib.rs:998-1036) on host CSR
+  // arrays (SparseCsrMatrix::new's checks first, as the reference's caller
+  // builds the matrix before the fit).
+  void fit_csr_host(int64_t rows, int64_t cols, const int64_t* indptr, const int64_t* indices,
+                    const float* data, int64_t nnz, float* y_out) {
+    const double t0 = now_sec();
+    if (!indptr || (nnz > 0 && (!indices || !data))) throw_invalid("null argument");
+    csr_validate_host(rows, cols, indptr, rows + 1, indices, data, nnz);
+    if (rows == 0) throw Error(kEmptyData, "input data is empty");
+    if (rows < 2) throw Error(kNeedTwoSamples, "need at least two samples");
+    validate_params(p, rows);
+    validate_sparse_params(p);
+    ensure_stream();
+    CsrDevice c;
+    {
+      AllocStreamScope alloc_scope(stream);
+      csr_upload(c, rows, cols, indptr, indices, data, stream);
+    }
+    fit_device(nullptr, rows, cols, nullptr, nullptr, true, t0, &c);
+    if (y_out) {
+      const double s0 = now_sec();
+      HIP_CHECK(hipMemcpyAsync(y_out, y.get(), 4 * n * p.n_components, hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      timings.output_copy_sec = now_sec() - s0;
+      timings.total_sec = now_sec() - t0;
+    }
+  }
+
   // transform_rows + transform_from_knn (lib.rs:1604-1685, 1738-1780).
   void transform_plain_device(const float* d_q, int64_t m, float* d_out) {
     ensure_stream();
     hipStream_t s = stream;
     AllocStreamScope alloc_scope(s);
     const int k = static_cast<int>(std::min<int64_t>(p.n_neighbors, n));
-    const int32_t n_epochs = p.n_epochs > 0 ? std::max(p.n_epochs, 1) : (m <= 10000 ? 100 : 30);
     DevBuf<int32_t> idx(static_cast<size_t>(m) * k);
     DevBuf<float> dist(static_cast<size_t>(m) * k);
-    knn_device(d_q, m, x.get(), n, d, k, p.metric, p.knn_precision, p.rerank_factor, idx.get(),
-               dist.get(), kws, s, nullptr, p.metric_p);
+    if (sparse)  // dense queries against the sparse fit (sparse.rs:753-900)
+      dense_to_csr_knn_device(d_q, m, csr, k, p.metric, p.metric_p, idx.get(), dist.get(), s);
+    else
+      knn_device(d_q, m, x.get(), n, d, k, p.metric, p.knn_precision, p.rerank_factor, idx.get(),
+                 dist.get(), kws, s, nullptr, p.metric_p);
+    transform_from_knn_device(idx.get(), dist.get(), m, k, d_out);
+  }
+
+  // transform_sparse_csr (lib.rs:1687-1737): CSR queries on a sparse fit.
+  void transform_csr_host(int64_t m, int64_t cols, const int64_t* indptr, const int64_t* indices,
+                          const float* data, int64_t nnz, float* out) {
+    if (!fitted) throw Error(kNotFitted, "model is not fitted yet");
+    if (cols != d)
+      throw Error(kFeatureMismatch,
+                  "feature mismatch: expected " + std::to_string(d) + ", got " + std::to_string(cols),
+                  -1, d, cols);
+    if (!sparse) throw_invalid("sparse query transform is supported only for sparse-fitted models");
+    if (m == 0) return;
+    csr_validate_host(m, cols, indptr, m + 1, indices, data, nnz);
+    ensure_stream();
+    hipStream_t s = stream;
+    AllocStreamScope alloc_scope(s);
+    CsrDevice qc;
+    csr_upload(qc, m, cols, indptr, indices, data, s);
+    const int k = static_cast<int>(std::min<int64_t>(p.n_neighbors, n));
+    DevBuf<int32_t> idx(static_cast<size_t>(m) * k);
+    DevBuf<float> dist(static_cast<size_t>(m) * k);
+    csr_knn_device(qc, csr, false, k, p.metric, p.metric_p, idx.get(), dist.get(), s);
+    DevBuf<float> dout(static_cast<size_t>(m) * p.n_components);
+    transform_from_knn_device(idx.get(), dist.get(), m, k, dout.get());
+    HIP_CHECK(hipMemcpyAsync(out, dout.get(), 4 * m * p.n_components, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+
+  // transform_from_knn (lib.rs:1738-1780) on a device query kNN [m x k].
+  void transform_from_knn_device(const int32_t* d_idx, const float* d_dist, int64_t m, int k,
+                                 float* d_out) {
+    hipStream_t s = stream;
+    const int32_t n_epochs = p.n_epochs > 0 ? std::max(p.n_epochs, 1) : (m <= 10000 ? 100 : 30);
     DevBuf<float> sg(m), rh(m);
     const float lc = std::max(p.local_connectivity - 1.0f, 0.0f);
-    smooth_knn_dist_device(dist.get(), m, k, static_cast<float>(k), lc, 1.0f, false,
+    smooth_knn_dist_device(d_dist, m, k, static_cast<float>(k), lc, 1.0f, false,
                            p.smooth_fp64 != 0, sg.get(), rh.get(), gws, s);
     Graph g;
-    bipartite_graph_device(idx.get(), dist.get(), sg.get(), rh.get(), m, k, n_epochs, g, gws, s);
+    bipartite_graph_device(d_idx, d_dist, sg.get(), rh.get(), m, k, n_epochs, g, gws, s);
     init_graph_transform_device(m, p.n_components, g, y.get(), d_out, s);
     SgdArgs sa;
     sa.n_epochs = n_epochs;
@@ -564,6 +656,7 @@ struct umapx_handle {
   // lib.rs:1803-1847: fitted model, then per-row width and finiteness.
   void check_embedded(int64_t m, int64_t dq) {
     if (!fitted) throw Error(kNotFitted, "model is not fitted yet");
+    if (sparse) throw_invalid("inverse_transform is not supported for sparse-trained models yet");
     if (m > 0 && dq != p.n_components)
       throw Error(kEmbeddingDimensionMismatch,
                   "embedding dimension mismatch: expected " + std::to_string(p.n_components) +
@@ -590,6 +683,7 @@ struct umapx_handle {
   void run_transform(const float* d_q, int64_t m, float* d_out, int mode) {
     AllocStreamScope alloc_scope(stream);
     if (mode == UMAPX_TRANSFORM_LEARNED) {
+      if (sparse) throw_invalid("the learned transform supports dense input only");
       if (!mlp.trained) {
         mlp_train(mlp, x.get(), n, d, y.get(), p.n_components, graph, a, b, p, stream);
       }
@@ -759,6 +853,33 @@ int32_t umapx_profile_fit_transform(umapx_handle* h, const float* x, int64_t n, 
     h->count_schedule = true;
     h->fit_host(x, n, d, nullptr, nullptr, y_out);
     if (t) *t = h->timings;
+  });
+}
+
+int32_t umapx_fit_transform_csr(umapx_handle* h, int64_t n_rows, int64_t n_cols,
+                                const int64_t* indptr, const int64_t* indices, const float* data,
+                                int64_t nnz, float* y_out) {
+  return guarded([&] {
+    require_handle(h);
+    h->fit_csr_host(n_rows, n_cols, indptr, indices, data, nnz, y_out);
+  });
+}
+
+int32_t umapx_transform_csr(umapx_handle* h, int64_t m, int64_t n_cols, const int64_t* indptr,
+                            const int64_t* indices, const float* data, int64_t nnz, float* out) {
+  return guarded([&] {
+    require_handle(h);
+    if (m > 0 && (!indptr || !out || (nnz > 0 && (!indices || !data))))
+      throw_invalid("null argument");
+    h->transform_csr_host(m, n_cols, indptr, indices, data, nnz, out);
+  });
+}
+
+int32_t umapx_is_sparse_fit(const umapx_handle* h, int32_t* sparse) {
+  return guarded([&] {
+    require_handle(h);
+    if (!h->fitted) throw Error(kNotFitted, "model is not fitted yet");
+    *sparse = h->sparse ? 1 : 0;
   });
 }
 
@@ -981,6 +1102,29 @@ int32_t knn_host(const float* q, int64_t m, const float* r, int64_t n, int64_t d
   });
 }
 
+// The sparse kNN stages on host buffers: upload, run on a private stream,
+// copy back [m x k] int64 indices and f32 distances.
+template <typename Run>
+int32_t csr_knn_host(int64_t m, int32_t k, int64_t* idx, float* dist, Run run) {
+  return guarded([&] {
+    if (k < 1) throw_invalid("n_neighbors must be >= 1");
+    if (m == 0) return;
+    if (!idx || !dist) throw_invalid("null argument");
+    umapx_handle::StreamOwner so;
+    HIP_CHECK(hipStreamCreateWithFlags(&so.s, hipStreamNonBlocking));
+    std::vector<int32_t> hi(static_cast<size_t>(m) * k);
+    {
+      AllocStreamScope alloc_scope(so.s);
+      DevBuf<int32_t> di(hi.size());
+      DevBuf<float> dd(hi.size());
+      run(di.get(), dd.get(), so.s);
+      HIP_CHECK(hipMemcpyAsync(hi.data(), di.get(), 4 * hi.size(), hipMemcpyDeviceToHost, so.s));
+      HIP_CHECK(hipMemcpyAsync(dist, dd.get(), 4 * hi.size(), hipMemcpyDeviceToHost, so.s));
+      HIP_CHECK(hipStreamSynchronize(so.s));
+    }
+    idx64_from32(hi, idx);
+  });
+}
 }  // namespace
 
 extern "C" {
@@ -1020,6 +1164,54 @@ int32_t umapx_knn_approx(const float* x, int64_t n, int64_t d, int32_t k, int32_
       HIP_CHECK(hipStreamSynchronize(so.s));
     }
     idx64_from32(hi, idx);
+  });
+}
+
+
+int32_t umapx_knn_csr(int64_t n, int64_t n_cols, const int64_t* indptr, const int64_t* indices,
+                      const float* data, int64_t nnz, int32_t k, int32_t metric, float metric_p,
+                      int64_t* idx, float* dist) {
+  return csr_knn_host(n, k, idx, dist, [&](int32_t* di, float* dd, hipStream_t s) {
+    validate_metric(metric, metric_p);
+    csr_validate_host(n, n_cols, indptr, n + 1, indices, data, nnz);
+    CsrDevice c;
+    csr_upload(c, n, n_cols, indptr, indices, data, s);
+    csr_knn_device(c, c, true, k, metric, metric_p, di, dd, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int32_t umapx_knn_csr_query(int64_t m, const int64_t* q_indptr, const int64_t* q_indices,
+                            const float* q_data, int64_t q_nnz, int64_t n,
+                            const int64_t* r_indptr, const int64_t* r_indices,
+                            const float* r_data, int64_t r_nnz, int64_t n_cols, int32_t k,
+                            int32_t metric, float metric_p, int64_t* idx, float* dist) {
+  return csr_knn_host(m, k, idx, dist, [&](int32_t* di, float* dd, hipStream_t s) {
+    validate_metric(metric, metric_p);
+    csr_validate_host(m, n_cols, q_indptr, m + 1, q_indices, q_data, q_nnz);
+    csr_validate_host(n, n_cols, r_indptr, n + 1, r_indices, r_data, r_nnz);
+    CsrDevice qc, rc;
+    csr_upload(qc, m, n_cols, q_indptr, q_indices, q_data, s);
+    csr_upload(rc, n, n_cols, r_indptr, r_indices, r_data, s);
+    csr_knn_device(qc, rc, false, k, metric, metric_p, di, dd, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int32_t umapx_knn_dense_to_csr(const float* q, int64_t m, int64_t n, const int64_t* r_indptr,
+                               const int64_t* r_indices, const float* r_data, int64_t r_nnz,
+                               int64_t n_cols, int32_t k, int32_t metric, float metric_p,
+                               int64_t* idx, float* dist) {
+  return csr_knn_host(m, k, idx, dist, [&](int32_t* di, float* dd, hipStream_t s) {
+    validate_metric(metric, metric_p);
+    if (!q) throw_invalid("null argument");
+    csr_validate_host(n, n_cols, r_indptr, n + 1, r_indices, r_data, r_nnz);
+    CsrDevice rc;
+    csr_upload(rc, n, n_cols, r_indptr, r_indices, r_data, s);
+    DevBuf<float> dq(static_cast<size_t>(m) * n_cols);
+    HIP_CHECK(hipMemcpyAsync(dq.get(), q, 4 * m * n_cols, hipMemcpyHostToDevice, s));
+    dense_to_csr_knn_device(dq.get(), m, rc, k, metric, metric_p, di, dd, s);
+    HIP_CHECK(hipStreamSynchronize(s));
   });
 }
 

@@ -82,4 +82,32 @@ void knn_metric_device(const float* q, int64_t m, const float* r, int64_t n, int
                        int metric, float p, int32_t* out_idx, float* out_dist, KnnWorkspace& ws,
                        hipStream_t s);
 
+// CSR rows on the device as the reference's SparseCsrMatrix holds them
+// (rust_umap/src/sparse.rs:9-107): int64 row offsets, int32 column indices
+// strictly increasing within a row, f32 values, and the per-row squared norm,
+// L2 norm and sum in the reference's nonzero order.
+struct CsrDevice {
+  int64_t rows = 0, cols = 0, nnz = 0;
+  DevBuf<int64_t> ptr;
+  DevBuf<int32_t> ind;
+  DevBuf<float> val, sq, l2, sum;
+};
+// SparseCsrMatrix::new's checks (sparse.rs:20-90) on host arrays; throws
+// InvalidParameter with the reference's message.
+void csr_validate_host(int64_t rows, int64_t cols, const int64_t* indptr, int64_t indptr_len,
+                       const int64_t* indices, const float* data, int64_t nnz);
+// Uploads validated host CSR arrays and forms the row summaries.
+void csr_upload(CsrDevice& c, int64_t rows, int64_t cols, const int64_t* indptr,
+                const int64_t* indices, const float* data, hipStream_t s);
+// Exact top-k of every CSR query row against the CSR reference rows, each
+// distance in the reference's sparse arithmetic (sparse.rs:220-603); self:
+// q is the fit's own rows and row i is at distance 0.0 from itself
+// (exact_nearest_neighbors, sparse.rs:605-626), else no self case
+// (exact_nearest_neighbors_sparse_query, :628-657).  metric codes 0-5; k <= 64.
+void csr_knn_device(const CsrDevice& q, const CsrDevice& r, bool self, int k, int metric,
+                    float p, int32_t* out_idx, float* out_dist, hipStream_t s);
+// Dense query rows q [m x r.cols] against CSR reference rows
+// (exact_nearest_neighbors_dense_query, sparse.rs:659-900).
+void dense_to_csr_knn_device(const float* q, int64_t m, const CsrDevice& r, int k, int metric,
+                             float p, int32_t* out_idx, float* out_dist, hipStream_t s);
 }  // namespace umapx

@@ -19,6 +19,7 @@
 // Smaller components take the reference's dense Laplacian path on the host,
 // and tiny ones its random layout, exactly as lib.rs:4670-4764 decides.
 #include "spectral.hpp"
+#include "knn.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -566,6 +567,25 @@ __global__ void centroid_kernel(const int32_t* __restrict__ items, const int32_t
   }
 }
 
+// component_centroids_sparse (lib.rs:4487-4516): the item's rows' nonzeros
+// added into their columns (zeros add nothing, so the sums equal the dense
+// ones).
+__global__ void centroid_csr_kernel(const int32_t* __restrict__ items,
+                                    const int32_t* __restrict__ perm,
+                                    const int64_t* __restrict__ ptr,
+                                    const int32_t* __restrict__ ind,
+                                    const float* __restrict__ val, int64_t d,
+                                    double* __restrict__ cent) {
+  const int32_t slot = items[blockIdx.x * 3 + 0];
+  const int32_t p0 = items[blockIdx.x * 3 + 1];
+  const int32_t p1 = items[blockIdx.x * 3 + 2];
+  for (int32_t p = p0 + static_cast<int32_t>(threadIdx.x); p < p1; p += blockDim.x) {
+    const int64_t row = perm[p];
+    for (int64_t e = ptr[row]; e < ptr[row + 1]; ++e)
+      atomicAdd(&cent[static_cast<int64_t>(slot) * d + ind[e]], static_cast<double>(val[e]));
+  }
+}
+
 __global__ void scatter_rows_kernel(const int32_t* __restrict__ rows_idx, int64_t count, int dim,
                                     const float* __restrict__ vals, float* __restrict__ y) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -729,7 +749,7 @@ int64_t connected_components_device(const Graph& g, std::vector<int32_t>& labels
 
 bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const Graph& g,
                           uint64_t seed, float* y, SpectralWorkspace& ws, hipStream_t s,
-                          SpectralReport* report) {
+                          SpectralReport* report, const CsrDevice* csr) {
   maybe_inject_fault("init");
   SpectralReport rep;
   if (n <= dim + 1 || g.n_edges == 0) return false;
@@ -784,8 +804,14 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     HIP_CHECK(hipMemcpyAsync(ws.perm.get(), perm_all.data(), perm_all.size() * 4, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(ws.items.get(), items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemsetAsync(ws.cent.get(), 0, sizeof(double) * nc * d, s));
-    hipLaunchKernelGGL(centroid_kernel, dim3(static_cast<unsigned>(items.size() / 3)), dim3(256), 0, s,
-                       ws.items.get(), ws.perm.get(), x, static_cast<int>(d), ws.cent.get());
+    if (csr)
+      hipLaunchKernelGGL(centroid_csr_kernel, dim3(static_cast<unsigned>(items.size() / 3)),
+                         dim3(256), 0, s, ws.items.get(), ws.perm.get(), csr->ptr.get(),
+                         csr->ind.get(), csr->val.get(), d, ws.cent.get());
+    else
+      hipLaunchKernelGGL(centroid_kernel, dim3(static_cast<unsigned>(items.size() / 3)), dim3(256),
+                         0, s, ws.items.get(), ws.perm.get(), x, static_cast<int>(d),
+                         ws.cent.get());
     KERNEL_CHECK();
     std::vector<double> csum(static_cast<size_t>(nc) * d);
     HIP_CHECK(hipMemcpyAsync(csum.data(), ws.cent.get(), sizeof(double) * nc * d, hipMemcpyDeviceToHost, s));

@@ -31,8 +31,12 @@ int64_t connected_components_device(const Graph& g, std::vector<int32_t>& labels
 // initialize_embedding with init=spectral (lib.rs:3855-3870, 4355-4376,
 // 4624-4764).  Returns false when the reference would fall back to random
 // init; y (device, n x dim) is written only on success.
+// For a sparse fit x is null and csr holds the rows (the component centroids
+// of the multi-component layout are then summed from its nonzeros, as
+// component_centroids_sparse does, lib.rs:4487-4516).
+struct CsrDevice;
 bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const Graph& g,
                           uint64_t seed, float* y, SpectralWorkspace& ws, hipStream_t s,
-                          SpectralReport* report);
+                          SpectralReport* report, const CsrDevice* csr = nullptr);
 
 }  // namespace umapx
