@@ -143,6 +143,12 @@ def test_auto_ann_dispatch_matches_reference_predicate(umapx):
     p.approx_knn_threshold = 0  # ann_mode="approximate"
     assert f(4200, 96)
     p.approx_knn_threshold = 4096
+    p.n_neighbors = 50  # past the GPU rounds' 32 neighbours, auto keeps the exact kNN
+    assert not f(100_000, 160)
+    p.approx_knn_threshold = 0
+    assert f(100_000, 160)  # forced: the rounds report their limit
+    p.n_neighbors = 15
+    p.approx_knn_threshold = 4096
     p.use_approximate_knn = 0  # ann_mode="exact"
     assert not f(100_000, 160)
 

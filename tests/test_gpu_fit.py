@@ -210,3 +210,29 @@ def test_transform_launch_configuration_determinism(gpu):
     assert np.array_equal(full, m.transform(q))
     Umap(n_epochs=30, random_seed=9, ann_mode="exact").fit_transform(x[:900])
     assert np.array_equal(full, m.transform(q))
+
+
+def test_fit_with_50_neighbours(gpu, oracle):
+    # n_neighbors = 50 (more than the candidate lists' 32): exact kNN on the
+    # vector ALUs, a graph of up to 64 entries per row, and the transforms;
+    # quality within 0.01 of the oracle's over three seeds, both ways
+    from umapers_amd.diagnostics import knn_preservation
+
+    x = datasets.gaussian_mixture(3000, 32, 6, 3.0, 21, 1.0)
+    tg, to, kg, ko = [], [], [], []
+    for s in (1, 2, 3):
+        m = Umap(n_neighbors=50, n_epochs=200, random_seed=s, ann_mode="exact")
+        y = m.fit_transform(x)
+        yo = oracle.fit(x, oracle.OracleParams(n_neighbors=50, n_epochs=200,
+                                               random_seed=s)).embedding
+        tg.append(trustworthiness(x, y))
+        to.append(trustworthiness(x, yo))
+        kg.append(knn_preservation(x, y))
+        ko.append(knn_preservation(x, yo))
+    print(f"k=50 fit: trust gpu {np.mean(tg):.4f} oracle {np.mean(to):.4f}; knn-pres gpu "
+          f"{np.mean(kg):.4f} oracle {np.mean(ko):.4f}", flush=True)
+    assert abs(np.mean(tg) - np.mean(to)) <= 0.01, (tg, to)
+    assert abs(np.mean(kg) - np.mean(ko)) <= 0.01, (kg, ko)
+    q = x[:200] + np.float32(0.05)
+    assert np.isfinite(m.transform(q)).all()
+    assert np.isfinite(m.inverse_transform(y[:20])).all()

@@ -173,3 +173,33 @@ def test_wide_rows_several_score_blocks(gpu, oracle, rng, monkeypatch, metric):
     idx, dist = capi.knn_to_reference(q, x, 15, metric, 1)
     oi, od = oracle.knn_to_reference(q, x, 15, "cosine" if metric else "euclidean")
     assert recall_at_k(idx, oi) >= 0.99
+
+
+BIG_K = [(700, 16, 33), (900, 70, 48), (600, 128, 64), (500, 300, 40)]
+
+
+@pytest.mark.parametrize("n,d,k", BIG_K)
+@pytest.mark.parametrize("metric", [0, 1])
+def test_more_than_32_neighbours_bit_exact(gpu, oracle, rng, n, d, k, metric):
+    # n_neighbors 33-64: the exact distances on the vector ALUs (euclidean
+    # ranked on the squared distance, its eight lane sums for d >= 64;
+    # cosine from the dot product and the rows' norms), bit-exact
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    x[5] = x[9]  # a duplicate pair: a tie resolved by index
+    name = ["euclidean", "cosine"][metric]
+    gi, gd = capi.knn(x, k, metric, 0, 2)
+    oi, od = oracle.knn(x, k, name)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    q = rng.normal(size=(37, d)).astype(np.float32)
+    gi, gd = capi.knn_to_reference(q, x, k, metric)
+    oi, od = oracle.knn_to_reference(q, x, k, name)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+@pytest.mark.parametrize("metric", ["manhattan", "chebyshev", "correlation", "canberra",
+                                    "braycurtis"])
+def test_more_than_32_neighbours_other_metrics(gpu, oracle, rng, metric):
+    x = rng.normal(size=(800, 72)).astype(np.float32)
+    gi, gd = capi.knn_metric(x, x, 50, metric)
+    oi, od = oracle.knn_metric(x, x, 50, metric)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)

@@ -206,6 +206,9 @@ bool should_use_approximate_knn(const umapx_params& p, int64_t n_samples, int64_
   constexpr int64_t kLowDimMaxFeatures = 16, kLowDimMultiplier = 4, kGeneralMultiplier = 4;
   const int64_t threshold = std::max<int64_t>(p.approx_knn_threshold, 0);
   if (!p.use_approximate_knn || n_samples <= threshold) return false;
+  // the GPU's approximate rounds hold at most 32 neighbours: auto mode keeps
+  // the exact kNN above that (a forced approximate run reports the limit)
+  if (p.n_neighbors > 32 && threshold > 0) return false;
   if (threshold > 0) {
     const int64_t mult = n_features <= kLowDimMaxFeatures ? kLowDimMultiplier : kGeneralMultiplier;
     if (n_samples <= threshold * mult) return false;

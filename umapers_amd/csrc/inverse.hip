@@ -302,7 +302,7 @@ __global__ __launch_bounds__(64 * kInvWaves) void inverse_init_kernel(
 // moves are then applied one after another as the reference does.  Each
 // query's edge state (weight, tail, eps and both schedule words) sits in LDS.
 constexpr int kNegBatch = 4;
-constexpr int kInvMaxK = 32;  // the kNN width bound
+constexpr int kInvMaxK = 64;  // the kNN width bound (n_neighbors <= 64)
 
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {

@@ -804,7 +804,9 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
                 float metric_p) {
   maybe_inject_fault("knn");
   if (m == 0) return;
-  if (metric >= 2) {
+  // other metrics, and more neighbours than the candidate lists hold (32):
+  // the exact distances on the vector ALUs, ranked directly
+  if (metric >= 2 || k > 32) {
     knn_metric_device(q, m, r, n, d, k, metric, metric_p, out_idx, out_dist, ws, s);
     return;
   }
@@ -816,7 +818,7 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
     dpad = static_cast<int>((d + 63) / 64 * 64);
   }
   const int kc = knn_candidate_count(k, rerank_factor);
-  if (kc < 0) throw_invalid("n_neighbors must be <= 32 and n_neighbors * rerank_factor <= 64");
+  if (kc < 0) throw_invalid("n_neighbors * rerank_factor must be <= 64 for n_neighbors <= 32");
   if (k > n) throw_invalid("n_neighbors must not exceed the number of reference rows");
   const bool self = (q == r) && (m == n);
   const bool bf16 = precision == 1;

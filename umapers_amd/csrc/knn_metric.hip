@@ -50,7 +50,12 @@ __device__ __forceinline__ float abs_pow(float v, float p) {
 // One feature of one (query, row) pair, in the reference's f32 operations.
 template <int METRIC>
 __device__ __forceinline__ void pair_step(float& acc, float& acc2, float xv, float yv, float p) {
-  if (METRIC == 2) {
+  if (METRIC == 0) {
+    const float df = fsub(xv, yv);
+    acc = fadd(acc, fmul(df, df));
+  } else if (METRIC == 1) {
+    acc = fadd(acc, fmul(xv, yv));
+  } else if (METRIC == 2) {
     acc = fadd(acc, fabsf(fsub(xv, yv)));
   } else if (METRIC == 3) {
     acc = fmaxf(acc, fabsf(fsub(xv, yv)));
@@ -65,6 +70,40 @@ __device__ __forceinline__ void pair_step(float& acc, float& acc2, float xv, flo
     acc = fadd(acc, fabsf(fsub(xv, yv)));
     acc2 = fadd(acc2, fabsf(fadd(xv, yv)));
   }
+}
+
+// One feature's term of the metrics summed in eight lanes for d >= 64:
+// squared euclidean (lib.rs:2440-2492), dot product (lib.rs:2700-2737) and
+// manhattan (lib.rs:2518-2541).
+template <int METRIC>
+__device__ __forceinline__ float lane_term(float x, float y) {
+  if (METRIC == 0) {
+    const float df = fsub(x, y);
+    return fmul(df, df);
+  }
+  if (METRIC == 1) return fmul(x, y);
+  return fabsf(fsub(x, y));
+}
+
+// L2 norms of rows, the reference's dot product of a row with itself then
+// sqrt (eight lane sums for d >= 64; lib.rs:2700-2765).
+__global__ void row_norms_kernel(const float* __restrict__ x, int64_t rows, int d,
+                                 float* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const float* r = x + i * d;
+  float s = 0.f;
+  if (d >= 64) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int f = 0;
+    for (; f + 8 <= d; f += 8)
+      for (int l = 0; l < 8; ++l) acc[l] = fadd(acc[l], fmul(r[f + l], r[f + l]));
+    for (int l = 0; l < 8; ++l) s = fadd(s, acc[l]);
+    for (; f < d; ++f) s = fadd(s, fmul(r[f], r[f]));
+  } else {
+    for (int f = 0; f < d; ++f) s = fadd(s, fmul(r[f], r[f]));
+  }
+  out[i] = fsqrt(s);
 }
 
 // Per-row sum and sum of squares in feature order (the row halves of
@@ -83,8 +122,10 @@ __global__ void row_summaries_kernel(const float* __restrict__ x, int64_t rows, 
   sq[i] = q;
 }
 
-// METRIC: 2 manhattan, 3 chebyshev, 4 minkowski, 5 correlation, 6 canberra,
-// 7 braycurtis.  H: kept candidates per query (>= k).  QG: the transposed
+// METRIC: 0 euclidean (squared, the root taken at output), 1 cosine, 2
+// manhattan, 3 chebyshev, 4 minkowski, 5 correlation, 6 canberra, 7
+// braycurtis.  H: kept candidates per query (>= k; 64 serves n_neighbors of
+// 33-64 under every metric, euclidean and cosine included).  QG: the transposed
 // query tile lives in the block's slice of qglob (HBM, read through the
 // caches) instead of LDS, for rows too wide for both tiles in LDS.
 template <int METRIC, int H, bool QG>
@@ -119,6 +160,7 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
     xs = qsum[i];
     xq = qsq[i];
   }
+  if (METRIC == 1 && act) xq = qsq[i];  // the query row's L2 norm
   int keys[H];
   int32_t ids[H];
 #pragma unroll
@@ -136,7 +178,7 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
     }
     __syncthreads();
     float dist[kMetricRows];
-    if (METRIC == 2 && d >= 64 && v4) {
+    if ((METRIC == 0 || METRIC == 1 || METRIC == 2) && d >= 64 && v4) {
       // the eight lane sums as two float4 halves of each 8-feature block,
       // four rows at a time; the lanes added in order, then the tail
       constexpr int kPass = 4;
@@ -154,14 +196,14 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
 #pragma unroll
           for (int t = 0; t < kPass; ++t) {
             const float4 ya = rt4[(t0 + t) * d4 + 2 * b], yb = rt4[(t0 + t) * d4 + 2 * b + 1];
-            lo[t].x = fadd(lo[t].x, fabsf(fsub(xa.x, ya.x)));
-            lo[t].y = fadd(lo[t].y, fabsf(fsub(xa.y, ya.y)));
-            lo[t].z = fadd(lo[t].z, fabsf(fsub(xa.z, ya.z)));
-            lo[t].w = fadd(lo[t].w, fabsf(fsub(xa.w, ya.w)));
-            hi[t].x = fadd(hi[t].x, fabsf(fsub(xb.x, yb.x)));
-            hi[t].y = fadd(hi[t].y, fabsf(fsub(xb.y, yb.y)));
-            hi[t].z = fadd(hi[t].z, fabsf(fsub(xb.z, yb.z)));
-            hi[t].w = fadd(hi[t].w, fabsf(fsub(xb.w, yb.w)));
+            lo[t].x = fadd(lo[t].x, lane_term<METRIC>(xa.x, ya.x));
+            lo[t].y = fadd(lo[t].y, lane_term<METRIC>(xa.y, ya.y));
+            lo[t].z = fadd(lo[t].z, lane_term<METRIC>(xa.z, ya.z));
+            lo[t].w = fadd(lo[t].w, lane_term<METRIC>(xa.w, ya.w));
+            hi[t].x = fadd(hi[t].x, lane_term<METRIC>(xb.x, yb.x));
+            hi[t].y = fadd(hi[t].y, lane_term<METRIC>(xb.y, yb.y));
+            hi[t].z = fadd(hi[t].z, lane_term<METRIC>(xb.z, yb.z));
+            hi[t].w = fadd(hi[t].w, lane_term<METRIC>(xb.w, yb.w));
           }
         }
 #pragma unroll
@@ -178,15 +220,15 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
           if (nb8 * 2 < d4) {  // a 4-feature tail
             const float4 xa = qt4[(2 * nb8) * 64 + lane];
             const float4 ya = rt4[(t0 + t) * d4 + 2 * nb8];
-            sum = fadd(sum, fabsf(fsub(xa.x, ya.x)));
-            sum = fadd(sum, fabsf(fsub(xa.y, ya.y)));
-            sum = fadd(sum, fabsf(fsub(xa.z, ya.z)));
-            sum = fadd(sum, fabsf(fsub(xa.w, ya.w)));
+            sum = fadd(sum, lane_term<METRIC>(xa.x, ya.x));
+            sum = fadd(sum, lane_term<METRIC>(xa.y, ya.y));
+            sum = fadd(sum, lane_term<METRIC>(xa.z, ya.z));
+            sum = fadd(sum, lane_term<METRIC>(xa.w, ya.w));
           }
           dist[t0 + t] = sum;
         }
       }
-    } else if (METRIC == 2 && d >= 64) {
+    } else if ((METRIC == 0 || METRIC == 1 || METRIC == 2) && d >= 64) {
       // eight lane sums over the full 8-feature blocks, added in lane order,
       // then the tail one feature at a time
       float tot[kMetricRows];
@@ -200,7 +242,7 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
           const int f = 8 * b + l;
           const float xv = qt[f * 64 + lane];
 #pragma unroll
-          for (int t = 0; t < kMetricRows; ++t) acc[t] = fadd(acc[t], fabsf(fsub(xv, rt[t * d + f])));
+          for (int t = 0; t < kMetricRows; ++t) acc[t] = fadd(acc[t], lane_term<METRIC>(xv, rt[t * d + f]));
         }
 #pragma unroll
         for (int t = 0; t < kMetricRows; ++t) tot[t] = fadd(tot[t], acc[t]);
@@ -208,7 +250,7 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
       for (int f = 8 * nb; f < d; ++f) {
         const float xv = qt[f * 64 + lane];
 #pragma unroll
-        for (int t = 0; t < kMetricRows; ++t) tot[t] = fadd(tot[t], fabsf(fsub(xv, rt[t * d + f])));
+        for (int t = 0; t < kMetricRows; ++t) tot[t] = fadd(tot[t], lane_term<METRIC>(xv, rt[t * d + f]));
       }
 #pragma unroll
       for (int t = 0; t < kMetricRows; ++t) dist[t] = tot[t];
@@ -264,7 +306,20 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
         }
       }
     }
-    // top-H by (distance total order, index), rows in increasing index order
+    if (METRIC == 1) {  // cosine_distance_with_norms (lib.rs:2739-2765) on the dot
+#pragma unroll
+      for (int t = 0; t < kMetricRows; ++t) {
+        const float ny = rsq[r0 + (t < rows ? t : 0)];
+        if (xq == 0.f && ny == 0.f)
+          dist[t] = 0.f;
+        else if (xq == 0.f || ny == 0.f)
+          dist[t] = 1.f;
+        else
+          dist[t] = fsub(1.f, clamp_pm1(fdiv(dist[t], fmul(xq, ny))));
+      }
+    }
+    // top-H by (distance total order, index), rows in increasing index order;
+    // euclidean ranks the squared distances (lib.rs:2829-2834)
 #pragma unroll
     for (int t = 0; t < kMetricRows; ++t) {
       if (t >= rows) break;
@@ -293,7 +348,7 @@ __global__ __launch_bounds__(64) void knn_metric_kernel(
   for (int s = 0; s < H; ++s)
     if (s < k) {
       out_idx[i * k + s] = ids[s];
-      out_dist[i * k + s] = key_to_float(keys[s]);
+      out_dist[i * k + s] = METRIC == 0 ? fsqrt(key_to_float(keys[s])) : key_to_float(keys[s]);
     }
 }
 
@@ -323,6 +378,8 @@ void dispatch_metric(int metric, const float* q, int64_t m, const float* r, int6
                      const float* rq, int32_t* out_idx, float* out_dist, float* qg,
                      hipStream_t s) {
   switch (metric) {
+    case 0: launch_metric<0, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
+    case 1: launch_metric<1, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
     case 2: launch_metric<2, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
     case 3: launch_metric<3, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
     case 4: launch_metric<4, H>(q, m, r, n, d, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s); break;
@@ -337,10 +394,11 @@ void dispatch_metric(int metric, const float* q, int64_t m, const float* r, int6
 void knn_metric_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d, int k,
                        int metric, float p, int32_t* out_idx, float* out_dist, KnnWorkspace& ws,
                        hipStream_t s) {
-  if (metric < 2 || metric > 7) throw_invalid("unknown metric code");
+  if (metric < 0 || metric > 7) throw_invalid("unknown metric code");
   if (d < 1 || d > kMetricMaxFeatures)
-    throw_invalid("exact GPU kNN under this metric supports at most 2560 features");
-  if (k < 1 || k > 32 || k > n) throw_invalid("n_neighbors must be <= 32 on the GPU path");
+    throw_invalid(metric < 2 ? "exact GPU kNN with n_neighbors > 32 supports at most 2560 features"
+                             : "exact GPU kNN under this metric supports at most 2560 features");
+  if (k < 1 || k > 64 || k > n) throw_invalid("n_neighbors must be <= 64 on the GPU path");
   if (n >= (int64_t{1} << 31)) throw_invalid("exact GPU kNN supports fewer than 2^31 rows");
   if (m == 0) return;
   const float* qs = nullptr;
@@ -359,6 +417,16 @@ void knn_metric_device(const float* q, int64_t m, const float* r, int64_t n, int
     rq = base + n;
     qs = base + 2 * n;
     qq = base + 2 * n + m;
+  } else if (metric == 1) {  // L2 norms of both row sets
+    ws.msum.ensure(static_cast<size_t>(n + m));
+    float* base = ws.msum.get();
+    hipLaunchKernelGGL(row_norms_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, r, n,
+                       static_cast<int>(d), base);
+    hipLaunchKernelGGL(row_norms_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, s, q, m,
+                       static_cast<int>(d), base + n);
+    KERNEL_CHECK();
+    rq = base;
+    qq = base + n;
   }
   const int dd = static_cast<int>(d);
   float* qg = nullptr;  // per-block transposed query tiles of wide rows
@@ -368,8 +436,10 @@ void knn_metric_device(const float* q, int64_t m, const float* r, int64_t n, int
   }
   if (k <= 16)
     dispatch_metric<16>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s);
-  else
+  else if (k <= 32)
     dispatch_metric<32>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s);
+  else
+    dispatch_metric<64>(metric, q, m, r, n, dd, k, p, qs, qq, rs, rq, out_idx, out_dist, qg, s);
 }
 
 }  // namespace umapx

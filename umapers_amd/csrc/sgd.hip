@@ -521,13 +521,15 @@ void sgd_epoch2_kernel(
           float2 ytl = yt;
           float2 dh = make_float2(0.f, 0.f), dt = make_float2(0.f, 0.f);
           if (MIRROR) {
-            if (h < t) {
-              pull2(P, cur, ytl, h, t, dh, dt);
-              pull2(P, ytl, cur, t, h, dt, dh);
-            } else {
-              pull2(P, ytl, cur, t, h, dt, dh);
-              pull2(P, cur, ytl, h, t, dh, dt);
-            }
+            // row min(h, t)'s pull first; the roles are swapped by selects,
+            // not branches, so lanes of both orders share one instruction
+            // stream
+            const bool hf = h < t;
+            float2 ya = hf ? cur : ytl, yb = hf ? ytl : cur;
+            float2 da = dh, db = dt;
+            pull2(P, ya, yb, hf ? h : t, hf ? t : h, da, db);
+            pull2(P, yb, ya, hf ? t : h, hf ? h : t, db, da);
+            cur = hf ? ya : yb;
           } else {
             pull2(P, cur, ytl, h, t, dh, dt);
             dt_sum = dt;
