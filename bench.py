@@ -143,6 +143,13 @@ def leg_learned(lib, L, dev, m_q: int, k: int) -> dict:
         torch.cuda.synchronize()
         tl.append(time.perf_counter() - t0)
     mlp_sec = float(np.median(tl))
+    km = []
+    for _ in range(5):
+        L.check(lib.umapx_transform_learned_device(*args_l))
+        ms = ctypes.c_double()
+        L.check(lib.umapx_get_learned_kernel_ms(h5, ctypes.byref(ms)))
+        km.append(ms.value)
+    kernel_ms = float(np.median(km))
     check_rows = np.ascontiguousarray(q5[:10000])
     y_mlp = yq5_d[:10000].cpu().numpy()
     y_plain = np.empty((check_rows.shape[0], 2), np.float32)
@@ -152,13 +159,20 @@ def leg_learned(lib, L, dev, m_q: int, k: int) -> dict:
     per_row = np.linalg.norm(y_mlp - y_plain, axis=1)
     extent = float(np.ptp(y5, axis=0).max())
     flops_row = 2.0 * (192 * 256 + 256 * 256 + 256 * 2)
+    bytes_row = 192 * 4 + 2 * 4  # f32 row in, f32 z out
     mlp = {"workload": "c5: 100000x192 fit (20 clusters, seed 31), encoder 192-256-256-2 "
                        f"trained {steps.value} steps x 4096 edges x 5 negatives; {m_q} fresh "
                        "queries (seed 32), device resident; plain transform of the first 10000 as the check",
            "fit_sec_incl_training": round(fit5, 3),
            "train_loss_first_last": [round(first.value, 4), round(last.value, 4)],
            "learned_rows_per_sec": round(m_q / mlp_sec, 1),
-           "learned_tflops": round(flops_row * m_q / mlp_sec / 1e12, 2),
+           "learned_kernel_ms": round(kernel_ms, 4),
+           "learned_kernel_rows_per_sec": round(m_q / (kernel_ms * 1e-3), 1),
+           "learned_tflops": round(flops_row * m_q / (kernel_ms * 1e-3) / 1e12, 2),
+           "learned_hbm_gbs": round(bytes_row * m_q / (kernel_ms * 1e-3) / 1e9, 1),
+           # time at the roofline (the larger of the MFMA and HBM floors) / measured
+           "learned_roofline_frac": round(max(flops_row * m_q / 2.5e15, bytes_row * m_q / 8e12)
+                                          / (kernel_ms * 1e-3), 4),
            "plain_check_rows_per_sec": round(check_rows.shape[0] / plain_sec, 1),
            "per_row_dist_median": round(float(np.median(per_row)), 4),
            "per_row_dist_p90": round(float(np.percentile(per_row, 90)), 4),
@@ -169,6 +183,32 @@ def leg_learned(lib, L, dev, m_q: int, k: int) -> dict:
     lib.umapx_destroy(h5)
     del q5_d, yq5_d
     return mlp
+
+
+def leg_encoder(lib, L, m_q: int) -> dict:
+    """The fused encoder kernel alone over m_q rows of the learned-transform
+    query distribution, with He-initialised 192-256-256-2 weights (no fit, no
+    training): the shortest run that holds the kernel, for counter passes."""
+    from umapers_amd import datasets
+
+    d, h, dim = 192, 256, 2
+    _, xq = datasets.c5(n_fit=1, n_query=m_q)
+    rng = np.random.default_rng(5)
+    dp = (d + 31) // 32 * 32
+    sizes = [dp * h, h, h * h, h, h * dim, dim]
+    fans = [d, 0, h, 0, h, 0]
+    parts = [(rng.uniform(-1, 1, sz) * np.sqrt(6.0 / f)) if f else np.zeros(sz) for sz, f in zip(sizes, fans)]
+    parts[0].reshape(dp, h)[d:] = 0.0
+    params = np.concatenate(parts).astype(np.float32)
+    mean = xq.mean(0).astype(np.float32)
+    inv = (1.0 / xq.std(0)).astype(np.float32)
+    z = np.empty((m_q, dim), np.float32)
+    t0 = time.perf_counter()
+    L.check(lib.umapx_mlp_forward(d, h, dim, L.ptr(params), L.ptr(mean), L.ptr(inv), L.ptr(xq), m_q, 1,
+                                  L.ptr(z)))
+    return {"workload": f"fused encoder 192-256-256-2 over {m_q} rows (host buffers, one call)",
+            "call_sec_incl_copies": round(time.perf_counter() - t0, 4),
+            "z_finite": bool(np.isfinite(z).all())}
 
 
 def leg_k30(lib, L, dev, n4: int, epochs: int) -> dict:
@@ -243,7 +283,7 @@ def main() -> int:
                     help="1: the cosine variant of the workload (rows L2-normalised), both precisions")
     ap.add_argument("--c4-rows", type=int, default=1_000_000,
                     help="rows of the k=30 SGD-dominated leg (1M x 64 mixture; 0 skips it)")
-    ap.add_argument("--only", choices=("k30", "learned"), default=None,
+    ap.add_argument("--only", choices=("k30", "learned", "encoder"), default=None,
                     help="run only that leg and print its figures as one JSON line (for "
                          "profiling one configuration)")
     ap.add_argument("--cpu-full", action="store_true",
@@ -269,8 +309,12 @@ def main() -> int:
     n, k, epochs = args.n, 15, 200
     if args.only:
         dev = torch.device("cuda:0")
-        res = (leg_k30(lib, L, dev, args.c4_rows, epochs) if args.only == "k30"
-               else leg_learned(lib, L, dev, args.mlp_queries, k))
+        if args.only == "k30":
+            res = leg_k30(lib, L, dev, args.c4_rows, epochs)
+        elif args.only == "encoder":
+            res = leg_encoder(lib, L, args.mlp_queries)
+        else:
+            res = leg_learned(lib, L, dev, args.mlp_queries, k)
         print(json.dumps(res), flush=True)
         return 0
     log(f"bench: generating {n} x 128 rows")

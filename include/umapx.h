@@ -322,6 +322,17 @@ int32_t umapx_mlp_batch_grad(int32_t d, int32_t hidden, int32_t dim, const float
                              const float* mean, const float* inv_std, const float* x_rows,
                              const float* y_rows, int32_t n_edges, int32_t n_neg, float a,
                              float b, float mse_weight, float* grads, float* loss, float* z_out);
+/* The encoder's forward pass of given parameters over host rows x
+ * [rows x d] into z [rows x dim]: fused = 0 runs the unfused layer chain
+ * that training uses, fused = 1 the single inference kernel that
+ * umapx_transform_learned runs (mlp_hidden 64, 128, 192 or 256, d % 4 == 0,
+ * d <= 1024).  For checking the two against each other and a host forward. */
+int32_t umapx_mlp_forward(int32_t d, int32_t hidden, int32_t dim, const float* params,
+                          const float* mean, const float* inv_std, const float* x, int64_t rows,
+                          int32_t fused, float* z);
+/* HIP-event time (ms, on the handle's stream) of the encoder kernels of the
+ * last learned transform; training excluded. */
+int32_t umapx_get_learned_kernel_ms(const umapx_handle* h, double* ms);
 /* Training loss of the learned transform at its first and last step. */
 int32_t umapx_get_mlp_loss(const umapx_handle* h, double* first, double* last, int32_t* steps);
 

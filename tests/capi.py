@@ -139,6 +139,14 @@ def mlp_batch_grad(params, mean, inv_std, x_rows, y_rows, d, h, dim, n_edges, n_
     return loss.value, grads, z
 
 
+def mlp_forward(params, mean, inv_std, x, d, h, dim, fused):
+    params, mean, inv_std, x = _f32(params), _f32(mean), _f32(inv_std), _f32(x)
+    z = np.empty((x.shape[0], dim), np.float32)
+    L.check(L.lib().umapx_mlp_forward(d, h, dim, L.ptr(params), L.ptr(mean), L.ptr(inv_std),
+                                      L.ptr(x), x.shape[0], 1 if fused else 0, L.ptr(z)))
+    return z
+
+
 def inverse_init(x_train, y_train, q, k, a, b):
     x_train, y_train, q = _f32(x_train), _f32(y_train), _f32(q)
     n, d = x_train.shape

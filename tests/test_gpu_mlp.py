@@ -54,6 +54,51 @@ def test_batch_gradients_match_fp64_oracle(gpu, d, h, dim):
     assert np.all(g[:L["w1"][1]].reshape(L["dp"], h)[d:] == 0)
 
 
+@pytest.mark.parametrize("d,h,dim,rows", [(192, 256, 2, 70001), (192, 256, 2, 1), (100, 256, 1, 33),
+                                           (64, 128, 3, 999), (36, 64, 8, 70), (1024, 192, 5, 257)])
+def test_fused_encoder_matches_chain_and_oracle(gpu, d, h, dim, rows):
+    """The fused inference kernel against the unfused chain (the training
+    forward) and the FP64 oracle of the BF16 network, on ragged row counts
+    (one row, partial 32-row tiles, more tiles than one pass of the grid) and
+    input widths that end inside a lane's 8-input step."""
+    rng = np.random.default_rng(d + h + dim + rows)
+    L = M.layout(d, h, dim)
+    p = np.zeros(L["total"], np.float32)
+    for name, fan in (("w1", d), ("w2", h), ("w3", h)):
+        o, size = L[name]
+        p[o:o + size] = rng.uniform(-1, 1, size) * np.sqrt(6.0 / fan)
+    p[:L["w1"][1]].reshape(L["dp"], h)[d:] = 0.0
+    for name in ("b1", "b2", "b3"):
+        o, size = L[name]
+        p[o:o + size] = rng.normal(0, 0.05, size)
+    x = rng.normal(1.5, 2, size=(rows, d)).astype(np.float32)
+    mean = (x.mean(0) + 0.1).astype(np.float32)
+    inv = (1.0 / (x.std(0) + 0.5)).astype(np.float32)
+    z_f = capi.mlp_forward(p, mean, inv, x, d, h, dim, fused=True)
+    z_c = capi.mlp_forward(p, mean, inv, x, d, h, dim, fused=False)
+    zo, _ = M.forward(p, x, mean, inv, d, h, dim, emulate_bf16=True)
+    tol = 2e-3 * np.abs(zo).max()
+    assert np.isfinite(z_f).all()
+    # FP32 accumulation in a different order: a hidden value may round to the
+    # neighbouring BF16 value, so elementwise closeness, not equality
+    assert np.abs(z_f - zo).max() <= tol, np.abs(z_f - zo).max() / np.abs(zo).max()
+    assert np.abs(z_f - z_c).max() <= tol
+    assert np.median(np.abs(z_f - zo)) <= 1e-4 * np.abs(zo).max()
+
+
+def test_fused_encoder_limits(gpu):
+    """Widths the fused kernel does not take are refused by the test entry
+    point (the transform itself runs the unfused chain for them)."""
+    d, h, dim = 37, 64, 2
+    L = M.layout(d, h, dim)
+    p = np.zeros(L["total"], np.float32)
+    x = np.ones((4, d), np.float32)
+    with pytest.raises(ValueError, match="fused encoder"):
+        capi.mlp_forward(p, np.zeros(d, np.float32), np.ones(d, np.float32), x, d, h, dim, fused=True)
+    z = capi.mlp_forward(p, np.zeros(d, np.float32), np.ones(d, np.float32), x, d, h, dim, fused=False)
+    assert z.shape == (4, dim)
+
+
 def test_learned_transform_trains_and_gains_over_plain(gpu):
     x_fit, x_q = datasets.c5(n_fit=20000, n_query=100000)
     judge = x_q[:2000]

@@ -178,6 +178,10 @@ SIGNATURES = {
         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p,
         ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p,
         _f32p, _f32p]),
+    "umapx_get_learned_kernel_ms": (ctypes.c_int32, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "umapx_mlp_forward": (ctypes.c_int32, [
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, ctypes.c_int64,
+        ctypes.c_int32, _f32p]),
     "umapx_get_mlp_loss": (ctypes.c_int32, [_vp, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_int32)]),

@@ -319,6 +319,7 @@ struct umapx_handle {
   SpectralWorkspace spws;
   Graph graph;
   MlpModel mlp;
+  double learned_kernel_ms = 0.0;  // HIP-event time of the last learned transform's kernels
   umapx_timings timings{};
   bool count_schedule = false;  // set by the profile entry point
 
@@ -690,8 +691,23 @@ struct umapx_handle {
       if (!mlp.trained) {
         mlp_train(mlp, x.get(), n, d, y.get(), p.n_components, graph, a, b, p, stream);
       }
+      hipEvent_t ev[2] = {nullptr, nullptr};
+      HIP_CHECK(hipEventCreate(&ev[0]));
+      struct EventGuard {
+        hipEvent_t* e;
+        ~EventGuard() {
+          for (int i = 0; i < 2; ++i)
+            if (e[i]) (void)hipEventDestroy(e[i]);
+        }
+      } ev_guard{ev};
+      HIP_CHECK(hipEventCreate(&ev[1]));
+      HIP_CHECK(hipEventRecord(ev[0], stream));
       mlp_forward_rows(mlp, d_q, m, d_out, stream);
+      HIP_CHECK(hipEventRecord(ev[1], stream));
       HIP_CHECK(hipStreamSynchronize(stream));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      learned_kernel_ms = ms;
     } else {
       transform_plain_device(d_q, m, d_out);
     }
@@ -997,6 +1013,15 @@ int32_t umapx_mlp_batch_grad(int32_t d, int32_t hidden, int32_t dim, const float
   });
 }
 
+int32_t umapx_mlp_forward(int32_t d, int32_t hidden, int32_t dim, const float* params,
+                          const float* mean, const float* inv_std, const float* x, int64_t rows,
+                          int32_t fused, float* z) {
+  return guarded([&] {
+    if (!params || !mean || !inv_std || !x || !z) throw_invalid("null argument");
+    mlp_forward_host(d, hidden, dim, params, mean, inv_std, x, rows, fused != 0, z);
+  });
+}
+
 int32_t umapx_get_mlp_loss(const umapx_handle* h, double* first, double* last, int32_t* steps) {
   return guarded([&] {
     require_handle(h);
@@ -1004,6 +1029,14 @@ int32_t umapx_get_mlp_loss(const umapx_handle* h, double* first, double* last, i
     if (first) *first = h->mlp.first_loss;
     if (last) *last = h->mlp.last_loss;
     if (steps) *steps = h->mlp.steps_done;
+  });
+}
+
+int32_t umapx_get_learned_kernel_ms(const umapx_handle* h, double* ms) {
+  return guarded([&] {
+    require_handle(h);
+    if (!ms) throw_invalid("null argument");
+    *ms = h->learned_kernel_ms;
   });
 }
 

@@ -565,6 +565,8 @@ void check_shape(int d, int h, int dim) {
 
 }  // namespace
 
+void pack_model(MlpModel& m, hipStream_t s);
+
 void mlp_train(MlpModel& m, const float* x, int64_t n, int64_t d, const float* y, int dim,
                const Graph& g, float a, float b, const umapx_params& p, hipStream_t s) {
   maybe_inject_fault("mlp_train");
@@ -660,14 +662,304 @@ void mlp_train(MlpModel& m, const float* x, int64_t n, int64_t d, const float* y
       m.last_loss = l;
     }
   }
+  pack_model(m, s);
   HIP_CHECK(hipStreamSynchronize(s));
   m.steps_done = p.mlp_steps;
   m.trained = true;
 }
 
-void mlp_forward_rows(const MlpModel& m, const float* x, int64_t rows, float* out, hipStream_t s) {
-  if (!m.trained) throw Error(kNotFitted, "the learned transform has not been trained");
-  const MlpShape& sh = m.shape;
+namespace {
+
+// ---------------------------------------------------------------------------
+// Fused inference: raw f32 rows in, z out, in one kernel.
+//
+// Each wave carries 32 rows through all three layers in the transposed form
+// H^T = W^T X^T: the weights are the MFMA's A operand (32 output features x
+// 16 inputs) and the rows its B operand (16 inputs x 32 rows), so a lane
+// ends a layer holding one row's outputs for 16 features of every 32-feature
+// tile: features 8q + 4g + r (q, r = 0..3) of the tile, g = lane / 32.  The
+// sum inside one MFMA step does not depend on which 8 inputs a lane supplies,
+// as long as the A and B operands agree; so the next layer takes that
+// register footprint as its B operand unchanged (step 2t + h uses the lane's
+// values 8h .. 8h + 7 of tile t) and its weights are packed in the matching
+// input order.  H1 and H2 never leave the registers; the only HBM traffic is
+// the rows in (4 d bytes) and z out (4 dim bytes).
+//
+// W2's packed fragments (H^2 BF16 values), the biases, W3 (f32 copies of the
+// BF16 values) and the standardisation live in LDS; W1's fragments (dp H
+// values) are read through the caches, one 16-byte vector per lane per MFMA.
+// Layer 1 takes inputs 16 s + 8 g .. + 7 at step s, so the two lanes of a row
+// read the two halves of one 64-byte piece of it (one cache line per row per
+// load, where a lane-contiguous order touched two).
+// The arithmetic is the unfused chain's: standardise in f32, round to BF16,
+// FP32 accumulation, bias + ReLU rounded to BF16 for H1 and H2, an f32 dot
+// product with W3 (only the order of the f32 sums differs).
+constexpr int kFusedMaxDp = 1024;
+constexpr int kFusedRowsPerWave = 32;
+constexpr int kFusedWaves = 8;
+
+// Packed weights (g = l / 32, ks1 = dp / 16, ks2 = h / 16, p(s, g, j) =
+// 32 (s / 2) + 16 (s % 2) + 8 (j / 4) + 4 g + j % 4, the input order of the
+// register footprint):
+//   w1p[((o ks1 + s) 64 + l) 8 + j] = W1[16 s + 8 g + j][32 o + l % 32]
+//   w2p[((o ks2 + s) 64 + l) 8 + j] = W2[p(s, g, j)][32 o + l % 32]
+//   w3p[(s 64 + l) 8 + j]          = W3[p(s, g, j)][l % 32], 0 for l % 32 >= dim
+__global__ void mlp_pack_kernel(const uint16_t* __restrict__ wh, MlpShape sh,
+                                uint16_t* __restrict__ w1p, uint16_t* __restrict__ w2p,
+                                uint16_t* __restrict__ w3p) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n1 = static_cast<int64_t>(sh.dp) * sh.h, n2 = static_cast<int64_t>(sh.h) * sh.h;
+  const int64_t n3 = static_cast<int64_t>(sh.h) * 32;
+  const int ks1 = sh.dp / 16, ks2 = sh.h / 16;
+  if (i < n1) {
+    const int j = static_cast<int>(i & 7), l = static_cast<int>((i >> 3) & 63);
+    const int64_t os = i >> 9;
+    const int o = static_cast<int>(os / ks1), s = static_cast<int>(os % ks1);
+    const int in = 16 * s + 8 * (l >> 5) + j;
+    w1p[i] = wh[sh.off_w1 + static_cast<int64_t>(in) * sh.h + 32 * o + (l & 31)];
+  } else if (i < n1 + n2) {
+    const int64_t k = i - n1;
+    const int j = static_cast<int>(k & 7), l = static_cast<int>((k >> 3) & 63);
+    const int64_t os = k >> 9;
+    const int o = static_cast<int>(os / ks2), s = static_cast<int>(os % ks2);
+    const int in = 32 * (s >> 1) + 16 * (s & 1) + 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
+    w2p[k] = wh[sh.off_w2 + static_cast<int64_t>(in) * sh.h + 32 * o + (l & 31)];
+  } else if (i < n1 + n2 + n3) {
+    const int64_t k = i - n1 - n2;
+    const int j = static_cast<int>(k & 7), l = static_cast<int>((k >> 3) & 63);
+    const int s = static_cast<int>(k >> 9);
+    const int in = 32 * (s >> 1) + 16 * (s & 1) + 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
+    const int c = l & 31;
+    w3p[k] = c < sh.dim ? wh[sh.off_w3 + static_cast<int64_t>(in) * sh.dim + c] : uint16_t{0};
+  }
+}
+
+// One workgroup of kFusedWaves waves per CU; wave w of block b takes the
+// 32-row tiles b * kFusedWaves + w, then every gridDim.x * kFusedWaves-th.
+// Requires d % 4 == 0, x 16-byte aligned, dp <= kFusedMaxDp, dim <= kMaxDim.
+template <int H>
+__global__ __launch_bounds__(kFusedWaves * 64) void mlp_infer_kernel(
+    const float* __restrict__ x, int64_t rows, int d, int dp, int dim,
+    const uint16_t* __restrict__ w1p, const uint16_t* __restrict__ w2p,
+    const float* __restrict__ b1, const float* __restrict__ b2, const uint16_t* __restrict__ w3p,
+    const float* __restrict__ b3, const float* __restrict__ mean,
+    const float* __restrict__ inv_std, float* __restrict__ z) {
+  constexpr int T = H / 32, KS2 = H / 16;
+  __shared__ u32x4 w2s[T * KS2 * 64];
+  __shared__ __align__(16) float b1s[H];
+  __shared__ __align__(16) float b2s[H];
+  __shared__ u32x4 w3s[KS2 * 64];
+  __shared__ __align__(16) float mus[kFusedMaxDp];
+  __shared__ __align__(16) float iss[kFusedMaxDp];
+  for (int i = threadIdx.x; i < T * KS2 * 64; i += blockDim.x)
+    w2s[i] = reinterpret_cast<const u32x4*>(w2p)[i];
+  for (int i = threadIdx.x; i < H; i += blockDim.x) {
+    b1s[i] = b1[i];
+    b2s[i] = b2[i];
+  }
+  for (int i = threadIdx.x; i < KS2 * 64; i += blockDim.x)
+    w3s[i] = reinterpret_cast<const u32x4*>(w3p)[i];
+  for (int i = threadIdx.x; i < dp; i += blockDim.x) {
+    mus[i] = i < d ? mean[i] : 0.f;
+    iss[i] = i < d ? inv_std[i] : 0.f;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, g = lane >> 5, r32 = lane & 31;
+  const int ks1 = dp >> 4;
+  const u32x4* w1v = reinterpret_cast<const u32x4*>(w1p) + lane;
+  const int64_t n_tiles = (rows + kFusedRowsPerWave - 1) / kFusedRowsPerWave;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kFusedWaves;
+  const float4 f4zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kFusedWaves + (threadIdx.x >> 6);
+       t < n_tiles; t += stride) {
+    const int64_t row = t * kFusedRowsPerWave + r32;
+    const bool live = row < rows;
+    const float* xr = x + (live ? row : 0) * static_cast<int64_t>(d) + 8 * g;
+    const int fmax = live ? d - 8 * g : 0;  // inputs at or past this lane's first that exist
+    floatx16 acc[T];
+#pragma unroll
+    for (int o = 0; o < T; ++o)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[o][i] = 0.f;
+
+    // ---- layer 1: acc = W1^T x^T, x standardised and rounded to BF16 ----
+    float4 xa_lo = f4zero, xa_hi = f4zero, xb_lo = f4zero, xb_hi = f4zero;
+    if (0 < fmax) xa_lo = *reinterpret_cast<const float4*>(xr);
+    if (4 < fmax) xa_hi = *reinterpret_cast<const float4*>(xr + 4);
+    if (ks1 > 1) {
+      if (16 < fmax) xb_lo = *reinterpret_cast<const float4*>(xr + 16);
+      if (20 < fmax) xb_hi = *reinterpret_cast<const float4*>(xr + 20);
+    }
+    // W1 fragments of the next step, read through the caches (every wave of
+    // the chip reads the same dp H values)
+    u32x4 wa[T];
+#pragma unroll
+    for (int o = 0; o < T; ++o) wa[o] = w1v[o * ks1 * 64];
+    for (int s = 0; s < ks1; ++s) {
+      const int f = 16 * s + 8 * g;
+      const float4 m_lo = *reinterpret_cast<const float4*>(&mus[f]);
+      const float4 m_hi = *reinterpret_cast<const float4*>(&mus[f + 4]);
+      const float4 i_lo = *reinterpret_cast<const float4*>(&iss[f]);
+      const float4 i_hi = *reinterpret_cast<const float4*>(&iss[f + 4]);
+      u32x4 xv;
+      xv[0] = static_cast<uint32_t>(to_bf16((xa_lo.x - m_lo.x) * i_lo.x)) |
+              (static_cast<uint32_t>(to_bf16((xa_lo.y - m_lo.y) * i_lo.y)) << 16);
+      xv[1] = static_cast<uint32_t>(to_bf16((xa_lo.z - m_lo.z) * i_lo.z)) |
+              (static_cast<uint32_t>(to_bf16((xa_lo.w - m_lo.w) * i_lo.w)) << 16);
+      xv[2] = static_cast<uint32_t>(to_bf16((xa_hi.x - m_hi.x) * i_hi.x)) |
+              (static_cast<uint32_t>(to_bf16((xa_hi.y - m_hi.y) * i_hi.y)) << 16);
+      xv[3] = static_cast<uint32_t>(to_bf16((xa_hi.z - m_hi.z) * i_hi.z)) |
+              (static_cast<uint32_t>(to_bf16((xa_hi.w - m_hi.w) * i_hi.w)) << 16);
+      // inputs two steps ahead (rows are read once, from HBM)
+      xa_lo = xb_lo;
+      xa_hi = xb_hi;
+      xb_lo = f4zero;
+      xb_hi = f4zero;
+      const int f2 = 16 * (s + 2);
+      if (s + 2 < ks1) {
+        if (f2 < fmax) xb_lo = *reinterpret_cast<const float4*>(xr + f2);
+        if (f2 + 4 < fmax) xb_hi = *reinterpret_cast<const float4*>(xr + f2 + 4);
+      }
+      const bf16x8 xf = __builtin_bit_cast(bf16x8, xv);
+#pragma unroll
+      for (int o = 0; o < T; ++o)
+        acc[o] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wa[o]), xf,
+                                                         acc[o], 0, 0, 0);
+      if (s + 1 < ks1) {
+#pragma unroll
+        for (int o = 0; o < T; ++o) wa[o] = w1v[(o * ks1 + s + 1) * 64];
+      }
+    }
+
+    // ---- H1 = BF16(relu(acc + b1)), kept as layer 2's B operand ----
+    u32x4 h1[T][2];
+#pragma unroll
+    for (int o = 0; o < T; ++o) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 bb = *reinterpret_cast<const float4*>(&b1s[32 * o + 8 * q + 4 * g]);
+        const uint32_t lo = static_cast<uint32_t>(to_bf16(fmaxf(acc[o][4 * q + 0] + bb.x, 0.f))) |
+                            (static_cast<uint32_t>(to_bf16(fmaxf(acc[o][4 * q + 1] + bb.y, 0.f))) << 16);
+        const uint32_t hi = static_cast<uint32_t>(to_bf16(fmaxf(acc[o][4 * q + 2] + bb.z, 0.f))) |
+                            (static_cast<uint32_t>(to_bf16(fmaxf(acc[o][4 * q + 3] + bb.w, 0.f))) << 16);
+        h1[o][q >> 1][2 * (q & 1)] = lo;
+        h1[o][q >> 1][2 * (q & 1) + 1] = hi;
+      }
+    }
+
+    // ---- layer 2 (W2 from LDS), two output tiles at a time, each folded
+    // into z as soon as it is done: H2 = BF16(relu(acc + b2)), z = H2 W3 ----
+    floatx16 a3;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) a3[i] = 0.f;
+#pragma unroll
+    for (int op = 0; op < T; op += 2) {
+      floatx16 a2[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) a2[u][i] = 0.f;
+      // W2 fragments one step ahead; the scheduling barrier keeps the
+      // compiler from hoisting every step's LDS reads (register spills)
+      u32x4 wf[2] = {w2s[(op * KS2) * 64 + lane], w2s[((op + 1) * KS2) * 64 + lane]};
+#pragma unroll
+      for (int s2 = 0; s2 < KS2; ++s2) {
+        const bf16x8 hb = __builtin_bit_cast(bf16x8, h1[s2 >> 1][s2 & 1]);
+        u32x4 wc[2] = {wf[0], wf[1]};
+        if (s2 + 1 < KS2) {
+          wf[0] = w2s[(op * KS2 + s2 + 1) * 64 + lane];
+          wf[1] = w2s[((op + 1) * KS2 + s2 + 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          a2[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wc[u]), hb,
+                                                          a2[u], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // H2 of these two tiles in layer 2's output footprint is again a B
+      // operand: the out layer z^T = W3^T H2^T runs as one more MFMA per
+      // 16 features, with W3^T's rows past dim zero
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        u32x4 h2[2];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float4 bb = *reinterpret_cast<const float4*>(&b2s[32 * (op + u) + 8 * q + 4 * g]);
+          h2[q >> 1][2 * (q & 1)] =
+              static_cast<uint32_t>(to_bf16(fmaxf(a2[u][4 * q + 0] + bb.x, 0.f))) |
+              (static_cast<uint32_t>(to_bf16(fmaxf(a2[u][4 * q + 1] + bb.y, 0.f))) << 16);
+          h2[q >> 1][2 * (q & 1) + 1] =
+              static_cast<uint32_t>(to_bf16(fmaxf(a2[u][4 * q + 2] + bb.z, 0.f))) |
+              (static_cast<uint32_t>(to_bf16(fmaxf(a2[u][4 * q + 3] + bb.w, 0.f))) << 16);
+        }
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh)
+          a3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              __builtin_bit_cast(bf16x8, w3s[(2 * (op + u) + hh) * 64 + lane]),
+              __builtin_bit_cast(bf16x8, h2[hh]), a3, 0, 0, 0);
+      }
+    }
+    // lane (g, row) holds z[row][4 g + i] in a3[i], i < 4
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c = 4 * g + i;
+        if (c < dim) z[row * dim + c] = a3[i] + b3[c];
+      }
+    }
+  }
+}
+
+bool fused_eligible(const MlpShape& sh, const float* x) {
+  return (sh.h == 64 || sh.h == 128 || sh.h == 192 || sh.h == 256) && sh.d % 4 == 0 &&
+         sh.dp <= kFusedMaxDp && sh.dim <= kMaxDim &&
+         (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
+}
+
+// Packs a trained model's BF16 weights for the fused kernel.
+void pack_fused(const MlpShape& sh, const uint16_t* wh, DevBuf<uint16_t>& w1p,
+                DevBuf<uint16_t>& w2p, DevBuf<uint16_t>& w3p, hipStream_t s) {
+  const int64_t n1 = static_cast<int64_t>(sh.dp) * sh.h, n2 = static_cast<int64_t>(sh.h) * sh.h;
+  const int64_t n3 = static_cast<int64_t>(sh.h) * 32;
+  w1p.ensure(n1);
+  w2p.ensure(n2);
+  w3p.ensure(n3);
+  hipLaunchKernelGGL(mlp_pack_kernel, dim3(blocks_for(n1 + n2 + n3, 256)), dim3(256), 0, s, wh, sh,
+                     w1p.get(), w2p.get(), w3p.get());
+  KERNEL_CHECK();
+}
+
+void run_fused(const MlpShape& sh, const float* params, const uint16_t* w1p, const uint16_t* w2p,
+               const uint16_t* w3p, const float* mean, const float* inv_std, const float* x,
+               int64_t rows, float* out, hipStream_t s) {
+  if (rows <= 0) return;
+  int dev = 0, cus = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t tiles = (rows + kFusedRowsPerWave - 1) / kFusedRowsPerWave;
+  const unsigned blocks = static_cast<unsigned>(
+      std::max<int64_t>(1, std::min<int64_t>((tiles + kFusedWaves - 1) / kFusedWaves, cus)));
+  const float* b1 = params + sh.off_b1;
+  const float* b2 = params + sh.off_b2;
+  const float* b3 = params + sh.off_b3;
+#define UMAPX_FUSED(HH)                                                                          \
+  hipLaunchKernelGGL(mlp_infer_kernel<HH>, dim3(blocks), dim3(kFusedWaves * 64), 0, s, x, rows, \
+                     sh.d, sh.dp, sh.dim, w1p, w2p, b1, b2, w3p, b3, mean, inv_std, out)
+  switch (sh.h) {
+    case 64: UMAPX_FUSED(64); break;
+    case 128: UMAPX_FUSED(128); break;
+    case 192: UMAPX_FUSED(192); break;
+    case 256: UMAPX_FUSED(256); break;
+    default: throw_invalid("the fused encoder supports mlp_hidden of 64, 128, 192 or 256");
+  }
+#undef UMAPX_FUSED
+  KERNEL_CHECK();
+}
+
+// The unfused chain (the training forward) over row chunks.
+void forward_chain(const MlpShape& sh, const float* params, const uint16_t* wh, const float* mean,
+                   const float* inv_std, const float* x, int64_t rows, float* out, hipStream_t s) {
   constexpr int64_t kChunk = 65536;
   Pass ps;
   ps.ensure(sh, std::min<int64_t>(pad64(rows), kChunk));
@@ -675,15 +967,73 @@ void mlp_forward_rows(const MlpModel& m, const float* x, int64_t rows, float* ou
     const int64_t cnt = std::min(kChunk, rows - r0);
     const int64_t cpad = pad64(cnt);
     hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_for(cpad * sh.dp, 256)), dim3(256), 0, s, x,
-                       sh.d, sh.dp, static_cast<const int32_t*>(nullptr), r0, cnt, m.mean.get(),
-                       m.inv_std.get(), ps.xb.get(), static_cast<const float*>(nullptr), sh.dim,
+                       sh.d, sh.dp, static_cast<const int32_t*>(nullptr), r0, cnt, mean, inv_std,
+                       ps.xb.get(), static_cast<const float*>(nullptr), sh.dim,
                        static_cast<float*>(nullptr));
     KERNEL_CHECK();
     if (cpad > cnt)
       HIP_CHECK(hipMemsetAsync(ps.xb.get() + cnt * sh.dp, 0, sizeof(uint16_t) * (cpad - cnt) * sh.dp, s));
-    forward(sh, m.params.get(), m.wh.get(), ps, cpad, s);
+    forward(sh, params, wh, ps, cpad, s);
     HIP_CHECK(hipMemcpyAsync(out + r0 * sh.dim, ps.z.get(), sizeof(float) * cnt * sh.dim,
                              hipMemcpyDeviceToDevice, s));
+  }
+}
+
+}  // namespace
+
+void pack_model(MlpModel& m, hipStream_t s) {
+  m.packed = false;
+  if (m.shape.h != 64 && m.shape.h != 128 && m.shape.h != 192 && m.shape.h != 256) return;
+  if (m.shape.dp > kFusedMaxDp) return;
+  pack_fused(m.shape, m.wh.get(), m.w1p, m.w2p, m.w3p, s);
+  m.packed = true;
+}
+
+void mlp_forward_rows(const MlpModel& m, const float* x, int64_t rows, float* out, hipStream_t s) {
+  if (!m.trained) throw Error(kNotFitted, "the learned transform has not been trained");
+  const MlpShape& sh = m.shape;
+  if (m.packed && fused_eligible(sh, x)) {
+    run_fused(sh, m.params.get(), m.w1p.get(), m.w2p.get(), m.w3p.get(), m.mean.get(),
+              m.inv_std.get(), x, rows, out, s);
+    return;
+  }
+  forward_chain(sh, m.params.get(), m.wh.get(), m.mean.get(), m.inv_std.get(), x, rows, out, s);
+}
+
+void mlp_forward_host(int d, int h, int dim, const float* params, const float* mean,
+                      const float* inv_std, const float* x, int64_t rows, bool fused, float* z) {
+  check_shape(d, h, dim);
+  if (rows < 1) throw_invalid("rows must be >= 1");
+  const MlpShape sh = MlpShape::make(d, h, dim);
+  hipStream_t s = nullptr;
+  HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{s};
+  DevBuf<float> dp(sh.total), dmean(d), dinv(d), dx(rows * d), dz(rows * dim);
+  DevBuf<uint16_t> dwh(sh.total);
+  HIP_CHECK(hipMemcpyAsync(dp.get(), params, 4 * sh.total, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dmean.get(), mean, 4 * d, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dinv.get(), inv_std, 4 * d, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dx.get(), x, 4 * rows * d, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(sh.total, 256)), dim3(256), 0, s, dp.get(),
+                     sh.total, dwh.get());
+  KERNEL_CHECK();
+  if (fused) {
+    if (!fused_eligible(sh, dx.get()))
+      throw_invalid("the fused encoder needs mlp_hidden in {64, 128, 192, 256}, d % 4 == 0 and d <= 1024");
+    DevBuf<uint16_t> w1p, w2p;
+    DevBuf<uint16_t> w3p;
+    pack_fused(sh, dwh.get(), w1p, w2p, w3p, s);
+    run_fused(sh, dp.get(), w1p.get(), w2p.get(), w3p.get(), dmean.get(), dinv.get(), dx.get(), rows,
+              dz.get(), s);
+    HIP_CHECK(hipMemcpyAsync(z, dz.get(), 4 * rows * dim, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    forward_chain(sh, dp.get(), dwh.get(), dmean.get(), dinv.get(), dx.get(), rows, dz.get(), s);
+    HIP_CHECK(hipMemcpyAsync(z, dz.get(), 4 * rows * dim, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
   }
 }
 

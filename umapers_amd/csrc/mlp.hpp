@@ -31,6 +31,9 @@ struct MlpModel {
   DevBuf<float> params, grads, adam_m, adam_v;
   DevBuf<uint16_t> wh;  // BF16 copies of W1, W2, W3 (same offsets as params)
   DevBuf<float> mean, inv_std;  // input standardisation over the training rows
+  // the BF16 weights packed for the fused inference kernel (see mlp.hip)
+  bool packed = false;
+  DevBuf<uint16_t> w1p, w2p, w3p;
   double last_loss = 0.0, first_loss = 0.0;
   int steps_done = 0;
 };
@@ -38,6 +41,13 @@ struct MlpModel {
 void mlp_train(MlpModel& m, const float* x, int64_t n, int64_t d, const float* y, int dim,
                const Graph& g, float a, float b, const umapx_params& p, hipStream_t s);
 void mlp_forward_rows(const MlpModel& m, const float* x, int64_t rows, float* out, hipStream_t s);
+
+// Forward pass of given parameters over host rows x [rows x d] into host z
+// [rows x dim], for tests: fused = false runs the unfused chain (the
+// training forward), fused = true the fused inference kernel (an error where
+// the fused kernel does not apply).
+void mlp_forward_host(int d, int h, int dim, const float* params, const float* mean,
+                      const float* inv_std, const float* x, int64_t rows, bool fused, float* z);
 
 // One forward / backward pass on a given batch (host arrays), for gradient
 // checks: rows are laid out as B heads, B tails, then B * n_neg negatives
