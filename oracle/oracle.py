@@ -76,6 +76,11 @@ def lib() -> ctypes.CDLL:
             build()
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_find_ab_params.argtypes = [ctypes.c_float, ctypes.c_float, _f32p, _f32p]
+        _u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.orc_xoshiro256pp.argtypes = [_u64p, ctypes.c_int64, _u64p]
+        L.orc_splitmix64.argtypes = [ctypes.c_uint64, ctypes.c_int64, _u64p]
+        L.orc_smallrng_u64.argtypes = [ctypes.c_uint64, ctypes.c_int64, _u64p]
+        L.orc_smallrng_index.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64, _u64p]
         L.orc_knn.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
                               ctypes.c_int32, ctypes.c_int32, _i64p, _f32p]
         L.orc_knn_to_reference.argtypes = [_f32p, ctypes.c_int64, _f32p, ctypes.c_int64,
@@ -183,6 +188,36 @@ def _i64(a) -> np.ndarray:
 
 METRICS = {"euclidean": 0, "cosine": 1, "manhattan": 2, "chebyshev": 3, "minkowski": 4,
            "correlation": 5, "canberra": 6, "braycurtis": 7}
+
+
+def _u64_out(n):
+    out = np.empty(n, np.uint64)
+    return out, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def xoshiro256pp(state, n: int):
+    st = np.ascontiguousarray(state, dtype=np.uint64)
+    out, p = _u64_out(n)
+    lib().orc_xoshiro256pp(st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n, p)
+    return out
+
+
+def splitmix64(seed: int, n: int):
+    out, p = _u64_out(n)
+    lib().orc_splitmix64(seed, n, p)
+    return out
+
+
+def smallrng_u64(seed: int, n: int):
+    out, p = _u64_out(n)
+    lib().orc_smallrng_u64(seed, n, p)
+    return out
+
+
+def smallrng_index(seed: int, range_: int, n: int):
+    out, p = _u64_out(n)
+    lib().orc_smallrng_index(seed, range_, n, p)
+    return out
 
 
 def find_ab_params(spread: float, min_dist: float) -> tuple[float, float]:

@@ -30,9 +30,12 @@
  *   membership / union  closed forms, symmetry, Laplacian KAT      pinned
  *   schedule counts     closed-form per-edge counts                pinned
  *   spectral init       eigh of I + D^-1/2 W D^-1/2 (subspace)     pinned up to sign/rotation
- *   SmallRng streams    xoshiro256++ / SplitMix64 / Lemire restated from their
- *                       published algorithms (rand 0.8.5); no reference output
- *                       exists to check them against          parity unpinned
+ *   SmallRng core       xoshiro256++ and SplitMix64 equal to the published
+ *                       reference vectors; seed_from_u64 = 4 SplitMix64 outputs;
+ *                       gen_range(int) = Lemire's widening multiply, checked
+ *                       on the raw stream                      pinned
+ *   gen_range(float)    the [1, 2) mantissa method restated from rand 0.8.5;
+ *                       no reference output to check it against  parity unpinned
  *   Normal noise        Box-Muller, not rand_distr's ziggurat   parity unpinned
  *   symmetric eigen     cyclic Jacobi, not nalgebra 0.33.3      parity unpinned (signs)
  *   inverse init        small LU solve vs numpy.linalg.solve, fixed rows exact,
@@ -87,18 +90,26 @@ double now_sec() {
  * gen_range over integers is Lemire's widening multiply with the
  * "leading zeros" rejection zone; over floats it is the [1,2) mantissa trick
  * (rand 0.8.5 UniformInt / UniformFloat::sample_single). */
+/* One step of SplitMix64 (Steele, Lea & Flood; Vigna's splitmix64.c). */
+static inline uint64_t splitmix64_next(uint64_t& state) {
+  state += 0x9e3779b97f4a7c15ULL;
+  uint64_t z = state;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
 struct SmallRng {
   uint64_t s[4];
+  SmallRng() : s{0, 0, 0, 0} {}
   explicit SmallRng(uint64_t state) {
-    const uint64_t PHI = 0x9e3779b97f4a7c15ULL;
-    for (int i = 0; i < 4; ++i) {
-      state += PHI;
-      uint64_t z = state;
-      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-      s[i] = z ^ (z >> 31);
-    }
+    for (int i = 0; i < 4; ++i) s[i] = splitmix64_next(state);
     if ((s[0] | s[1] | s[2] | s[3]) == 0) *this = SmallRng(0);
+  }
+  static SmallRng from_state(const uint64_t st[4]) {
+    SmallRng r;
+    for (int i = 0; i < 4; ++i) r.s[i] = st[i];
+    return r;
   }
   static inline uint64_t rotl(uint64_t x, int k) {
     return (x << k) | (x >> (64 - k));
@@ -1680,6 +1691,24 @@ float curve_loss(float a, float b, const vector<float>& xv, const vector<float>&
 }  // namespace
 
 extern "C" {
+
+/* Generator streams for known-answer tests: xoshiro256++ from a raw state,
+ * SplitMix64 from a seed, and SmallRng::seed_from_u64 (the two composed). */
+void orc_xoshiro256pp(const uint64_t* state, int64_t n, uint64_t* out) {
+  SmallRng r = SmallRng::from_state(state);
+  for (int64_t i = 0; i < n; ++i) out[i] = r.next_u64();
+}
+void orc_splitmix64(uint64_t seed, int64_t n, uint64_t* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = splitmix64_next(seed);
+}
+void orc_smallrng_u64(uint64_t seed, int64_t n, uint64_t* out) {
+  SmallRng r(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = r.next_u64();
+}
+void orc_smallrng_index(uint64_t seed, uint64_t range, int64_t n, uint64_t* out) {
+  SmallRng r(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = r.gen_index(range);
+}
 
 void orc_find_ab_params(float spread, float min_dist, float* a_out, float* b_out) {
   const int n = 300;

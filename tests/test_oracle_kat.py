@@ -253,3 +253,47 @@ def test_densmap_zero_lambda_is_the_standard_embedding(oracle):
     dm = oracle.fit(x, oracle.OracleParams(n_epochs=60, random_seed=9, densmap=True,
                                            dens_lambda=2.0)).embedding
     assert not np.array_equal(plain, dm)
+
+
+# The generators behind rand 0.8.5's SmallRng (xoshiro256++, seeded through
+# SplitMix64), against the published reference outputs: xoshiro256++ from the
+# state {1, 2, 3, 4} (Blackman & Vigna's xoshiro256plusplus.c, the vector
+# rand_xoshiro tests against; the first two outputs also follow by hand:
+# rotl(1 + 4, 23) + 1 = 41943041, then 58720359), and SplitMix64 from seed
+# 1234567 (Vigna's splitmix64.c).
+XOSHIRO256PP_1234 = [41943041, 58720359, 3588806011781223, 3591011842654386,
+                     9228616714210784205, 9973669472204895162, 14011001112246962877,
+                     12406186145184390807, 15849039046786891736, 10450023813501588000]
+SPLITMIX64_1234567 = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                      4593380528125082431, 16408922859458223821]
+
+
+def test_xoshiro256pp_reference_vector(oracle):
+    assert [int(v) for v in oracle.xoshiro256pp([1, 2, 3, 4], 10)] == XOSHIRO256PP_1234
+
+
+def test_splitmix64_reference_vector(oracle):
+    assert [int(v) for v in oracle.splitmix64(1234567, 5)] == SPLITMIX64_1234567
+
+
+def test_smallrng_seeding_composes_splitmix_and_xoshiro(oracle):
+    # seed_from_u64: the state is the first four SplitMix64 outputs
+    for seed in (0, 1, 42, 0x7A5B3D91CE42A117):
+        st = oracle.splitmix64(seed, 4)
+        assert np.array_equal(oracle.smallrng_u64(seed, 16), oracle.xoshiro256pp(st, 16))
+
+
+def test_gen_index_is_lemire_widening_multiply(oracle):
+    # rand 0.8.5 UniformInt::sample_single for u64: v * range as a 128-bit
+    # product, accept when its low half is at most (range << lz(range)) - 1,
+    # return the high half; checked on the raw stream in Python integers
+    for seed, rng_ in ((3, 10), (7, 1000003), (11, 1 << 20), (13, 3 * (1 << 40) + 1)):
+        raw = [int(v) for v in oracle.smallrng_u64(seed, 400)]
+        zone = ((rng_ << (64 - rng_.bit_length())) - 1) & ((1 << 64) - 1)
+        want, i = [], 0
+        while len(want) < 100:
+            m = raw[i] * rng_
+            i += 1
+            if (m & ((1 << 64) - 1)) <= zone:
+                want.append(m >> 64)
+        assert [int(v) for v in oracle.smallrng_index(seed, rng_, 100)] == want
