@@ -114,6 +114,50 @@ def test_spectral_init_components(gpu, oracle):
         assert _subspace_cos(yg[m] - yg[m].mean(0), yo[m] - yo[m].mean(0)) > 0.99
 
 
+def test_spectral_init_far_apart_components(gpu, oracle):
+    # twelve blobs far apart (centroid affinities exp(-d^2) = 0), as in the
+    # 32-cluster configuration: the meta layout puts two components on the
+    # axes and packs the other ten within ~1e-4 of the origin, so their local
+    # layouts are scaled into boxes of half-width <= 1e-4.  The GPU stops
+    # iterating those after 16 products (any layout in such a box is within
+    # 2e-4 of any other, the size of the N(0, 1e-4) noise both add) and
+    # iterates the two large ones alone to the end, which must still match the
+    # oracle's 256 iterations as subspaces.
+    rng = np.random.default_rng(9)
+    k_blobs, size = 12, 300
+    centres = rng.normal(0.0, 20.0, size=(k_blobs, 16))
+    xs, hs, ts, ws = [], [], [], []
+    for i, c in enumerate(centres):
+        xb = _stretched_blob(rng, size, 16, c)
+        h, t, w = _graph(oracle, xb)
+        xs.append(xb)
+        hs.append(h + i * size)
+        ts.append(t + i * size)
+        ws.append(w)
+    x = np.concatenate(xs)
+    h, t, w = np.concatenate(hs), np.concatenate(ts), np.concatenate(ws)
+    nc, comp = oracle.connected_components(x.shape[0], h, t)
+    assert nc == k_blobs
+    yo, ok_o = oracle.initialize_embedding(x, 2, h, t, w, "spectral", 42)
+    yg, ok_g = capi.init_embedding(x, 2, h, t, w, init=1, seed=42)
+    assert ok_o and ok_g and np.isfinite(yg).all()
+    means_g = np.array([yg[comp == c].mean(0) for c in range(nc)])
+    means_o = np.array([yo[comp == c].mean(0) for c in range(nc)])
+    flips = [np.array(f) for f in ((1, 1), (1, -1), (-1, 1), (-1, -1))]
+    flip = min(flips, key=lambda f: np.abs(means_g - means_o * f).max())
+    spans = np.array([np.ptp(yo[comp == c], axis=0).max() for c in range(nc)])
+    large = spans > 1.0
+    assert large.sum() == 2 and (spans[~large] < 2e-3).all(), spans
+    for c in range(nc):
+        m = comp == c
+        if large[c]:
+            assert abs(np.ptp(yg[m], axis=0).max() / spans[c] - 1.0) < 0.02
+            assert _subspace_cos(yg[m] - yg[m].mean(0), yo[m] - yo[m].mean(0)) > 0.99
+        else:
+            # anchors equal, local layouts within the box plus both noises
+            assert np.abs(yg[m] - yo[m] * flip).max() <= 2e-3
+
+
 @pytest.mark.parametrize("dim,scales", [(1, (4.0,)), (3, (4.0, 3.2, 2.6))])
 def test_spectral_init_other_dims(gpu, oracle, dim, scales):
     # (dim + 1) * 6 iterated columns (lib.rs:4231-4234) for 1-D and 3-D output:

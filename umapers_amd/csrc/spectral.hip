@@ -331,17 +331,22 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void spmm_kernel(
   }
 }
 
+// Rows of the slots still iterated (active[slot] != 0) only.
 __global__ void long_rows_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
-                                 int32_t* __restrict__ list, int32_t* __restrict__ count) {
+                                 const int32_t* __restrict__ slot_of_pos,
+                                 const int32_t* __restrict__ active, int32_t* __restrict__ list,
+                                 int32_t* __restrict__ count) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p < rows && prow_ptr[p + 1] - prow_ptr[p] > kSeg)
+  if (p < rows && active[slot_of_pos[p]] && prow_ptr[p + 1] - prow_ptr[p] > kSeg)
     list[atomicAdd(count, 1)] = static_cast<int32_t>(p);
 }
 
 __global__ void short_flag_kernel(int64_t rows, const int32_t* __restrict__ prow_ptr,
-                                  int32_t* __restrict__ flag) {
+                                  const int32_t* __restrict__ slot_of_pos,
+                                  const int32_t* __restrict__ active, int32_t* __restrict__ flag) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p < rows) flag[p] = prow_ptr[p + 1] - prow_ptr[p] <= kSeg ? 1 : 0;
+  if (p < rows)
+    flag[p] = (active[slot_of_pos[p]] && prow_ptr[p + 1] - prow_ptr[p] <= kSeg) ? 1 : 0;
 }
 
 __global__ void short_fill_kernel(int64_t rows, const int32_t* __restrict__ flag,
@@ -407,7 +412,9 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 template <int KS>
 __global__ __launch_bounds__(64) void chol_kernel(double* __restrict__ gram,
                                                   double* __restrict__ rinv,
-                                                  int32_t* __restrict__ fail) {
+                                                  int32_t* __restrict__ fail,
+                                                  const int32_t* __restrict__ active) {
+  if (!active[blockIdx.x]) return;  // a frozen slot (block-uniform)
   const int c = threadIdx.x;
   double* g = gram + static_cast<int64_t>(blockIdx.x) * KS * KS;
   double a[KS];
@@ -519,6 +526,17 @@ __global__ __launch_bounds__(256) void project_kernel(const int32_t* __restrict_
   const int32_t prev = __shfl_up(slot, 1);
   if (slot >= 0 && (lane == 0 || prev != slot))
     atomicMax(reinterpret_cast<unsigned int*>(&maxabs[slot]), __float_as_uint(mx));
+}
+
+// Rows of the slots flagged in take[] copied from src to dst.
+template <int KS>
+__global__ void take_rows_kernel(int64_t rows, const int32_t* __restrict__ slot_of_pos,
+                                 const int32_t* __restrict__ take, const float* __restrict__ src,
+                                 float* __restrict__ dst) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= rows || !take[slot_of_pos[p]]) return;
+  const int64_t o = p * KS;
+  for (int b = 0; b < KS; ++b) dst[o + b] = src[o + b];
 }
 
 // scale[slot] = target[slot] / max|coord| of the slot, 1 when that is 0
@@ -1001,13 +1019,22 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     float* yb = ws.y.get();
     float* zb = ws.z.get();
     const int32_t* it_items = ws.items.get();
+    int n_act_items = n_items;
     ws.rmat.ensure(static_cast<size_t>(glen));
+    // slots still iterated: a slot whose Ritz vectors have stopped moving is
+    // frozen (its rows and Ritz vectors kept, its rows left out of the
+    // products); components share no edges, so the others' products are
+    // unchanged
+    std::vector<int32_t> active(n_slots, 1);
+    DevBuf<int32_t> d_active(n_slots), d_take(n_slots);
+    HIP_CHECK(hipMemcpyAsync(d_active.get(), active.data(), 4 * n_slots, hipMemcpyHostToDevice, s));
     // Orthonormalise Z into Y (CholeskyQR): g1 holds Z^T Z on entry and is
     // cleared for the next accumulation.
     auto orth_into_y = [&]() {
-      hipLaunchKernelGGL(chol_kernel<KS>, dim3(n_slots), dim3(64), 0, s, g1, ws.rmat.get(), ws.fail.get());
-      hipLaunchKernelGGL(rotate_kernel<KS>, dim3(n_items), dim3(kItemRows), 0, s, it_items, ws.rmat.get(),
-                         zb, yb);
+      hipLaunchKernelGGL(chol_kernel<KS>, dim3(n_slots), dim3(64), 0, s, g1, ws.rmat.get(),
+                         ws.fail.get(), d_active.get());
+      hipLaunchKernelGGL(rotate_kernel<KS>, dim3(n_act_items), dim3(kItemRows), 0, s, it_items,
+                         ws.rmat.get(), zb, yb);
       KERNEL_CHECK();
     };
     // start block: Z = Y, g1 = Y^T Y
@@ -1015,54 +1042,164 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
     hipLaunchKernelGGL((gram_kernel<false, KS>), dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
     KERNEL_CHECK();
     orth_into_y();
-    // rows of more than kSeg entries (one wave each) and the others in order
+    // rows of more than kSeg entries (one wave each) and the others in order,
+    // of the active slots
     DevBuf<int32_t> long_list(rows), long_count(1), short_flag(rows), short_off(rows),
         short_list(rows);
-    HIP_CHECK(hipMemsetAsync(long_count.get(), 0, 4, s));
-    hipLaunchKernelGGL(long_rows_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
-                       prow_ptr.get(), long_list.get(), long_count.get());
-    hipLaunchKernelGGL(short_flag_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
-                       prow_ptr.get(), short_flag.get());
-    KERNEL_CHECK();
-    const int64_t n_short = exclusive_scan(short_flag.get(), short_off.get(), rows, scan_ws, s);
-    hipLaunchKernelGGL(short_fill_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
-                       short_flag.get(), short_off.get(), short_list.get());
-    KERNEL_CHECK();
     int32_t n_long = 0;
-    HIP_CHECK(hipMemcpyAsync(&n_long, long_count.get(), 4, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    const int64_t spmm_waves = n_long + (n_short + kGroups - 1) / kGroups;
+    int64_t n_short = 0;
+    auto build_lists = [&]() {
+      HIP_CHECK(hipMemsetAsync(long_count.get(), 0, 4, s));
+      hipLaunchKernelGGL(long_rows_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                         prow_ptr.get(), d_slot, d_active.get(), long_list.get(), long_count.get());
+      hipLaunchKernelGGL(short_flag_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                         prow_ptr.get(), d_slot, d_active.get(), short_flag.get());
+      KERNEL_CHECK();
+      n_short = exclusive_scan(short_flag.get(), short_off.get(), rows, scan_ws, s);
+      hipLaunchKernelGGL(short_fill_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                         short_flag.get(), short_off.get(), short_list.get());
+      KERNEL_CHECK();
+      HIP_CHECK(hipMemcpyAsync(&n_long, long_count.get(), 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    };
+    build_lists();
     auto product = [&](const float* src, float* dst) {
+      const int64_t spmm_waves = n_long + (n_short + kGroups - 1) / kGroups;
+      if (spmm_waves == 0) return;
       hipLaunchKernelGGL(spmm_kernel<KS>, dim3(blocks_for(spmm_waves, kSpmmWaves)),
                          dim3(64 * kSpmmWaves), 0, s, static_cast<int64_t>(n_long), n_short,
                          long_list.get(), short_list.get(), prow_ptr.get(), pcol.get(), pnw.get(),
                          src, dst);
       KERNEL_CHECK();
     };
+    std::vector<double> vmat(static_cast<size_t>(n_slots) * KS * dim, 0.0);
+    std::vector<float> anchor(static_cast<size_t>(n_slots) * dim);
+    std::vector<double> ritz(static_cast<size_t>(n_slots) * KS * KS);
+    std::vector<int32_t> fail(n_slots);
+    // Ritz vectors of slot sl from its Ritz matrix Y^T (I + N) Y (upper
+    // triangle in ritz): the eigenvectors of the 2nd .. (dim+1)-th largest
+    // eigenvalues, into out (KS x dim)
+    auto ritz_vectors = [&](int sl, double* out) {
+      std::vector<double> a(ritz.begin() + static_cast<size_t>(sl) * KS * KS,
+                            ritz.begin() + static_cast<size_t>(sl + 1) * KS * KS);
+      for (int i = 0; i < KS; ++i)
+        for (int j = 0; j < i; ++j) a[i * KS + j] = a[j * KS + i];
+      std::vector<double> ev, vec;
+      symmetric_eigen(a, KS, ev, vec);
+      std::vector<int> order(KS);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return ev[q] < ev[p]; });
+      for (int b = 0; b < KS; ++b)
+        for (int k = 0; k < dim; ++k) out[static_cast<size_t>(b) * dim + k] = vec[b * KS + order[k + 1]];
+    };
+    // independent per slot: with many slots they are spread over host threads
+    auto for_slots = [&](const std::vector<int>& list, auto fn) {
+      const int cnt = static_cast<int>(list.size());
+      const int n_threads = std::min(8, cnt / 4);
+      if (n_threads <= 1) {
+        for (int sl : list) fn(sl);
+        return;
+      }
+      std::atomic<int> next{0};
+      std::vector<std::exception_ptr> errors(n_threads);
+      std::vector<std::thread> pool;
+      for (int t = 0; t < n_threads; ++t)
+        pool.emplace_back([&, t] {
+          try {
+            for (int i = next++; i < cnt; i = next++) fn(list[i]);
+          } catch (...) {
+            errors[t] = std::current_exception();
+            next = cnt;  // the other threads stop at their next slot
+          }
+        });
+      for (auto& th : pool) th.join();
+      for (auto& err : errors)
+        if (err) std::rethrow_exception(err);
+    };
+    // Ritz matrices of the active slots (Z = (I + N) Y computed here) into ritz
+    auto ritz_matrices = [&]() {
+      product(yb, zb);
+      hipLaunchKernelGGL((gram_kernel<true, KS>), dim3(n_act_items), dim3(256), 0, s, it_items, yb,
+                         zb, g1);
+      KERNEL_CHECK();
+      HIP_CHECK(hipMemcpyAsync(ritz.data(), g1, 8 * ritz.size(), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipMemcpyAsync(fail.data(), ws.fail.get(), 4 * n_slots, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    };
+    // the frozen and final rows of Y
+    DevBuf<float> yfin(static_cast<size_t>(rows) * KS);
+    std::vector<int32_t> take(n_slots);
+    // Components that the meta layout packs into a box of half-width
+    // range <= kTinyRange (components placed within ~1e-4 of each other,
+    // e.g. 30 of config 3's 32) are scaled into that box, where any layout
+    // differs from any other by at most 2 * range, the size of the N(0, 1e-4)
+    // noise the reference adds to every coordinate (lib.rs:4706-4740).  Their
+    // iteration stops at kTinyIters products: their rows leave the products
+    // (components share no edges, so the other components' iterates are
+    // unchanged) and their Ritz vectors are taken there.
+    constexpr float kTinyRange = 1e-4f;
+    constexpr int kTinyIters = 16;
+    std::vector<int32_t> tiny(n_slots, 0);
+    int n_tiny = 0;
+    if (multi)
+      for (int sl = 0; sl < n_slots; ++sl)
+        if (range[gpu_comps[sl]] <= kTinyRange) {
+          tiny[sl] = 1;
+          ++n_tiny;
+        }
+    // freezes the tiny slots: Ritz vectors into vmat, rows into yfin, the
+    // active lists rebuilt; false when no slot is left to iterate
+    auto freeze_tiny = [&]() -> bool {
+      ritz_matrices();
+      std::vector<int> list;
+      std::fill(take.begin(), take.end(), 0);
+      for (int sl = 0; sl < n_slots; ++sl)
+        if (tiny[sl] && !fail[sl]) {
+          list.push_back(sl);
+          take[sl] = 1;
+          active[sl] = 0;
+        }
+      for_slots(list, [&](int sl) { ritz_vectors(sl, vmat.data() + static_cast<size_t>(sl) * KS * dim); });
+      HIP_CHECK(hipMemsetAsync(g1, 0, sizeof(double) * glen, s));
+      if (list.empty()) return true;
+      HIP_CHECK(hipMemcpyAsync(d_take.get(), take.data(), 4 * n_slots, hipMemcpyHostToDevice, s));
+      HIP_CHECK(hipMemcpyAsync(d_active.get(), active.data(), 4 * n_slots, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(take_rows_kernel<KS>, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                         d_slot, d_take.get(), yb, yfin.get());
+      KERNEL_CHECK();
+      std::vector<int32_t> act_items;
+      for (size_t i = 0; i < items.size(); i += 3)
+        if (active[items[i]]) act_items.insert(act_items.end(), items.begin() + i, items.begin() + i + 3);
+      n_act_items = static_cast<int>(act_items.size() / 3);
+      if (n_act_items > 0)
+        HIP_CHECK(hipMemcpyAsync(ws.items.get(), act_items.data(), 4 * act_items.size(),
+                                 hipMemcpyHostToDevice, s));
+      build_lists();  // synchronises (the host item list is freed on return)
+      return n_act_items > 0;
+    };
     for (int it = 0; it < kIters; ++it) {
       product(yb, zb);
       if ((it + 1) % kOrthEvery == 0 || it + 1 == kIters) {
-        hipLaunchKernelGGL((gram_kernel<false, KS>), dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
+        hipLaunchKernelGGL((gram_kernel<false, KS>), dim3(n_act_items), dim3(256), 0, s, it_items,
+                           yb, zb, g1);
         KERNEL_CHECK();
         orth_into_y();
       } else {
         std::swap(yb, zb);
       }
+      static_assert(kTinyIters % kOrthEvery == 0, "the tiny slots stop at an orthonormalised block");
+      if (n_tiny > 0 && it + 1 == kTinyIters && !freeze_tiny()) break;  // every slot frozen
     }
-    // Ritz matrix Y^T (I + N) Y, upper triangle
-    product(yb, zb);
-    hipLaunchKernelGGL((gram_kernel<true, KS>), dim3(n_items), dim3(256), 0, s, it_items, yb, zb, g1);
-    KERNEL_CHECK();
-    std::vector<double> ritz(static_cast<size_t>(n_slots) * KS * KS);
-    std::vector<int32_t> fail(n_slots);
-    HIP_CHECK(hipMemcpyAsync(ritz.data(), g1, 8 * ritz.size(), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(fail.data(), ws.fail.get(), 4 * n_slots, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    std::vector<double> vmat(static_cast<size_t>(n_slots) * KS * dim);
-    std::vector<float> anchor(static_cast<size_t>(n_slots) * dim);
-    std::vector<int> ok_slots;
+    // the Ritz vectors of the slots iterated to the end (lib.rs:4267-4300)
+    std::vector<int> ok_slots, last;
+    if (n_act_items > 0) ritz_matrices();
     for (int sl = 0; sl < n_slots; ++sl) {
       const int64_t c = gpu_comps[sl];
+      if (!active[sl]) {
+        ok_slots.push_back(sl);  // frozen: its Ritz vectors are in vmat
+        for (int k = 0; k < dim; ++k) anchor[sl * dim + k] = layout[c * dim + k];
+        continue;
+      }
       if (fail[sl]) {
         // lib.rs:4339-4348: a failed subspace iteration falls back to the
         // dense Laplacian only up to 4096 rows
@@ -1091,46 +1228,20 @@ bool spectral_init_device(const float* x, int64_t n, int64_t d, int dim, const G
         continue;
       }
       ok_slots.push_back(sl);
-    }
-    // The Ritz problems of the converged slots are independent: with many
-    // components they are spread over a few host threads (each slot's result
-    // is the same whichever thread solves it).
-    auto ritz_slot = [&](int sl) {
-      const int64_t c = gpu_comps[sl];
-      std::vector<double> a(ritz.begin() + static_cast<size_t>(sl) * KS * KS,
-                            ritz.begin() + static_cast<size_t>(sl + 1) * KS * KS);
-      for (int i = 0; i < KS; ++i)
-        for (int j = 0; j < i; ++j) a[i * KS + j] = a[j * KS + i];
-      std::vector<double> ev, vec;
-      symmetric_eigen(a, KS, ev, vec);
-      std::vector<int> order(KS);
-      std::iota(order.begin(), order.end(), 0);
-      std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return ev[q] < ev[p]; });
-      for (int b = 0; b < KS; ++b)
-        for (int k = 0; k < dim; ++k) vmat[(static_cast<size_t>(sl) * KS + b) * dim + k] = vec[b * KS + order[k + 1]];
+      last.push_back(sl);
       for (int k = 0; k < dim; ++k) anchor[sl * dim + k] = layout[c * dim + k];
-    };
-    const int n_ok = static_cast<int>(ok_slots.size());
-    const int n_threads = std::min(8, n_ok / 4);
-    if (n_threads <= 1) {
-      for (int sl : ok_slots) ritz_slot(sl);
-    } else {
-      std::atomic<int> next{0};
-      std::vector<std::exception_ptr> errors(n_threads);
-      std::vector<std::thread> pool;
-      for (int t = 0; t < n_threads; ++t)
-        pool.emplace_back([&, t] {
-          try {
-            for (int i = next++; i < n_ok; i = next++) ritz_slot(ok_slots[i]);
-          } catch (...) {
-            errors[t] = std::current_exception();
-            next = n_ok;  // the other threads stop at their next slot
-          }
-        });
-      for (auto& th : pool) th.join();
-      for (auto& err : errors)
-        if (err) std::rethrow_exception(err);
     }
+    for_slots(last, [&](int sl) { ritz_vectors(sl, vmat.data() + static_cast<size_t>(sl) * KS * dim); });
+    if (!last.empty()) {
+      std::fill(take.begin(), take.end(), 0);
+      for (int sl : last) take[sl] = 1;
+      HIP_CHECK(hipMemcpyAsync(d_take.get(), take.data(), 4 * n_slots, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(take_rows_kernel<KS>, dim3(blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                         d_slot, d_take.get(), yb, yfin.get());
+      KERNEL_CHECK();
+    }
+    yb = yfin.get();
+    const int n_ok = static_cast<int>(ok_slots.size());
     rep.n_iterative += n_ok;
     ws.vmat.ensure(vmat.size());
     ws.maxabs.ensure(n_slots);
