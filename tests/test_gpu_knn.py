@@ -103,6 +103,48 @@ def test_knn_to_reference_many_queries(gpu, oracle, rng):
         assert np.array_equal(dist, od)
 
 
+# 17 .. 32 neighbours: four top-16 lists per query (two interleaved splits of
+# the reference rows, two halves each), with the queries whose lists may have
+# dropped one of their 32 best scores redone by the 32-entry kernel.
+K32_SHAPES = [(2000, 32, 17), (1500, 100, 20), (3000, 64, 24), (2500, 128, 32), (40, 8, 30), (20, 5, 17)]
+
+
+@pytest.mark.parametrize("n,d,k", K32_SHAPES)
+@pytest.mark.parametrize("metric", [0, 1])
+def test_17_to_32_neighbours_bit_exact(gpu, oracle, rng, n, d, k, metric):
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    idx, dist = capi.knn(x, k, metric)
+    oi, od = oracle.knn(x, k, "cosine" if metric else "euclidean")
+    assert np.array_equal(idx, oi) and np.array_equal(dist, od)
+
+
+def test_17_to_32_neighbours_sorted_and_duplicated_rows(gpu, oracle):
+    # rows sorted by cluster (a query's neighbours sit together in the row
+    # order), and 40 copies of each of 60 rows (every query has 39 ties at
+    # distance 0, so every list is full of its 32 best: all redone)
+    x, lab = datasets.gaussian_mixture(4000, 48, 20, 3.0, 3, (0.5, 1.5), True)
+    x = np.ascontiguousarray(x[np.argsort(lab, kind="stable")])
+    for xs in (x, np.repeat(np.random.default_rng(8).normal(size=(60, 24)).astype(np.float32), 40, axis=0)):
+        for metric in (0, 1):
+            idx, dist = capi.knn(xs, 30, metric)
+            oi, od = oracle.knn(xs, 30, "cosine" if metric else "euclidean")
+            assert np.array_equal(idx, oi) and np.array_equal(dist, od)
+        # BF16 candidates with the exact re-rank
+        idx, dist = capi.knn(xs, 30, 0, 1, 2)
+        oi, od = oracle.knn(xs, 30)
+        assert recall_at_k(idx, oi) >= 0.99
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_17_to_32_neighbours_bf16_recall(gpu, oracle, metric):
+    x = datasets.gaussian_mixture(5000, 64, 8, 3.0, 5, (0.5, 1.5))
+    idx, dist = capi.knn(x, 30, metric, 1, 2)
+    oi, od = oracle.knn(x, 30, "cosine" if metric else "euclidean")
+    assert recall_at_k(idx, oi) >= 0.99
+    same = idx == oi
+    assert np.array_equal(dist[same].view(np.int32), od[same].view(np.int32))
+
+
 # Rows wider than 256 features take the GEMM score-block path with a
 # selection kernel; the re-rank makes FP32 results equal to the oracle's.
 WIDE = [(700, 300, 15), (1100, 784, 15), (400, 520, 30), (257, 257, 5), (20, 300, 5)]

@@ -254,7 +254,8 @@ template <typename T, int DPAD, int H>
 __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_candidates_kernel(
     const T* __restrict__ xq, int64_t m, const T* __restrict__ xr,
     const float* __restrict__ nr_half, int64_t n, int64_t full_groups, int tail_split,
-    int64_t tail_len, int32_t* __restrict__ cand_idx, uint2* __restrict__ cand_pairs) {
+    int64_t tail_len, int interleave, int share, int32_t* __restrict__ cand_idx,
+    uint2* __restrict__ cand_pairs) {
   using C = CandShape<T, DPAD, H>;
   constexpr int WAVES = C::kWaves;
   constexpr int BUF = C::kBuf;
@@ -278,9 +279,18 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   const int64_t group = split_block ? full_groups + tail_block / tail_split : blockIdx.x;
   const int64_t q0 = group * C::kQueries + wave * 32;
   const bool live = q0 + l32 < m;
-  const int64_t r0 = split_block ? static_cast<int64_t>(split) * tail_len : 0;
-  const int64_t r1 = split_block ? (r0 + tail_len < n ? r0 + tail_len : n) : n;
-  const int n_tiles = static_cast<int>((r1 - r0 + kTile - 1) / kTile);
+  // a split takes the reference rows [r0, r1) (tail_len rows each), or with
+  // interleave > 1 every interleave-th 32-row tile from tile `split` on
+  const bool inter = split_block && interleave > 1;
+  const int64_t r0 = split_block && !inter ? static_cast<int64_t>(split) * tail_len : 0;
+  const int64_t r1 = split_block && !inter ? (r0 + tail_len < n ? r0 + tail_len : n) : n;
+  const int64_t all_tiles = (n + kTile - 1) / kTile;
+  const int n_tiles = inter ? static_cast<int>((all_tiles - split + interleave - 1) / interleave)
+                            : static_cast<int>((r1 - r0 + kTile - 1) / kTile);
+  auto tile_row = [&](int t) -> int64_t {
+    return inter ? (static_cast<int64_t>(t) * interleave + split) * kTile
+                 : r0 + static_cast<int64_t>(t) * kTile;
+  };
   uint2* wbuf = reinterpret_cast<uint2*>(lds + C::kTiles + (C::kSuper == 1 ? C::kFlagBytes : 0)) +
                 wave * 64 * BUF + lane;
 
@@ -300,7 +310,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
     float nr;
   };
   auto fetch = [&](int tile, Pre& p) {
-    const int64_t base = r0 + static_cast<int64_t>(tile) * kTile;
+    const int64_t base = tile_row(tile);
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i) {
       const int c = tid + i * kThreads;
@@ -392,6 +402,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
     }
     cnt = 0;
     if constexpr (!decltype(with_union)::value) return;
+    if (!share) return;  // each half keeps its own top H (kernel-uniform)
     // The two halves of a query only need the H best of their union: a
     // score below the H-th best of both lists can never make that set.  That
     // H-th best is the max over i + j = H of min(i-th best here, j-th best
@@ -436,7 +447,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
         // the barrier that closed the previous iteration
         stage(tile + 2, pre);
         fetch(tile + 3, pre);
-        filter(acc, static_cast<int32_t>(r0 + static_cast<int64_t>(tile) * kTile + 4 * half));
+        filter(acc, static_cast<int32_t>(tile_row(tile) + 4 * half));
         const bool need = __ballot(cnt > BUF - 16) != 0ULL;
         if (lane == 0) s_flag[(tile & 1) * 8 + wave] = need ? 1 : 0;
         __syncthreads();
@@ -473,7 +484,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
           __syncthreads();
           if (tile + 1 < n_tiles) load(tile + 1, cur);
         }
-        filter(acc, static_cast<int32_t>(r0 + static_cast<int64_t>(tile) * kTile + 4 * half));
+        filter(acc, static_cast<int32_t>(tile_row(tile) + 4 * half));
         need = __ballot(cnt > BUF - 16) != 0ULL;
       }
       if (need) flush(std::true_type{});
@@ -518,6 +529,60 @@ __global__ void knn_merge_splits_kernel(const uint2* __restrict__ pairs, int64_t
   int32_t* dst = cand_idx + q * (2 * H) + half * H;
 #pragma unroll
   for (int j = 0; j < H; ++j) dst[j] = top.c[j];
+}
+
+// 17 .. 32 neighbours from the 16-entry lists: each query's reference rows
+// are split two ways by interleaved 32-row tiles and each split's two halves
+// keep their own top 16 (no shared threshold), so every query gets four
+// top-16 lists of disjoint row sets, 64 candidates in all.  They hold the
+// query's 32 best scores unless some list was full with its 16th best at or
+// above the 32nd best of the union (it may have dropped one of them); such
+// queries are listed for the 32-entry kernel.  One thread per query.
+__global__ void knn_merge_check_kernel(const uint2* __restrict__ pairs, int64_t m,
+                                       int32_t* __restrict__ cand, int32_t* __restrict__ redo,
+                                       int32_t* __restrict__ redo_count) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const uint2* src = pairs + q * 64;
+  LaneTop<32> top;
+  top.init();
+  for (int j = 0; j < 64; ++j) {
+    const uint2 e = src[j];
+    const int32_t c = static_cast<int32_t>(e.y);
+    cand[q * 64 + j] = c;
+    const float x = __uint_as_float(e.x);
+    if (c >= 0 && x > top.v[31]) top.insert(x, c);
+  }
+  const float s32 = top.v[31];  // -inf with fewer than 32 entries in all
+  bool safe = true;
+  for (int l = 0; l < 4; ++l) {
+    const uint2 last = src[l * 16 + 15];
+    if (static_cast<int32_t>(last.y) >= 0 && !(__uint_as_float(last.x) < s32)) safe = false;
+  }
+  if (!safe) redo[atomicAdd(redo_count, 1)] = static_cast<int32_t>(q);
+}
+
+// rows[i] = src[list[i]] for rows of `width` elements of T (16-byte vectors)
+template <typename T>
+__global__ void gather_rows16_kernel(const T* __restrict__ src, const int32_t* __restrict__ list,
+                                     int64_t count, int width, T* __restrict__ dst) {
+  const int vecs = width * static_cast<int>(sizeof(T)) / 16;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count * vecs) return;
+  const int64_t i = t / vecs;
+  const int v = static_cast<int>(t % vecs);
+  reinterpret_cast<u32x4*>(dst + i * width)[v] =
+      reinterpret_cast<const u32x4*>(src + static_cast<int64_t>(list[i]) * width)[v];
+}
+
+// cand[list[i]] = sub[i], 64 candidates a row
+__global__ void scatter_cand64_kernel(const int32_t* __restrict__ sub,
+                                      const int32_t* __restrict__ list, int64_t count,
+                                      int32_t* __restrict__ cand) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count * 64) return;
+  const int64_t i = t >> 6;
+  cand[static_cast<int64_t>(list[i]) * 64 + (t & 63)] = sub[t];
 }
 
 // ------------------------------------------------------------- re-rank ----
@@ -715,7 +780,7 @@ void launch_candidates(const T* xq, int64_t m, const T* xr, const float* nr_half
   }
   const int64_t blocks = full + (groups - full) * tail_split;
   hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(blocks)), dim3(C::kWaves * 64), C::kBytes, s,
-                     xq, m, xr, nr_half, n, full, tail_split, tail_len, cand, pp);
+                     xq, m, xr, nr_half, n, full, tail_split, tail_len, 0, 1, cand, pp);
   KERNEL_CHECK();
   if (tail_rows > 0) {
     hipLaunchKernelGGL(knn_merge_splits_kernel<H>,
@@ -723,6 +788,49 @@ void launch_candidates(const T* xq, int64_t m, const T* xr, const float* nr_half
                        pp, full * C::kQueries, m, tail_split, cand);
     KERNEL_CHECK();
   }
+}
+
+// 64 candidates a query for 17 .. 32 neighbours from the 16-entry kernel,
+// which runs two waves per SIMD where the 32-entry lists hold it to one: the
+// reference rows are split two ways by interleaved tiles (so neighbours that
+// sit together in the row order spread over both splits), every query gets
+// four top-16 lists of disjoint rows, and the queries whose lists may have
+// dropped one of their 32 best scores (knn_merge_check_kernel; a list holds
+// a quarter of them on average) are redone with the 32-entry kernel.  The 64
+// then hold the 32 best scores, as the two top-32 lists of the 32-entry
+// kernel do.
+template <typename T, int DPAD>
+void launch_candidates_k32(const T* xq, int64_t m, const T* xr, const float* nr_half, int64_t n,
+                           int32_t* cand, DevBuf<uint2>& pairs, hipStream_t s) {
+  using C = CandShape<T, DPAD, 16>;
+  auto kern = knn_candidates_kernel<T, DPAD, 16>;
+  HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, C::kBytes));
+  const int64_t groups = (m + C::kQueries - 1) / C::kQueries;
+  pairs.ensure(static_cast<size_t>(m) * 64);
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups * 2)), dim3(C::kWaves * 64), C::kBytes,
+                     s, xq, m, xr, nr_half, n, int64_t{0}, 2, int64_t{0}, 2, 0, cand, pairs.get());
+  KERNEL_CHECK();
+  DevBuf<int32_t> redo(m + 1);
+  HIP_CHECK(hipMemsetAsync(redo.get() + m, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(knn_merge_check_kernel, dim3(blocks_for(static_cast<size_t>(m), 256)), dim3(256),
+                     0, s, pairs.get(), m, cand, redo.get(), redo.get() + m);
+  KERNEL_CHECK();
+  int32_t f = 0;
+  HIP_CHECK(hipMemcpyAsync(&f, redo.get() + m, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (f == 0) return;
+  DevBuf<T> qsub(static_cast<size_t>(f) * DPAD);
+  DevBuf<int32_t> csub(static_cast<size_t>(f) * 64);
+  DevBuf<uint2> pairs2;
+  const int vecs = DPAD * static_cast<int>(sizeof(T)) / 16;
+  hipLaunchKernelGGL(gather_rows16_kernel<T>, dim3(blocks_for(static_cast<size_t>(f) * vecs, 256)),
+                     dim3(256), 0, s, xq, redo.get(), static_cast<int64_t>(f), DPAD, qsub.get());
+  KERNEL_CHECK();
+  launch_candidates<T, DPAD, 32>(qsub.get(), f, xr, nr_half, n, csub.get(), pairs2, s);
+  hipLaunchKernelGGL(scatter_cand64_kernel, dim3(blocks_for(static_cast<size_t>(f) * 64, 256)),
+                     dim3(256), 0, s, csub.get(), redo.get(), static_cast<int64_t>(f), cand);
+  KERNEL_CHECK();
 }
 
 // Query rows per block of the candidate kernel for one shape.
@@ -752,6 +860,19 @@ void dispatch_dpad(int dpad, const T* xq, int64_t m, const T* xr, const float* n
     case 128: launch_candidates<T, 128, H>(xq, m, xr, nr_half, n, cand, pairs, s); break;
     case 192: launch_candidates<T, 192, H>(xq, m, xr, nr_half, n, cand, pairs, s); break;
     case 256: launch_candidates<T, 256, H>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    default: throw_invalid("unsupported padded feature count");
+  }
+}
+
+template <typename T>
+void dispatch_dpad_k32(int dpad, const T* xq, int64_t m, const T* xr, const float* nr_half,
+                       int64_t n, int32_t* cand, DevBuf<uint2>& pairs, hipStream_t s) {
+  switch (dpad) {
+    case 32: launch_candidates_k32<T, 32>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 64: launch_candidates_k32<T, 64>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 128: launch_candidates_k32<T, 128>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 192: launch_candidates_k32<T, 192>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 256: launch_candidates_k32<T, 256>(xq, m, xr, nr_half, n, cand, pairs, s); break;
     default: throw_invalid("unsupported padded feature count");
   }
 }
@@ -871,6 +992,17 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
   if (wide) {
     knn_wide_candidates(qp, m, ws.rprep.get(), ws.rnorm2.get(), r, n, d, dpad, bf16, kc,
                         ws.cand.get(), ws, s);
+  } else if (kc == 64 && k > 16 && !ws.pairs.borrowed) {
+    // 17 .. 32 neighbours through the 16-entry lists (the allocation-free
+    // entry point, whose buffers are borrowed, keeps the 32-entry kernel)
+    if (bf16)
+      dispatch_dpad_k32<uint16_t>(dpad, static_cast<const uint16_t*>(qp), m,
+                                  reinterpret_cast<const uint16_t*>(ws.rprep.get()),
+                                  ws.rnorm2.get(), n, ws.cand.get(), ws.pairs, s);
+    else
+      dispatch_dpad_k32<float>(dpad, static_cast<const float*>(qp), m,
+                               reinterpret_cast<const float*>(ws.rprep.get()), ws.rnorm2.get(),
+                               n, ws.cand.get(), ws.pairs, s);
   } else if (bf16) {
     if (kc == 32)
       dispatch_dpad<uint16_t, 16>(dpad, static_cast<const uint16_t*>(qp), m,
