@@ -78,16 +78,20 @@ __global__ void column_sums_normalized_kernel(const float* __restrict__ x, int64
 }
 
 // Writes the padded, centred (and, for cosine, normalised) compute copy of
-// each row and, when half_norm is given, -||row||^2 / 2 of the rounded row.
+// each row and, when half_norm is given, -||row||^2 / 2 of the rounded row,
+// with -inf for the rows [n, norm_rows) that pad the last reference tile.
 // One wave per row.
 template <typename T>
-__global__ void prep_rows_kernel(const float* __restrict__ x, int64_t n, int d, int dpad,
-                                 int normalize, const double* __restrict__ colsum,
+__global__ void prep_rows_kernel(const float* __restrict__ x, int64_t n, int64_t norm_rows, int d,
+                                 int dpad, int normalize, const double* __restrict__ colsum,
                                  double inv_n, T* __restrict__ out,
                                  float* __restrict__ half_norm) {
   const int lane = threadIdx.x & 63;
   const int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (r >= n) return;
+  if (r >= n) {
+    if (r < norm_rows && lane == 0 && half_norm) half_norm[r] = -INFINITY;
+    return;
+  }
   float inv = 1.0f;
   if (normalize) {
     float ss = 0.f;
@@ -195,7 +199,21 @@ struct LaneTop {
 };
 
 constexpr int kLdsBudget = 160 * 1024;
+
+// Direct global-to-LDS loads (the wave's LDS address is uniform; lane i writes
+// its bytes at dst + i * size).
+__device__ __forceinline__ void lds_load16(const void* src, unsigned char* dst) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+__device__ __forceinline__ void lds_load4(const void* src, unsigned char* dst) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 4, 0, 0);
+}
+// Waits for this wave's outstanding vector-memory loads (the direct loads).
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 constexpr int kTile = 32;   // reference rows per staged tile (one MFMA subtile)
+
+// Entries of the half-norm array: whole tiles, the padding -inf.
+int64_t knn_norm_rows(int64_t n) { return (n + kTile - 1) / kTile * kTile; }
 
 // LDS of one block: a ring of reference tiles with their half norms and the
 // waves' pass buffers ([slot][lane] pairs of 8 bytes).  Eight waves (two per
@@ -204,19 +222,32 @@ constexpr int kTile = 32;   // reference rows per staged tile (one MFMA subtile)
 // list leave room, else four.  The next tile's fragments are read into
 // registers ahead of use when they are at most eight 16-byte vectors per
 // lane.  kSuper tiles share one block-wide barrier: the ring holds two groups
-// of kSuper tiles (one being multiplied, the next being staged), the largest
-// of 4 and 2 that fits; when neither fits, the ring holds three tiles with a
+// of kSuper tiles (one being multiplied, the next arriving), the largest of 4
+// and 2 that fits; when neither fits, the ring holds three tiles with a
 // barrier per tile and block-synchronous drains (kSuper = 1).
+//
+// With kSuper > 1 the tiles are loaded straight from global memory into LDS
+// (global_load_lds_dwordx4: each lane of a wave writes 16 bytes, the wave one
+// contiguous kilobyte), a whole group ahead, so no registers hold tiles in
+// flight and the load of group g + 1 has all of group g's multiplies to
+// arrive.  Rows are then unpadded; row r's 16-byte chunk c sits at chunk
+// c ^ swz(r), so the 32 lanes that read one chunk of 32 rows at once hit
+// distinct banks.  The three-tile ring keeps register staging and rows
+// padded by 16 bytes.
 template <typename T, int DPAD, int H>
 struct CandShape {
   static constexpr int kNV = DPAD / 2 / (16 / static_cast<int>(sizeof(T)));
   static constexpr bool kPrefetch = kNV <= 8;
-  static constexpr int kRowBytes = DPAD * static_cast<int>(sizeof(T)) + 16;
-  static constexpr int kStageBytes = kTile * kRowBytes + kTile * 4;
+  static constexpr int kChunks = DPAD * static_cast<int>(sizeof(T)) / 16;  // per row
   static constexpr int kFlagBytes = 16;
+  static constexpr int row_bytes(bool direct) {
+    return DPAD * static_cast<int>(sizeof(T)) + (direct ? 0 : 16);
+  }
+  static constexpr int stage_bytes(bool direct) { return kTile * row_bytes(direct) + kTile * 4; }
   static constexpr int ring(int super) { return super == 1 ? 3 : 2 * super; }
   static constexpr int bytes(int waves, int super, int buf) {
-    return ring(super) * kStageBytes + (super == 1 ? kFlagBytes : 0) + waves * 64 * buf * 8;
+    return ring(super) * stage_bytes(super > 1) + (super == 1 ? kFlagBytes : 0) +
+           waves * 64 * buf * 8;
   }
   static constexpr int pick_buf(int waves, int super) {
     for (int b : {32, 28, 24, 20})
@@ -226,11 +257,22 @@ struct CandShape {
   static constexpr bool kRegsFit8 = H <= 16 && (kPrefetch || 4 * kNV + 2 * H <= 120);
   static constexpr int kWaves = (pick_buf(8, 1) && kRegsFit8) ? 8 : 4;
   static constexpr int kSuper = pick_buf(kWaves, 4) ? 4 : (pick_buf(kWaves, 2) ? 2 : 1);
+  static constexpr bool kDirect = kSuper > 1;
+  static constexpr int kRowBytes = row_bytes(kDirect);
+  static constexpr int kStageBytes = stage_bytes(kDirect);
   static constexpr int kRing = ring(kSuper);
   static constexpr int kTiles = kRing * kStageBytes;
   static constexpr int kBuf = pick_buf(kWaves, kSuper);
   static_assert(kBuf >= 20, "kNN candidate tiles do not fit in LDS");
   static constexpr int kBytes = bytes(kWaves, kSuper, kBuf);
+  // direct loads: kilobyte loads per tile, and the chunk swizzle
+  // swz(r) = (r ^ (r >> kSwzShift)) & kSwzMask, kSwzMask + 1 being the largest
+  // power of two up to 16 that divides kChunks (the swizzle stays in the row)
+  static constexpr int kLoads = kTile * kChunks / 64;
+  static constexpr int kSwzMask = (kChunks % 16 == 0 ? 16 : kChunks % 8 == 0 ? 8 : 4) - 1;
+  static constexpr int kSwzShift = kSwzMask == 15 ? 4 : kSwzMask == 7 ? 3 : 2;
+  static_assert(!kDirect || (kTile * kChunks) % 64 == 0, "direct tile loads are whole kilobytes");
+  static_assert(!kDirect || kChunks % 4 == 0, "direct tile rows are whole 64-byte lines");
   static constexpr int kQueries = kWaves * 32;  // queries per block
 };
 
@@ -270,7 +312,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int l32 = lane & 31;
   const int half = lane >> 5;
   const bool split_block = static_cast<int64_t>(blockIdx.x) >= full_groups;
@@ -287,10 +329,10 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   const int64_t all_tiles = (n + kTile - 1) / kTile;
   const int n_tiles = inter ? static_cast<int>((all_tiles - split + interleave - 1) / interleave)
                             : static_cast<int>((r1 - r0 + kTile - 1) / kTile);
-  auto tile_row = [&](int t) -> int64_t {
-    return inter ? (static_cast<int64_t>(t) * interleave + split) * kTile
-                 : r0 + static_cast<int64_t>(t) * kTile;
-  };
+  // first reference row of tile t: row0 + t * row_step (block-uniform)
+  const int64_t row0 = inter ? static_cast<int64_t>(split) * kTile : r0;
+  const int64_t row_step = inter ? static_cast<int64_t>(interleave) * kTile : kTile;
+  auto tile_row = [&](int t) -> int64_t { return row0 + static_cast<int64_t>(t) * row_step; };
   uint2* wbuf = reinterpret_cast<uint2*>(lds + C::kTiles + (C::kSuper == 1 ? C::kFlagBytes : 0)) +
                 wave * 64 * BUF + lane;
 
@@ -332,6 +374,27 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
     }
     if (tid < kTile) reinterpret_cast<float*>(t + kTile * C::kRowBytes)[tid] = p.nr;
   };
+  // this wave's share of tile `tile`'s direct loads: kilobyte j of its rows
+  // goes to wave j % WAVES, the half norms (padded with -inf to whole tiles
+  // by the caller) to wave kLoads % WAVES; rows past n repeat row n - 1,
+  // whose scores the -inf norms discard
+  auto issue = [&](int tile) {
+    if (tile >= n_tiles) return;
+    unsigned char* st = s_tiles + (tile % C::kRing) * C::kStageBytes;
+    const int64_t base = tile_row(tile);
+    for (int j = wave; j < C::kLoads; j += WAVES) {
+      const int pos = j * 64 + lane;  // chunk position in the tile's rows
+      const int r = pos / C::kChunks;
+      const int c = (pos % C::kChunks) ^ ((r ^ (r >> C::kSwzShift)) & C::kSwzMask);
+      const int64_t row = base + r < n ? base + r : n - 1;
+      lds_load16(xr + row * DPAD + c * VEC, st + j * 1024);
+    }
+    if (wave == C::kLoads % WAVES && lane < kTile)
+      lds_load4(nr_half + base + lane, st + kTile * C::kRowBytes);
+  };
+  // byte offset of this lane's fragment v (chunk half * NV + v of row l32)
+  const int swz = C::kDirect ? ((l32 ^ (l32 >> C::kSwzShift)) & C::kSwzMask) : 0;
+  auto frag_off = [&](int v) { return l32 * C::kRowBytes + ((half * NV + v) ^ swz) * 16; };
   // fragments (first MFMA operand) and accumulator seeds of one tile
   struct Frags {
     u32x4 rf[kPrefetch ? NV : 1];
@@ -340,9 +403,8 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   auto load = [&](int tile, Frags& f) {
     const unsigned char* st = s_tiles + (tile % C::kRing) * C::kStageBytes;
     if constexpr (kPrefetch) {
-      const unsigned char* rrow = st + l32 * C::kRowBytes + half * HALF * sizeof(T);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) f.rf[v] = *reinterpret_cast<const u32x4*>(rrow + v * 16);
+      for (int v = 0; v < NV; ++v) f.rf[v] = *reinterpret_cast<const u32x4*>(st + frag_off(v));
     }
     const float* tn = reinterpret_cast<const float*>(st + kTile * C::kRowBytes) + 4 * half;
 #pragma unroll
@@ -360,12 +422,11 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
 #pragma unroll
       for (int v = 0; v < NV; ++v) acc = MfmaSteps<T, MfmaOp<T>::kSteps>::run(f.rf[v], qf[v], acc);
     } else {
-      const unsigned char* rrow = s_tiles + (tile % C::kRing) * C::kStageBytes + l32 * C::kRowBytes +
-                                  half * HALF * sizeof(T);
+      const unsigned char* st = s_tiles + (tile % C::kRing) * C::kStageBytes;
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        acc = MfmaSteps<T, MfmaOp<T>::kSteps>::run(*reinterpret_cast<const u32x4*>(rrow + v * 16),
-                                                   qf[v], acc);
+        acc = MfmaSteps<T, MfmaOp<T>::kSteps>::run(
+            *reinterpret_cast<const u32x4*>(st + frag_off(v)), qf[v], acc);
     }
     return acc;
   };
@@ -459,31 +520,31 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
     }
   } else {
     constexpr int S = C::kSuper;
+    // group 0 now, group 1 while group 0 is multiplied
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
-      fetch(t, pre);
-      stage(t, pre);
-    }
-    fetch(S, pre);
+    for (int t = 0; t < S; ++t) issue(t);
+    wait_vmem();
     __syncthreads();
+#pragma unroll
+    for (int t = S; t < 2 * S; ++t) issue(t);
     load(0, cur);
     while (tile < n_tiles) {
       bool need = false;
       for (; tile < n_tiles && !need; ++tile) {
         const floatx16 acc = mma(tile, cur);
         __builtin_amdgcn_sched_barrier(0);
-        const bool last = tile % S == S - 1;
-        // within a group the next tile is already visible; across groups it
-        // becomes visible at the barrier below
-        if (!last && tile + 1 < n_tiles) load(tile + 1, cur);
-        // slot (tile + S) % 2S last held tile - S, whose fragments were all
-        // read before the barrier that closed the previous group
-        stage(tile + S, pre);
-        fetch(tile + S + 1, pre);
-        if (last) {
+        if (tile % S == S - 1) {
+          // the next group has landed (this wave's loads; the barrier
+          // publishes every wave's), and every wave has read this group's
+          // slots, which now take the group after the next
+          wait_vmem();
           __syncthreads();
-          if (tile + 1 < n_tiles) load(tile + 1, cur);
+#pragma unroll
+          for (int t = 1; t <= S; ++t) issue(tile + S + t);
         }
+        // within a group the next tile is already visible; across groups it
+        // became visible at the barrier above
+        if (tile + 1 < n_tiles) load(tile + 1, cur);
         filter(acc, static_cast<int32_t>(tile_row(tile) + 4 * half));
         need = __ballot(cnt > BUF - 16) != 0ULL;
       }
@@ -910,7 +971,7 @@ KnnScratchNeed knn_scratch_need(int64_t m, int64_t n, int64_t d, int k, int metr
   const TailPlan tp = plan_tail(m, n, queries, device_cus());
   need.colsum = static_cast<size_t>(d);
   need.rprep = static_cast<size_t>(n) * dpad * esz;
-  need.rnorm2 = static_cast<size_t>(n);
+  need.rnorm2 = static_cast<size_t>(knn_norm_rows(n));
   need.qprep = self ? 0 : static_cast<size_t>(m) * dpad * esz;
   need.cand = static_cast<size_t>(m) * kc;
   need.pairs = tp.tail_rows > 0 ? static_cast<size_t>(tp.tail_rows) * tp.tail_split * 2 * h : 0;
@@ -966,18 +1027,19 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
   const double inv_n = 1.0 / static_cast<double>(n);
 
   ws.rprep.ensure(static_cast<size_t>(n) * dpad * esz);
-  ws.rnorm2.ensure(static_cast<size_t>(n));
+  ws.rnorm2.ensure(static_cast<size_t>(knn_norm_rows(n)));
   if (!self) ws.qprep.ensure(static_cast<size_t>(m) * dpad * esz);
   auto prep = [&](const float* x, int64_t rows, void* out, float* nrm) {
-    const unsigned blocks = blocks_for(static_cast<size_t>(rows), 4);
+    const int64_t norm_rows = nrm ? knn_norm_rows(rows) : rows;
+    const unsigned blocks = blocks_for(static_cast<size_t>(norm_rows), 4);
     if (bf16)
       hipLaunchKernelGGL(prep_rows_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s, x, rows,
-                         static_cast<int>(d), dpad, metric == 1 ? 1 : 0, ws.colsum.get(), inv_n,
-                         static_cast<uint16_t*>(out), nrm);
+                         norm_rows, static_cast<int>(d), dpad, metric == 1 ? 1 : 0,
+                         ws.colsum.get(), inv_n, static_cast<uint16_t*>(out), nrm);
     else
       hipLaunchKernelGGL(prep_rows_kernel<float>, dim3(blocks), dim3(256), 0, s, x, rows,
-                         static_cast<int>(d), dpad, metric == 1 ? 1 : 0, ws.colsum.get(), inv_n,
-                         static_cast<float*>(out), nrm);
+                         norm_rows, static_cast<int>(d), dpad, metric == 1 ? 1 : 0,
+                         ws.colsum.get(), inv_n, static_cast<float*>(out), nrm);
     KERNEL_CHECK();
   };
   prep(r, n, ws.rprep.get(), ws.rnorm2.get());
