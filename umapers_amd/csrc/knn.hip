@@ -250,7 +250,7 @@ struct CandShape {
            waves * 64 * buf * 8;
   }
   static constexpr int pick_buf(int waves, int super) {
-    for (int b : {32, 28, 24, 20})
+    for (int b = 32; b >= 20; --b)
       if (bytes(waves, super, b) <= kLdsBudget) return b;
     return 0;
   }
