@@ -170,6 +170,36 @@ def test_wide_rows_bf16_recall(gpu, oracle, metric):
     assert np.array_equal(dist[same].view(np.int32), od[same].view(np.int32))
 
 
+# BF16 rows of 257 .. 896 features run the candidate kernel itself (query
+# fragments in registers, one-tile groups in LDS); 900 features, and 32-entry
+# lists past 384, stay on the score blocks.
+WIDE_BF16 = [(1100, 784, 15), (400, 520, 30), (300, 380, 24), (257, 257, 5), (20, 300, 5), (1500, 900, 15),
+             (600, 700, 20)]
+
+
+@pytest.mark.parametrize("n,d,k", WIDE_BF16)
+@pytest.mark.parametrize("metric", [0, 1])
+def test_wide_rows_bf16_candidates(gpu, oracle, n, d, k, metric):
+    x = datasets.gaussian_mixture(n, d, 8, 3.0, 7, (0.5, 1.5))
+    idx, dist = capi.knn(x, k, metric, 1, 2)
+    oi, od = oracle.knn(x, k, "cosine" if metric else "euclidean")
+    assert recall_at_k(idx, oi) >= 0.99
+    same = idx == oi
+    assert np.array_equal(dist[same].view(np.int32), od[same].view(np.int32))
+
+
+def test_wide_rows_bf16_duplicates(gpu, oracle, rng):
+    # 300 copies of one 784-feature row: identical rows score identically in
+    # the candidate kernel, so the ties resolve by index as the oracle's do
+    base = rng.normal(size=(1, 784)).astype(np.float32)
+    x = np.concatenate([np.repeat(base, 300, axis=0),
+                        rng.normal(size=(400, 784)).astype(np.float32)])
+    idx, dist = capi.knn(x, 10, 0, 1, 2)
+    oi, od = oracle.knn(x, 10)
+    assert np.array_equal(idx[:300], oi[:300]) and np.array_equal(dist[:300], od[:300])
+    assert recall_at_k(idx[300:], oi[300:]) >= 0.99
+
+
 def test_wide_rows_many_duplicates(gpu, oracle, rng):
     # 1500 copies of one row: their score rows hold more than a thousand
     # equal values at the top, which the selection resolves by bisection

@@ -224,16 +224,17 @@ int64_t knn_norm_rows(int64_t n) { return (n + kTile - 1) / kTile * kTile; }
 // lane.  kSuper tiles share one block-wide barrier: the ring holds two groups
 // of kSuper tiles (one being multiplied, the next arriving), the largest of 4
 // and 2 that fits; when neither fits, the ring holds three tiles with a
-// barrier per tile and block-synchronous drains (kSuper = 1).
+// barrier per tile and block-synchronous drains (register staging, below),
+// and for rows too wide for that (past 256 features) two one-tile groups.
 //
-// With kSuper > 1 the tiles are loaded straight from global memory into LDS
+// The groups of kSuper tiles are loaded straight from global memory into LDS
 // (global_load_lds_dwordx4: each lane of a wave writes 16 bytes, the wave one
 // contiguous kilobyte), a whole group ahead, so no registers hold tiles in
 // flight and the load of group g + 1 has all of group g's multiplies to
 // arrive.  Rows are then unpadded; row r's 16-byte chunk c sits at chunk
 // c ^ swz(r), so the 32 lanes that read one chunk of 32 rows at once hit
 // distinct banks.  The three-tile ring keeps register staging and rows
-// padded by 16 bytes.
+// padded by 16 bytes (kDirect false).
 template <typename T, int DPAD, int H>
 struct CandShape {
   static constexpr int kNV = DPAD / 2 / (16 / static_cast<int>(sizeof(T)));
@@ -244,27 +245,28 @@ struct CandShape {
     return DPAD * static_cast<int>(sizeof(T)) + (direct ? 0 : 16);
   }
   static constexpr int stage_bytes(bool direct) { return kTile * row_bytes(direct) + kTile * 4; }
-  static constexpr int ring(int super) { return super == 1 ? 3 : 2 * super; }
-  static constexpr int bytes(int waves, int super, int buf) {
-    return ring(super) * stage_bytes(super > 1) + (super == 1 ? kFlagBytes : 0) +
+  static constexpr int ring(int super, bool direct) { return direct ? 2 * super : 3; }
+  static constexpr int bytes(int waves, int super, bool direct, int buf) {
+    return ring(super, direct) * stage_bytes(direct) + (direct ? 0 : kFlagBytes) +
            waves * 64 * buf * 8;
   }
-  static constexpr int pick_buf(int waves, int super) {
+  static constexpr int pick_buf(int waves, int super, bool direct) {
     for (int b = 32; b >= 20; --b)
-      if (bytes(waves, super, b) <= kLdsBudget) return b;
+      if (bytes(waves, super, direct, b) <= kLdsBudget) return b;
     return 0;
   }
   static constexpr bool kRegsFit8 = H <= 16 && (kPrefetch || 4 * kNV + 2 * H <= 120);
-  static constexpr int kWaves = (pick_buf(8, 1) && kRegsFit8) ? 8 : 4;
-  static constexpr int kSuper = pick_buf(kWaves, 4) ? 4 : (pick_buf(kWaves, 2) ? 2 : 1);
-  static constexpr bool kDirect = kSuper > 1;
+  static constexpr int kWaves = (pick_buf(8, 1, false) && kRegsFit8) ? 8 : 4;
+  static constexpr int kSuper =
+      pick_buf(kWaves, 4, true) ? 4 : (pick_buf(kWaves, 2, true) ? 2 : 1);
+  static constexpr bool kDirect = kSuper > 1 || !pick_buf(kWaves, 1, false);
   static constexpr int kRowBytes = row_bytes(kDirect);
   static constexpr int kStageBytes = stage_bytes(kDirect);
-  static constexpr int kRing = ring(kSuper);
+  static constexpr int kRing = ring(kSuper, kDirect);
   static constexpr int kTiles = kRing * kStageBytes;
-  static constexpr int kBuf = pick_buf(kWaves, kSuper);
+  static constexpr int kBuf = pick_buf(kWaves, kSuper, kDirect);
   static_assert(kBuf >= 20, "kNN candidate tiles do not fit in LDS");
-  static constexpr int kBytes = bytes(kWaves, kSuper, kBuf);
+  static constexpr int kBytes = bytes(kWaves, kSuper, kDirect, kBuf);
   // direct loads: kilobyte loads per tile, and the chunk swizzle
   // swz(r) = (r ^ (r >> kSwzShift)) & kSwzMask, kSwzMask + 1 being the largest
   // power of two up to 16 that divides kChunks (the swizzle stays in the row)
@@ -286,7 +288,7 @@ struct CandShape {
 // the fragments of tile t + 1 are read from LDS into the registers tile t's
 // MFMAs have just consumed (when they fit) before tile t's scores are
 // filtered.  A wave whose buffers could not take another tile drains them
-// between tiles on its own.  With kSuper = 1 the ring holds three tiles
+// between tiles on its own.  With register staging the ring holds three tiles
 // (tile t + 2 staged during tile t) and a wave that needs a drain raises a
 // flag; after the per-tile barrier every wave of the block drains, so no wave
 // idles at the barrier while another drains.  Either way the drain sits
@@ -308,7 +310,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   constexpr bool kPrefetch = C::kPrefetch;
   extern __shared__ __align__(16) unsigned char lds[];
   unsigned char* s_tiles = lds;
-  uint8_t* s_flag = lds + C::kTiles;  // kSuper == 1 only
+  uint8_t* s_flag = lds + C::kTiles;  // register staging only
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -333,7 +335,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   const int64_t row0 = inter ? static_cast<int64_t>(split) * kTile : r0;
   const int64_t row_step = inter ? static_cast<int64_t>(interleave) * kTile : kTile;
   auto tile_row = [&](int t) -> int64_t { return row0 + static_cast<int64_t>(t) * row_step; };
-  uint2* wbuf = reinterpret_cast<uint2*>(lds + C::kTiles + (C::kSuper == 1 ? C::kFlagBytes : 0)) +
+  uint2* wbuf = reinterpret_cast<uint2*>(lds + C::kTiles + (C::kDirect ? 0 : C::kFlagBytes)) +
                 wave * 64 * BUF + lane;
 
   u32x4 qf[NV];
@@ -488,7 +490,7 @@ __global__ __launch_bounds__((CandShape<T, DPAD, H>::kWaves * 64)) void knn_cand
   Pre pre;
   Frags cur;
   int tile = 0;
-  if constexpr (C::kSuper == 1) {
+  if constexpr (!C::kDirect) {
     fetch(0, pre);
     stage(0, pre);
     fetch(1, pre);
@@ -938,6 +940,40 @@ void dispatch_dpad_k32(int dpad, const T* xq, int64_t m, const T* xr, const floa
   }
 }
 
+// BF16 rows of 257 .. 896 features: the candidate kernel with the queries'
+// fragments held in registers (four waves, one per SIMD, so up to 512 of
+// them), two one-tile groups of the reference in LDS.  The padded feature
+// count rounds up to a multiple of 128 from 384: rows of whole 256-byte
+// lines keep the 16-way chunk swizzle (a 200000 x 784 pass takes 130 ms at
+// 896 features against 138 ms at 832; narrower shapes run eight waves and
+// would spill); 0 past 896 (score blocks instead).
+int wide_candidate_pad(int64_t d) {
+  return d <= 384 ? 384 : (d <= 896 ? static_cast<int>((d + 127) / 128 * 128) : 0);
+}
+
+template <int H, int DPAD>
+void launch_wide(const uint16_t* xq, int64_t m, const uint16_t* xr, const float* nr_half,
+                 int64_t n, int32_t* cand, DevBuf<uint2>& pairs, hipStream_t s) {
+  if constexpr (H == 32 && DPAD > 448)
+    throw_invalid("32-entry lists take the score blocks past 448 features");
+  else
+    launch_candidates<uint16_t, DPAD, H>(xq, m, xr, nr_half, n, cand, pairs, s);
+}
+
+template <int H>
+void dispatch_wide(int dpad, const uint16_t* xq, int64_t m, const uint16_t* xr,
+                   const float* nr_half, int64_t n, int32_t* cand, DevBuf<uint2>& pairs,
+                   hipStream_t s) {
+  switch (dpad) {
+    case 384: launch_wide<H, 384>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 512: launch_wide<H, 512>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 640: launch_wide<H, 640>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 768: launch_wide<H, 768>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    case 896: launch_wide<H, 896>(xq, m, xr, nr_half, n, cand, pairs, s); break;
+    default: throw_invalid("unsupported padded feature count");
+  }
+}
+
 }  // namespace
 
 int knn_padded_dim(int64_t d) {
@@ -993,13 +1029,18 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
     return;
   }
   int dpad = knn_padded_dim(d);
-  const bool wide = dpad < 0;  // past 256 features: GEMM score blocks + selection
+  const bool wide = dpad < 0;  // past 256 features
+  const int kc = knn_candidate_count(k, rerank_factor);
+  // wide BF16 rows up to 896 features: the candidate kernel; otherwise GEMM
+  // score blocks + selection
+  // (32-entry lists only up to 448: past that they spill registers)
+  const int wide_try = wide && precision == 1 && (kc == 32 || kc == 64) ? wide_candidate_pad(d) : 0;
+  const int wide_pad = kc == 64 && wide_try > 448 ? 0 : wide_try;
   if (wide) {
     // the cosine column sums keep one f64 per feature in LDS
     if (d > 16384) throw_invalid("exact GPU kNN supports at most 16384 features");
-    dpad = static_cast<int>((d + 63) / 64 * 64);
+    dpad = wide_pad > 0 ? wide_pad : static_cast<int>((d + 63) / 64 * 64);
   }
-  const int kc = knn_candidate_count(k, rerank_factor);
   if (kc < 0) throw_invalid("n_neighbors * rerank_factor must be <= 64 for n_neighbors <= 32");
   if (k > n) throw_invalid("n_neighbors must not exceed the number of reference rows");
   const bool self = (q == r) && (m == n);
@@ -1051,7 +1092,16 @@ void knn_device(const float* q, int64_t m, const float* r, int64_t n, int64_t d,
 
   ws.cand.ensure(static_cast<size_t>(m) * kc);
   if (stats) HIP_CHECK(hipEventRecord(stats->cand_start, s));
-  if (wide) {
+  if (wide_pad > 0) {
+    if (kc == 32)
+      dispatch_wide<16>(dpad, static_cast<const uint16_t*>(qp), m,
+                        reinterpret_cast<const uint16_t*>(ws.rprep.get()), ws.rnorm2.get(), n,
+                        ws.cand.get(), ws.pairs, s);
+    else
+      dispatch_wide<32>(dpad, static_cast<const uint16_t*>(qp), m,
+                        reinterpret_cast<const uint16_t*>(ws.rprep.get()), ws.rnorm2.get(), n,
+                        ws.cand.get(), ws.pairs, s);
+  } else if (wide) {
     knn_wide_candidates(qp, m, ws.rprep.get(), ws.rnorm2.get(), r, n, d, dpad, bf16, kc,
                         ws.cand.get(), ws, s);
   } else if (kc == 64 && k > 16 && !ws.pairs.borrowed) {
