@@ -504,7 +504,7 @@ def main() -> int:
     log(f"bench: inverse transform {m} rows in {np.median(ti) * 1e3:.1f} ms, "
         f"rmse {inverse['rmse_vs_query_rows']} (rows' own rms {inverse['query_rms_about_mean']})")
 
-    # ---- rows wider than 256 features: GEMM score blocks + selection ----
+    # ---- rows wider than 256 features (BF16: the candidate kernel) ----
     wide = None
     if args.wide_rows > 0:
         from umapers_amd import Umap
@@ -512,7 +512,7 @@ def main() -> int:
         log(f"bench: wide-row leg ({n_w} x {d_w}, BF16 + re-rank fit)")
         xw = datasets.gaussian_mixture(n_w, d_w, 10, 3.0, 1, (0.5, 1.5))
         mw = Umap(n_epochs=200, random_seed=1, ann_mode="exact", knn_precision="bf16_rerank")
-        mw.profile_fit_transform(xw[:5000])  # loads the GEMM library's kernels
+        mw.profile_fit_transform(xw[:5000])  # warm-up
         pw = mw.profile_fit_transform(xw)
         tw = pw["timings"]
         wide = {"rows": n_w, "features": d_w, "knn_sec": round(tw["knn_sec"], 4),
