@@ -256,7 +256,10 @@ struct CandShape {
     return 0;
   }
   static constexpr bool kRegsFit8 = H <= 16 && (kPrefetch || 4 * kNV + 2 * H <= 120);
-  static constexpr int kWaves = (pick_buf(8, 1, false) && kRegsFit8) ? 8 : 4;
+  // fragments of at most four 16-byte vectors a lane leave room for three
+  // waves per SIMD (at most 168 registers each) when the direct ring fits
+  static constexpr bool kFit12 = H <= 16 && kNV <= 4 && bytes(12, 4, true, 20) <= kLdsBudget;
+  static constexpr int kWaves = kFit12 ? 12 : ((pick_buf(8, 1, false) && kRegsFit8) ? 8 : 4);
   static constexpr int kSuper =
       pick_buf(kWaves, 4, true) ? 4 : (pick_buf(kWaves, 2, true) ? 2 : 1);
   static constexpr bool kDirect = kSuper > 1 || !pick_buf(kWaves, 1, false);
