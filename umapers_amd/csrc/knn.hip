@@ -216,10 +216,12 @@ constexpr int kTile = 32;   // reference rows per staged tile (one MFMA subtile)
 int64_t knn_norm_rows(int64_t n) { return (n + kTile - 1) / kTile * kTile; }
 
 // LDS of one block: a ring of reference tiles with their half norms and the
-// waves' pass buffers ([slot][lane] pairs of 8 bytes).  Eight waves (two per
+// waves' pass buffers ([slot][lane] pairs of 8 bytes).  Twelve waves (three
+// per SIMD, at most 168 registers each) when the query fragments are at most
+// four 16-byte vectors a lane and a direct ring fits; else eight (two per
 // SIMD, at most 256 registers each) when the buffers can stay at least 20
 // deep in the three-tile ring and the resident query fragments plus the top
-// list leave room, else four.  The next tile's fragments are read into
+// list leave room; else four.  The next tile's fragments are read into
 // registers ahead of use when they are at most eight 16-byte vectors per
 // lane.  kSuper tiles share one block-wide barrier: the ring holds two groups
 // of kSuper tiles (one being multiplied, the next arriving), the largest of 4
