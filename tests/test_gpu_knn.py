@@ -200,6 +200,18 @@ def test_wide_rows_bf16_duplicates(gpu, oracle, rng):
     assert recall_at_k(idx[300:], oi[300:]) >= 0.99
 
 
+def test_wide_rows_bf16_to_reference(gpu, oracle):
+    # separate query rows (the transform's kNN) on the wide candidate path
+    r = datasets.gaussian_mixture(1200, 600, 8, 3.0, 3, (0.5, 1.5))
+    q = datasets.mixture_queries(150, 600, 8, 3.0, 3, (0.5, 1.5), 4)
+    for metric in (0, 1):
+        idx, dist = capi.knn_to_reference(q, r, 15, metric, 1)
+        oi, od = oracle.knn_to_reference(q, r, 15, "cosine" if metric else "euclidean")
+        assert recall_at_k(idx, oi) >= 0.99
+        same = idx == oi
+        assert np.array_equal(dist[same].view(np.int32), od[same].view(np.int32))
+
+
 def test_wide_rows_many_duplicates(gpu, oracle, rng):
     # 1500 copies of one row: their score rows hold more than a thousand
     # equal values at the top, which the selection resolves by bisection
